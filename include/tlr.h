@@ -1,0 +1,136 @@
+/*
+ * libtlr: MI355X-native temporal light field reconstruction (C ABI).
+ *
+ * Drop-in for the reconstruction slot of the reference's TreeGather
+ * (src/reconstruction_lib/reconstruction/Reconstruction.hpp:42-48): build the
+ * sample tree from a UVTSampleBuffer, then reconstruct an RGBA image.  Plain
+ * pointers and sizes only; the C++ facade in
+ * temporal-lightfield-reconstruction_amd/csrc/facade mirrors the reference
+ * classes on top of it.
+ *
+ * Threading: a context owns one HIP stream and its device buffers; use one
+ * context per host thread.  Calls on one context are serialized by the
+ * caller.  Every call returns a tlr_status; tlr_last_error() describes the
+ * last failure (the reference aborts through FW::fail at the same sites).
+ */
+#ifndef TLR_H
+#define TLR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tlr_status {
+    TLR_OK = 0,
+    TLR_E_INVALID = 1,          /* bad argument or non-finite sample data               */
+    TLR_E_IO = 2,               /* sample file missing/unreadable (SampleBuffer.cpp:233) */
+    TLR_E_ALL_DISCARDED = 3,    /* every sample discarded (Reconstruction.cpp:587,636)   */
+    TLR_E_UNSUPPORTED_SPP = 4,  /* spp rounds above 256 (ReconstructionOur.cpp:66)       */
+    TLR_E_IMAGE_TOO_SMALL = 5,  /* output smaller than the buffer (Reconstruction.cpp:223)*/
+    TLR_E_HIP = 6,              /* HIP runtime error                                     */
+    TLR_E_CAPACITY = 7,         /* a query exceeded the per-query scratch capacity       */
+    TLR_E_STATE = 8             /* call order violated (e.g. reconstruct before build)   */
+} tlr_status;
+
+typedef struct tlr_ctx tlr_ctx;
+
+/* Sample-buffer header: UVTSampleBuffer width/height/spp/CoC coefficients
+ * (SampleBuffer.hpp:138-183, v1.3 header SampleBuffer.cpp:244-268). */
+typedef struct tlr_header {
+    int   width, height, spp;
+    float coc_c0, coc_c1;      /* CoC radius = C0/w + C1 (CameraParams.hpp:72) */
+} tlr_header;
+
+/* SoA view of the per-sample fields the path reads, indexed (y*W+x)*spp+i
+ * (UVTSampleBuffer getters SampleBuffer.hpp:147-162).  Colour alpha is 1. */
+typedef struct tlr_soa {
+    const float *x, *y, *w, *u, *v, *t, *r, *g, *b, *mv_x, *mv_y, *mv_w;
+} tlr_soa;
+
+typedef struct tlr_recon_params {
+    int   out_spp;             /* output samples per pixel (reference: 128)        */
+    int   override_uvt;        /* constant (u,v,t): CameraParams::overrideUVT       */
+    float uvt[3];
+    int   lens_gaussian;       /* CameraParams::lensFilter == LENS_GAUSSIAN         */
+    int   time_gaussian;       /* CameraParams::timeFilter == TIME_GAUSSIAN         */
+    int   collect_stats;       /* fill the Stats counters (slower)                  */
+    int   gate;                /* 0: plain method; 1: learned gate (tlr_gate_*)     */
+    int   reserved[4];
+} tlr_recon_params;
+
+/* TreeGather::Stats (Reconstruction.hpp:134-147) as sums over output samples,
+ * plus device timings of the last build/reconstruct (milliseconds). */
+typedef struct tlr_stats {
+    double leaf_nodes, samples_in_leaf_nodes, samples_within_1r, samples_fetched;
+    double surfaces, num_2r, surfaces_merged, at_least_one, count, overflow;
+    double build_ms, query_ms, combine_ms, total_ms;
+    double num_leaves, num_samples;
+} tlr_stats;
+
+typedef struct tlr_build_info {
+    int   num_input, num_kept, num_bucketed;
+    int   spp_pow2;
+    float coc_adj[2];
+    float bbmin[2], bbmax[2];
+    float scale;
+    int   bbmin_int[2];
+    int   grid_w, grid_h;
+    int   max_hits;
+    int   num_initial_leaves;
+    int   num_leaves, num_valid_leaves;
+} tlr_build_info;
+
+/* Context lifetime.  device: HIP device ordinal. */
+tlr_status  tlr_create(tlr_ctx** out, int device);
+void        tlr_destroy(tlr_ctx* ctx);
+const char* tlr_last_error(const tlr_ctx* ctx);
+
+/* UVTSampleBuffer(const char*) (SampleBuffer.cpp:230-387): v1.3 file, text or
+ * binary, header inline or in "<file>.header".  Fills the header and returns
+ * records (16 floats each, alpha forced to 1) in a malloc'd buffer that the
+ * caller frees with tlr_free(). */
+tlr_status  tlr_load_sample_file(const char* path, tlr_header* h, float** records, int64_t* n);
+/* UVTSampleBuffer::serialize (SampleBuffer.cpp:404-564), v1.3. */
+tlr_status  tlr_save_sample_file(const char* path, const tlr_header* h, const float* records, int binary,
+                                 int separate_header);
+void        tlr_free(void* p);
+
+/* Input upload.  AoS records (16 floats, v1.3 order) from host memory, or an
+ * SoA view of host (on_device=0) or device (on_device=1) arrays.  The buffers
+ * are borrowed for the duration of the call. */
+tlr_status  tlr_set_samples(tlr_ctx* ctx, const tlr_header* h, const float* records, int64_t n);
+tlr_status  tlr_set_samples_soa(tlr_ctx* ctx, const tlr_header* h, const tlr_soa* soa, int on_device);
+
+/* TreeGather ctor (Reconstruction.cpp:43-215): spp rounding, CoC adjust,
+ * reprojection, bucketing, leaves and hierarchy, on the device. */
+tlr_status  tlr_build(tlr_ctx* ctx, float aperture_adjust, float focal_adjust);
+
+/* reconstructDofMotion (Reconstruction.cpp:221-295): rgba_out receives
+ * width*height*4 floats (row-major, the buffer's y order); on_device selects
+ * a device or host destination.  stats may be NULL. */
+tlr_status  tlr_reconstruct(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba_out, int out_on_device,
+                            tlr_stats* stats);
+
+/* Blocks until the context's stream is idle. */
+tlr_status  tlr_sync(tlr_ctx* ctx);
+
+/* ---- Introspection for parity checks (no reference counterpart) -------- */
+tlr_status  tlr_get_build_info(tlr_ctx* ctx, tlr_build_info* out);
+/* reprojected x,y,w,t per input sample (w = 0 if discarded) */
+tlr_status  tlr_copy_reprojected(tlr_ctx* ctx, float* x, float* y, float* w, float* t);
+/* tree sample order: input index of each sample (num_bucketed) */
+tlr_status  tlr_copy_layout(tlr_ctx* ctx, int32_t* src);
+/* initial leaves: x0,y0,x1,y1,offset,count (num_initial_leaves, row-major owner order) */
+tlr_status  tlr_copy_initial_leaves(tlr_ctx* ctx, int32_t* out6);
+/* leaves: s0,s1 and 14 bound floats (12 plane coefficients, tspan) */
+tlr_status  tlr_copy_leaves(tlr_ctx* ctx, int32_t* s01, float* bounds14);
+/* per output sample colour of the last reconstruct, (y,x,i) order, 4 floats */
+tlr_status  tlr_copy_query_colors(tlr_ctx* ctx, float* out4);
+/* output pattern: 64*M templates x,y,u,v,t (host computation) */
+int         tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, float* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,155 @@
+/*
+ * CPU oracle for the temporal light field reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C++17 restatement of the
+ * reference CPU path (TreeGather build + Filterer::reconstruct) used as the
+ * correctness checker for the HIP path and as the timed CPU baseline.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product library never links or calls it.
+ *
+ * Every stage cites the reference file:line it restates (paths relative to the
+ * reference repository root).  Arithmetic is IEEE FP32 with no contraction
+ * (built with -ffp-contract=off, no fast-math), in the reference's operation
+ * order; the triangle edge functions are evaluated in FP64 as the reference
+ * does.
+ *
+ * Where the reference is ill-defined or hierarchy-dependent the oracle fixes a
+ * rule, and the HIP path follows the same rule:
+ *   - query-side leaf order: sorted by (key, leaf id) (the reference's
+ *     unstable FW::sort tie order depends on its traversal order);
+ *   - leaves whose time span is empty are never gathered;
+ *   - box values of samples with an empty time span are zero (the reference
+ *     reads stale scratch contents there);
+ *   - the hierarchy above the leaves is a Morton LBVH (the reference's is an
+ *     agglomerative tree; the leaf set a query finds does not depend on it).
+ *
+ * Parity status of each function is listed in the header comment of
+ * oracle.cpp.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+/* Records: N x 16 floats in the v1.3 order
+ * x,y,z/w,w,u,v,t,r,g,b,a,mv_x,mv_y,mv_w,dwdx,dwdy, indexed (y*W+x)*spp+i. */
+orc_ctx* orc_create(int W, int H, int spp, float C0, float C1, const float* records);
+void     orc_destroy(orc_ctx* c);
+const char* orc_error(const orc_ctx* c);
+
+/* Build: spp rounding, CoC adjust, reprojection, grid, leaves, LBVH.
+ * threads: worker threads for the leaf stage (<=0: hardware concurrency).
+ * Returns 0 on success, else a TLR_E_* style code (see include/tlr.h). */
+int orc_build(orc_ctx* c, float aperture_adjust, float focal_adjust, int threads);
+
+/* Scalars of the build. */
+typedef struct orc_build_info {
+    int   num_input, num_kept, num_bucketed;
+    int   spp_pow2;
+    float coc_adj[2];
+    float bbmin[2], bbmax[2];       /* reprojected xy bbox (unscaled) */
+    float scale;
+    int   bbmin_int[2];
+    int   grid_w, grid_h;
+    int   max_hits;
+    int   num_initial_nodes, num_initial_leaves;
+    int   num_leaves, num_internal;
+    int   root;                     /* LBVH root id (internal id, or leaf id | 0x80000000) */
+} orc_build_info;
+void orc_get_build_info(const orc_ctx* c, orc_build_info* out);
+
+/* Reprojected samples in input order (only the kept ones), 10 floats each:
+ * x,y,w,t,mvx,mvy,mvw,r,g,b ; src[k] = input record index. */
+void orc_get_reprojected(const orc_ctx* c, float* out10, int32_t* src);
+/* Cell id (row-major in the grid) of each kept sample, -1 if discarded. */
+void orc_get_cells(const orc_ctx* c, int32_t* cell);
+/* Initial leaves in DFS order: x0,y0,x1,y1,offset,count. */
+void orc_get_initial_leaves(const orc_ctx* c, int32_t* out6);
+/* Layout (tree sample order): input record index of each sample. */
+void orc_get_layout(const orc_ctx* c, int32_t* src);
+/* Leaves in layout order: s0,s1 and 14 floats (12 plane coeffs, tspan). */
+void orc_get_leaves(const orc_ctx* c, int32_t* s01, float* bounds14);
+/* LBVH: sorted leaf order (leaf ids), internal nodes: child0, child1 (leaf ids
+ * tagged with 0x80000000) and 14 floats of bounds. */
+void orc_get_lbvh(const orc_ctx* c, int32_t* leaf_order, int32_t* children2, float* bounds14);
+
+/* Reconstruction parameters. */
+typedef struct orc_recon_params {
+    int   out_spp;          /* output samples per pixel (reference default 128) */
+    int   override_uvt;     /* 0/1: constant (u,v,t) for every output sample   */
+    float uvt[3];
+    int   lens_gaussian;    /* 0: box lens filter (default), 1: Gaussian        */
+    int   time_gaussian;    /* 0: box time filter (default), 1: Gaussian        */
+    int   threads;          /* <=0: hardware concurrency                        */
+    int   row_stride;       /* reconstruct every k-th scanline (1 = all)        */
+} orc_recon_params;
+
+/* Stats counters (sum over output samples), the reference's Stats layout:
+ * numLeafNodes, numSamplesInLeafNodes, numSamplesWithin1R, numSamplesFetched,
+ * numSurfaces, num2R, numSurfacesMerged, numAtLeastOne; plus count. */
+typedef struct orc_stats {
+    double leaf_nodes, samples_in_leaf_nodes, samples_within_1r, samples_fetched;
+    double surfaces, num_2r, surfaces_merged, at_least_one;
+    double count;
+} orc_stats;
+
+/* Output pattern: 64 * spp' templates, 5 floats each (x,y,u,v,t) + key. */
+int orc_output_pattern(int out_spp, int override_uvt, const float* uvt, float* out6);
+
+/* Full reconstruction: image W*H*4 floats (row-major, y down as in the
+ * sample buffer).  query_colors (optional): per output sample result colour,
+ * W*H*out_spp*4 floats in (y, x, i) order. */
+int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image,
+                    float* query_colors, orc_stats* stats);
+
+/* Debug gather for one output sample at radius R: leaf ids (sorted order),
+ * gathered sample layout indices and their surface ids.  Returns the number
+ * of gathered samples; *num_leaves receives the leaf count.  Buffers may be
+ * NULL to query sizes. */
+int orc_gather(orc_ctx* c, const float* o5 /* x,y,u,v,t */, float R,
+               int32_t* leaves, int max_leaves, int* num_leaves,
+               int32_t* samples, int32_t* surface_of, int max_samples);
+
+/* Per-query decision record for one output sample (pixel x,y, index i):
+ * fetch2R, number of surfaces, samples within 1R, final dispersion, colour. */
+typedef struct orc_query_info {
+    int   fetch2r, num_surfaces_1r, num_samples_1r, num_samples_final;
+    int   emergency_passes, found_surface, used_nearest;
+    float dispersion;
+    float color[4];
+} orc_query_info;
+int orc_query(orc_ctx* c, const orc_recon_params* p, int x, int y, int i, orc_query_info* out);
+
+/* Brute-force leaf set (every valid leaf whose own bounds intersect), for
+ * checking that the LBVH traversal loses nothing. */
+int orc_leaves_bruteforce(orc_ctx* c, const float* o5, float R, int32_t* leaves, int max_leaves);
+
+/* Primitive restatements exposed for known-answer tests. */
+uint32_t orc_hash_bits(uint32_t a, uint32_t b);
+void     orc_ranrot(uint32_t seed, int n, uint32_t* out);
+float    orc_sobol(int dim, int i);
+void     orc_sobol2d(int i, float* out2);
+float    orc_halton(int base, int i);
+float    orc_lp(int i);
+void     orc_to_unit_disk(float x, float y, float* out2);
+/* computeTUXPlanes: pos/motion homogeneous (3 floats), screen W,H, coc (2).
+ * out: 12 plane floats + 2 tspan; returns 1 if valid. */
+int      orc_tux_planes(const float* pos, const float* motion, float W, float H,
+                        const float* coc, float* out14);
+/* TimeLensBounds(tb0, tb1) merge: 14 + 14 -> 14. */
+void     orc_merge_bounds(const float* a14, const float* b14, float* out14);
+/* FW::sort replica over floats with ties: sorts (key, payload) pairs by key
+ * with the reference's unstable quicksort; returns the payload permutation. */
+void     orc_fw_sort(int n, float* keys, int32_t* payload);
+/* Edge-function point-in-triangle as the reference evaluates it. */
+int      orc_triangle_covers(const float* r0, const float* r1, const float* r2, const float* o);
+/* sameSurface on two 14-float bounds at output (u,v,t). */
+int      orc_same_surface(const float* n0, const float* n1, const float* o5);
+
+#ifdef __cplusplus
+}
+#endif

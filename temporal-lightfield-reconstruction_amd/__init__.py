@@ -1,0 +1,319 @@
+"""Python binding of libtlr (ctypes over the C ABI in include/tlr.h).
+
+The product is the native library (lib/libtlr.so: HIP kernels for gfx950 +
+C ABI); this module only marshals arguments.  Every compute call goes to the
+device; there is no host fallback: on a machine without a usable GPU, Ctx()
+raises.
+
+Classes mirroring the reference interface (TreeGather,
+src/reconstruction_lib/reconstruction/Reconstruction.hpp:42-48):
+    UVTSampleBuffer  sample records + header (SampleBuffer.hpp:138-183)
+    CameraParams     reconstruction flags used by the path (CameraParams.hpp:92-107)
+    TreeGather       build in the constructor, reconstruct_dof_motion(image)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+TLR_LIB = os.path.join(LIB_DIR, "libtlr.so")
+SCENE_LIB = os.path.join(LIB_DIR, "libtlr_scene.so")
+
+STATUS = {0: "TLR_OK", 1: "TLR_E_INVALID", 2: "TLR_E_IO", 3: "TLR_E_ALL_DISCARDED",
+          4: "TLR_E_UNSUPPORTED_SPP", 5: "TLR_E_IMAGE_TOO_SMALL", 6: "TLR_E_HIP",
+          7: "TLR_E_CAPACITY", 8: "TLR_E_STATE"}
+
+EXPORTED = [
+    "tlr_create", "tlr_destroy", "tlr_last_error", "tlr_load_sample_file", "tlr_save_sample_file",
+    "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_sync",
+    "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
+    "tlr_copy_leaves", "tlr_copy_query_colors", "tlr_output_pattern",
+]
+
+
+class TlrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+class Header(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int),
+                ("coc_c0", C.c_float), ("coc_c1", C.c_float)]
+
+
+class Soa(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("x", "y", "w", "u", "v", "t", "r", "g", "b",
+                                          "mv_x", "mv_y", "mv_w")]
+
+
+class ReconParams(C.Structure):
+    _fields_ = [("out_spp", C.c_int), ("override_uvt", C.c_int), ("uvt", C.c_float * 3),
+                ("lens_gaussian", C.c_int), ("time_gaussian", C.c_int),
+                ("collect_stats", C.c_int), ("gate", C.c_int), ("reserved", C.c_int * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
+        "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count", "overflow",
+        "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class BuildInfo(C.Structure):
+    _fields_ = [("num_input", C.c_int), ("num_kept", C.c_int), ("num_bucketed", C.c_int),
+                ("spp_pow2", C.c_int), ("coc_adj", C.c_float * 2), ("bbmin", C.c_float * 2),
+                ("bbmax", C.c_float * 2), ("scale", C.c_float), ("bbmin_int", C.c_int * 2),
+                ("grid_w", C.c_int), ("grid_h", C.c_int), ("max_hits", C.c_int),
+                ("num_initial_leaves", C.c_int), ("num_leaves", C.c_int),
+                ("num_valid_leaves", C.c_int)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int),
+                ("coc_c0", C.c_float), ("coc_c1", C.c_float),
+                ("num_quads", C.c_int), ("quad_min_frac", C.c_float), ("quad_max_frac", C.c_float),
+                ("num_occluders", C.c_int), ("occluder_min_px", C.c_float),
+                ("occluder_max_px", C.c_float), ("w_min", C.c_float), ("w_max", C.c_float),
+                ("w_log_uniform", C.c_int), ("mv_xy_px", C.c_float), ("mv_w_frac", C.c_float),
+                ("static_frac", C.c_float), ("checker_period_px", C.c_float),
+                ("seed", C.c_uint64), ("threads", C.c_int)]
+
+
+_tlr = None
+_scene = None
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def lib():
+    """Loads lib/libtlr.so (raises if it has not been built)."""
+    global _tlr
+    if _tlr is None:
+        if not os.path.exists(TLR_LIB):
+            raise RuntimeError("libtlr.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(TLR_LIB)
+        V = C.c_void_p
+        L.tlr_create.argtypes = [C.POINTER(V), C.c_int]
+        L.tlr_destroy.argtypes = [V]
+        L.tlr_last_error.restype = C.c_char_p
+        L.tlr_last_error.argtypes = [V]
+        L.tlr_load_sample_file.argtypes = [C.c_char_p, C.POINTER(Header), C.POINTER(C.POINTER(C.c_float)),
+                                           C.POINTER(C.c_int64)]
+        L.tlr_save_sample_file.argtypes = [C.c_char_p, C.POINTER(Header), V, C.c_int, C.c_int]
+        L.tlr_free.argtypes = [V]
+        L.tlr_set_samples.argtypes = [V, C.POINTER(Header), V, C.c_int64]
+        L.tlr_set_samples_soa.argtypes = [V, C.POINTER(Header), C.POINTER(Soa), C.c_int]
+        L.tlr_build.argtypes = [V, C.c_float, C.c_float]
+        L.tlr_reconstruct.argtypes = [V, C.POINTER(ReconParams), V, C.c_int, C.POINTER(Stats)]
+        L.tlr_sync.argtypes = [V]
+        L.tlr_get_build_info.argtypes = [V, C.POINTER(BuildInfo)]
+        L.tlr_copy_reprojected.argtypes = [V, V, V, V, V]
+        L.tlr_copy_layout.argtypes = [V, V]
+        L.tlr_copy_initial_leaves.argtypes = [V, V]
+        L.tlr_copy_leaves.argtypes = [V, V, V]
+        L.tlr_copy_query_colors.argtypes = [V, V]
+        L.tlr_output_pattern.restype = C.c_int
+        L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
+        _tlr = L
+    return _tlr
+
+
+def scene_lib():
+    global _scene
+    if _scene is None:
+        if not os.path.exists(SCENE_LIB):
+            raise RuntimeError("libtlr_scene.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(SCENE_LIB)
+        L.tlr_scene_preset.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(SceneDesc)]
+        L.tlr_scene_generate.argtypes = [C.POINTER(SceneDesc), C.c_void_p]
+        _scene = L
+    return _scene
+
+
+def scene_desc(preset, width, height, spp, seed, threads=0, **overrides):
+    d = SceneDesc()
+    if scene_lib().tlr_scene_preset(preset, width, height, spp, C.c_uint64(seed), C.byref(d)) != 0:
+        raise ValueError("unknown scene preset %r" % preset)
+    d.threads = threads
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return d
+
+
+def make_scene(preset, width, height, spp, seed, threads=0, **overrides):
+    """Synthetic sample buffer: returns (UVTSampleBuffer)."""
+    d = scene_desc(preset, width, height, spp, seed, threads, **overrides)
+    rec = np.empty((height, width, spp, 16), np.float32)
+    if scene_lib().tlr_scene_generate(C.byref(d), _p(rec)) != 0:
+        raise RuntimeError("scene generation failed")
+    return UVTSampleBuffer(width, height, spp, d.coc_c0, d.coc_c1, rec)
+
+
+def output_pattern(out_spp, override_uvt=None):
+    uvt = np.zeros(3, np.float32)
+    if override_uvt is not None:
+        uvt[:] = override_uvt
+    flag = 1 if override_uvt is not None else 0
+    M = lib().tlr_output_pattern(out_spp, flag, _p(uvt), None)
+    out = np.zeros((64 * M, 5), np.float32)
+    lib().tlr_output_pattern(out_spp, flag, _p(uvt), _p(out))
+    return out
+
+
+class UVTSampleBuffer:
+    """Sample records (v1.3 16-float layout) plus header; indexed (y*W+x)*spp+i."""
+
+    def __init__(self, width, height, spp, coc_c0, coc_c1, records):
+        self.width, self.height, self.spp = int(width), int(height), int(spp)
+        self.coc = (float(coc_c0), float(coc_c1))
+        self.records = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        if self.records.shape[0] != self.width * self.height * self.spp:
+            raise ValueError("record count does not match the header")
+
+    @classmethod
+    def load(cls, path):
+        h = Header()
+        ptr = C.POINTER(C.c_float)()
+        n = C.c_int64(0)
+        rc = lib().tlr_load_sample_file(path.encode(), C.byref(h), C.byref(ptr), C.byref(n))
+        if rc != 0:
+            raise TlrError(rc, "cannot load %s" % path)
+        try:
+            arr = np.ctypeslib.as_array(ptr, shape=(n.value * 16,)).copy()
+        finally:
+            lib().tlr_free(C.cast(ptr, C.c_void_p))
+        return cls(h.width, h.height, h.spp, h.coc_c0, h.coc_c1, arr)
+
+    def save(self, path, binary=False, separate_header=False):
+        rc = lib().tlr_save_sample_file(path.encode(), C.byref(self.header()), _p(self.records),
+                                        int(binary), int(separate_header))
+        if rc != 0:
+            raise TlrError(rc, "cannot save %s" % path)
+
+    def header(self):
+        return Header(self.width, self.height, self.spp, self.coc[0], self.coc[1])
+
+
+class CameraParams:
+    """The CameraParams fields the reconstruction reads (CameraParams.hpp:92-107)."""
+
+    def __init__(self, out_spp=128, override_uvt=None, lens_gaussian=False, time_gaussian=False):
+        self.out_spp = out_spp
+        self.override_uvt = override_uvt
+        self.lens_gaussian = lens_gaussian
+        self.time_gaussian = time_gaussian
+
+
+class Ctx:
+    """One libtlr context (device buffers + stream) on one GPU."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().tlr_create(C.byref(self.h), device)
+        if rc != 0:
+            raise TlrError(rc, "tlr_create failed (no usable GPU?)")
+
+    def close(self):
+        if self.h:
+            lib().tlr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise TlrError(rc, lib().tlr_last_error(self.h).decode())
+
+    def set_samples(self, sbuf):
+        self.sbuf = sbuf
+        self._chk(lib().tlr_set_samples(self.h, C.byref(sbuf.header()), _p(sbuf.records),
+                                        sbuf.records.shape[0]))
+
+    def build(self, aperture_adjust=1.0, focal_adjust=1.0):
+        self._chk(lib().tlr_build(self.h, aperture_adjust, focal_adjust))
+
+    def reconstruct(self, out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,
+                    collect_stats=False, gate=False, out=None):
+        p = ReconParams()
+        p.out_spp = out_spp
+        p.override_uvt = 1 if override_uvt is not None else 0
+        if override_uvt is not None:
+            for k in range(3):
+                p.uvt[k] = override_uvt[k]
+        p.lens_gaussian = int(lens_gaussian)
+        p.time_gaussian = int(time_gaussian)
+        p.collect_stats = int(collect_stats)
+        p.gate = int(gate)
+        img = out if out is not None else np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
+        st = Stats()
+        self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img), 0, C.byref(st)))
+        return img, st.as_dict()
+
+    def build_info(self):
+        bi = BuildInfo()
+        self._chk(lib().tlr_get_build_info(self.h, C.byref(bi)))
+        return bi
+
+    def reprojected(self):
+        n = self.sbuf.records.shape[0]
+        out = [np.empty(n, np.float32) for _ in range(4)]
+        self._chk(lib().tlr_copy_reprojected(self.h, *[_p(a) for a in out]))
+        return out
+
+    def layout(self):
+        out = np.empty(max(self.build_info().num_bucketed, 1), np.int32)
+        self._chk(lib().tlr_copy_layout(self.h, _p(out)))
+        return out[:self.build_info().num_bucketed]
+
+    def initial_leaves(self):
+        n = self.build_info().num_initial_leaves
+        out = np.empty((max(n, 1), 6), np.int32)
+        self._chk(lib().tlr_copy_initial_leaves(self.h, _p(out)))
+        return out[:n]
+
+    def leaves(self):
+        n = self.build_info().num_leaves
+        s01 = np.empty((max(n, 1), 2), np.int32)
+        b = np.empty((max(n, 1), 14), np.float32)
+        self._chk(lib().tlr_copy_leaves(self.h, _p(s01), _p(b)))
+        return s01[:n], b[:n]
+
+    def query_colors(self, out_spp):
+        out = np.empty((self.sbuf.height, self.sbuf.width, out_spp, 4), np.float32)
+        self._chk(lib().tlr_copy_query_colors(self.h, _p(out)))
+        return out
+
+
+class TreeGather:
+    """Mirror of FW::TreeGather (Reconstruction.hpp:42-48): the constructor
+    builds the hierarchy, reconstruct_dof_motion fills an RGBA image."""
+
+    def __init__(self, sbuf, params, aperture_adjust=1.0, focal_adjust=1.0, device=0):
+        self.sbuf = sbuf
+        self.params = params
+        self.ctx = Ctx(device)
+        self.ctx.set_samples(sbuf)
+        self.ctx.build(aperture_adjust, focal_adjust)
+
+    def reconstruct_dof_motion(self, image=None):
+        if image is not None and (image.shape[0] < self.sbuf.height or image.shape[1] < self.sbuf.width):
+            raise TlrError(5, "TreeGather::reconstructDofMotion image smaller than sample buffer")
+        p = self.params
+        img, st = self.ctx.reconstruct(out_spp=p.out_spp, override_uvt=p.override_uvt,
+                                       lens_gaussian=p.lens_gaussian, time_gaussian=p.time_gaussian)
+        if image is not None:
+            image[:self.sbuf.height, :self.sbuf.width] = img
+            return image
+        return img

@@ -1,0 +1,61 @@
+"""Builds the native libraries in-tree.
+
+lib/libtlr.so        HIP kernels + C ABI (hipcc, gfx950)
+lib/libtlr_scene.so  synthetic sample-buffer generator (g++, host only)
+
+FP rules: no contraction (-ffp-contract=off), IEEE division and square root
+(HIP's default correctly-rounded f32 div/sqrt), no fast-math, so device
+arithmetic matches an IEEE restatement of the reference operation for
+operation.  Packed FP32 instructions are disabled.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+
+HIP_SOURCES = ["prims.hip", "build.hip", "query.hip", "tlr.cpp", "pattern.cpp", "sample_io.cpp"]
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]))
+    return r
+
+
+def build_scene():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libtlr_scene.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out,
+          os.path.join(CSRC, "scene", "scene.cpp"), "-lpthread"])
+    return out
+
+
+def build_tlr():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libtlr.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run(["hipcc"] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    return [build_scene(), build_tlr()]
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)

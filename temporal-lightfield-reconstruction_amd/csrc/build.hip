@@ -1,0 +1,758 @@
+// Tree build kernels: reprojection to the (u,v,t) centre, bucketing grid,
+// initial kd split, per-cell sort / bounds / surface scan, leaves, LBVH.
+//
+// Stage <-> reference:
+//   k_reproject, k_hits_max, k_grid_setup, k_cell_keys
+//       TreeGather::reprojectToUVTCenter  Reconstruction.cpp:514-640
+//       Sample::reprojectToUVTCenter      Reconstruction.hpp:93-121
+//   k_sat_rows/cols, k_kd_descend, k_emit_initial, k_place
+//       buildInitialRecursive             ReconstructionTreeBuilder.cpp:34-96
+//       candidate collection              ReconstructionTreeBuilder.cpp:133-140
+//   k_leaf_sort                           ReconstructionTreeBuilder.cpp:153 (FW::sort,
+//                                          src/framework/base/Sort.cpp:63-211)
+//   k_points                              ReconstructionTreeBuilder.cpp:170-213
+//   k_surface_scan                        ReconstructionTreeBuilder.cpp:222-340
+//   k_leaf_emit                           ReconstructionTreeBuilder.cpp:345-378
+//   k_lbvh_*                              replaces emitNodes (:409-451) and the
+//                                          task merge (Reconstruction.cpp:110-203)
+#include "build.hpp"
+
+namespace tlr {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ unsigned lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        T o = __shfl_xor(v, d, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        T o = __shfl_xor(v, d, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Reprojection (one thread per input sample).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, int H, float C0, float C1,
+                                                      float* __restrict__ rx, float* __restrict__ ry,
+                                                      float* __restrict__ rw, float* __restrict__ rt,
+                                                      int* __restrict__ hits, GridInfo* gi) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    bool ok = false, bad = false;
+    float ox = 0, oy = 0;
+    if (i < n) {
+        const float x = in.x[i], y = in.y[i], w = in.w[i], u = in.u[i], v = in.v[i], t = in.t[i];
+        const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
+        const float coc = C0 / w + C1;
+        const float px = x - coc * u;
+        const float py = y - coc * v;
+        const float Px = px * w, Py = py * w, Pw = 1.0f * w;
+        const float ax = Px - t * mvx, ay = Py - t * mvy, aw = Pw - t * mvw;
+        float nt = 0.5f;
+        float X = ax + nt * mvx, Y = ay + nt * mvy, Wt = aw + nt * mvw;
+        if (Wt <= 0.f) {
+            nt = 0.0f;
+            X = ax + nt * mvx; Y = ay + nt * mvy; Wt = aw + nt * mvw;
+        }
+        if (Wt <= 0.f) {
+            nt = 1.0f;
+            X = ax + nt * mvx; Y = ay + nt * mvy; Wt = aw + nt * mvw;
+        }
+        if (Wt > 0.f) {
+            const float c = rcp(Wt);
+            ox = X * c;
+            oy = Y * c;
+            if (isfinite(ox) && isfinite(oy)) {
+                ok = true;
+                rx[i] = ox; ry[i] = oy; rw[i] = Wt; rt[i] = nt;
+                const int sx = floor_i(ox), sy = floor_i(oy);
+                if (sx >= 0 && sy >= 0 && sx < W && sy < H) atomicAdd(&hits[sy * W + sx], 1);
+            } else {
+                bad = true;
+            }
+        }
+        if (!ok) rw[i] = 0.0f;  // marks a discarded sample
+    }
+    // bbox and counts: wave reduction, one atomic per wave
+    int kmin_x = ok ? order_key(ox) : 0x7FFFFFFF;
+    int kmin_y = ok ? order_key(oy) : 0x7FFFFFFF;
+    int kmax_x = ok ? order_key(ox) : (int)0x80000000;
+    int kmax_y = ok ? order_key(oy) : (int)0x80000000;
+    kmin_x = wave_min(kmin_x); kmin_y = wave_min(kmin_y);
+    kmax_x = wave_max(kmax_x); kmax_y = wave_max(kmax_y);
+    const uint64_t okm = __ballot(ok), badm = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (okm) {
+            atomicMin(&gi->bbmin_key[0], kmin_x);
+            atomicMin(&gi->bbmin_key[1], kmin_y);
+            atomicMax(&gi->bbmax_key[0], kmax_x);
+            atomicMax(&gi->bbmax_key[1], kmax_y);
+            atomicAdd(&gi->num_kept, (unsigned)__popcll(okm));
+        }
+        if (badm) atomicAdd(&gi->num_bad, (unsigned)__popcll(badm));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_hits_max(const int* __restrict__ hits, int n, GridInfo* gi) {
+    int m = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) m = max(m, hits[i]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&gi->max_hits, m);
+}
+
+// Grid sizing: Reconstruction.cpp:587-603
+__global__ void k_grid_setup(GridInfo* gi) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    gi->error = 0;
+    if (gi->num_bad) {
+        gi->error = 1;  // TLR_E_INVALID: non-finite reprojected position
+        return;
+    }
+    const int mh = gi->max_hits;
+    if (mh == 0) {
+        gi->error = 3;  // TLR_E_ALL_DISCARDED
+        return;
+    }
+    const float bx0 = order_val(gi->bbmin_key[0]), by0 = order_val(gi->bbmin_key[1]);
+    const float bx1 = order_val(gi->bbmax_key[0]), by1 = order_val(gi->bbmax_key[1]);
+    gi->bbmin[0] = bx0; gi->bbmin[1] = by0;
+    gi->bbmax[0] = bx1; gi->bbmax[1] = by1;
+    float scale = sqrtf(1.f * mh / 48);
+    const float ext = fmx(bx1 - bx0, by1 - by0);
+    if (scale * ext > 4096) scale = 4096 / ext;
+    gi->scale = scale;
+    const int ix0 = floor_i(bx0 * scale), iy0 = floor_i(by0 * scale);
+    const int ix1 = floor_i(bx1 * scale) + 1, iy1 = floor_i(by1 * scale) + 1;
+    gi->bbmin_i[0] = ix0;
+    gi->bbmin_i[1] = iy0;
+    gi->gw = ix1 - ix0;
+    gi->gh = iy1 - iy0;
+}
+
+// Bucket index of each kept sample; counts per cell.
+__global__ __launch_bounds__(kBlock) void k_cell_keys(const float* __restrict__ rx, const float* __restrict__ ry,
+                                                      const float* __restrict__ rw, int n, GridInfo* gi,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                      uint32_t* __restrict__ counts) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    bool in = false;
+    if (i < n) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (rw[i] > 0.0f) {
+            const float scale = gi->scale;
+            const int sx = floor_i(scale * rx[i]) - gi->bbmin_i[0];
+            const int sy = floor_i(scale * ry[i]) - gi->bbmin_i[1];
+            if (!(sx < 0 || sy < 0 || sx >= gi->gw || sy >= gi->gh)) {
+                key = (uint32_t)(sy * gi->gw + sx);
+                atomicAdd(&counts[key], 1u);
+                in = true;
+            }
+        }
+        keys[i] = key;
+        vals[i] = (uint32_t)i;
+    }
+    const uint64_t m = __ballot(in);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&gi->num_bucketed, (unsigned)__popcll(m));
+}
+
+// Summed-area table over the cell counts, (gh+1) x (gw+1), zero first row/col.
+// Rows: one block per grid row.
+__global__ __launch_bounds__(kBlock) void k_sat_rows(const uint32_t* __restrict__ counts, int gw, int gh,
+                                                     uint32_t* __restrict__ sat) {
+    const int y = blockIdx.x;
+    __shared__ uint32_t carry;
+    __shared__ uint32_t lds4[4];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    uint32_t* row = sat + (size_t)(y + 1) * (gw + 1);
+    if (threadIdx.x == 0) row[0] = 0;
+    for (int base = 0; base < gw; base += kBlock) {
+        const int x = base + threadIdx.x;
+        uint32_t v = (x < gw) ? counts[(size_t)y * gw + x] : 0u;
+        // inclusive block scan
+        uint32_t inc = v;
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) lds4[w] = inc;
+        __syncthreads();
+        uint32_t off = carry;
+        for (int k = 0; k < w; k++) off += lds4[k];
+        if (x < gw) row[x + 1] = off + inc;
+        __syncthreads();
+        if (threadIdx.x == kBlock - 1) carry = off + inc;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sat_cols(uint32_t* __restrict__ sat, int gw, int gh) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    if (x > gw) return;
+    sat[x] = 0;
+    uint32_t run = 0;
+    for (int y = 1; y <= gh; y++) {
+        uint32_t* p = sat + (size_t)y * (gw + 1) + x;
+        run += *p;
+        *p = run;
+    }
+}
+
+__device__ __forceinline__ uint32_t rect_count(const uint32_t* __restrict__ sat, int gw, int x0, int y0, int x1,
+                                               int y1) {
+    const size_t s = (size_t)gw + 1;
+    return sat[(size_t)y1 * s + x1] - sat[(size_t)y0 * s + x1] - sat[(size_t)y1 * s + x0] +
+           sat[(size_t)y0 * s + x0];
+}
+
+// Each cell walks the kd split (buildInitialRecursive) from the root down to
+// the initial leaf containing it, accumulating the sample offset of the leaf
+// in depth-first order.  The leaf's top-left cell owns it.
+__global__ __launch_bounds__(kBlock) void k_kd_descend(const uint32_t* __restrict__ sat, int gw, int gh,
+                                                       int4* __restrict__ owner, uint32_t* __restrict__ own_flag,
+                                                       int* __restrict__ own_y1) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= gw * gh) return;
+    const int cx = c % gw, cy = c / gw;
+    int x0 = 0, x1 = gw, y0 = 0, y1 = gh;
+    uint32_t off = 0;
+    for (;;) {
+        const int dx = x1 - x0, dy = y1 - y0;
+        bool leaf = max(dx, dy) == 1;
+        if (!leaf && dx * dy <= 64) leaf = rect_count(sat, gw, x0, y0, x1, y1) <= 48u;
+        if (leaf) break;
+        if (dx > dy) {
+            const int xm = (x1 + x0) / 2;
+            if (cx < xm) {
+                x1 = xm;
+            } else {
+                off += rect_count(sat, gw, x0, y0, xm, y1);
+                x0 = xm;
+            }
+        } else {
+            const int ym = (y1 + y0) / 2;
+            if (cy < ym) {
+                y1 = ym;
+            } else {
+                off += rect_count(sat, gw, x0, y0, x1, ym);
+                y0 = ym;
+            }
+        }
+    }
+    owner[c] = make_int4(x0, y0, x1, (int)off);
+    const bool own = (cx == x0 && cy == y0);
+    own_flag[c] = own ? 1u : 0u;
+    if (own) own_y1[c] = y1;
+}
+
+// Initial leaf records from owner cells: x0,y0,x1,y1 | offset,count.
+__global__ __launch_bounds__(kBlock) void k_emit_initial(const int4* __restrict__ owner,
+                                                         const uint32_t* __restrict__ own_flag,
+                                                         const int* __restrict__ own_y1,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const uint32_t* __restrict__ sat, int gw, int gh,
+                                                         int4* __restrict__ rect, int2* __restrict__ offcnt) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= gw * gh) return;
+    if (!own_flag[c]) return;
+    const int4 o = owner[c];
+    const int y1 = own_y1[c];
+    const uint32_t k = idx[c];
+    rect[k] = make_int4(o.x, o.y, o.z, y1);
+    offcnt[k] = make_int2(o.w, (int)rect_count(sat, gw, o.x, o.y, o.z, y1));
+}
+
+// Place each bucketed sample at its candidate slot: initial-leaf offset +
+// samples in the leaf's cells before it in row-major order + rank in cell.
+__global__ __launch_bounds__(kBlock) void k_place(const uint32_t* __restrict__ skeys,
+                                                  const uint32_t* __restrict__ svals, int nb, int gw,
+                                                  const int4* __restrict__ owner,
+                                                  const uint32_t* __restrict__ sat,
+                                                  const uint32_t* __restrict__ cell_start,
+                                                  const float* __restrict__ rt, const float* __restrict__ rw,
+                                                  uint32_t* __restrict__ cand, float* __restrict__ ct,
+                                                  float* __restrict__ cw) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= nb) return;
+    const uint32_t key = skeys[s];
+    const uint32_t v = svals[s];
+    const int cx = (int)(key % (uint32_t)gw), cy = (int)(key / (uint32_t)gw);
+    const int4 o = owner[key];
+    uint32_t pos = (uint32_t)o.w;
+    pos += rect_count(sat, gw, o.x, o.y, o.z, cy);   // full rows above
+    pos += rect_count(sat, gw, o.x, cy, cx, cy + 1);  // cells to the left
+    pos += (uint32_t)s - cell_start[key];
+    cand[pos] = v;
+    ct[pos] = rt[v];
+    cw[pos] = rw[v];
+}
+
+// ---------------------------------------------------------------------------
+// FW::sort replica (src/framework/base/Sort.cpp:63-211) on (t, w) keys of one
+// initial leaf; sampleCompareFuncInc at ReconstructionTreeBuilder.cpp:99-117.
+// ---------------------------------------------------------------------------
+struct CandView {
+    uint32_t* idx;
+    float* t;
+    float* w;
+    __device__ __forceinline__ int cmp(int a, int b) const {
+        const float ta = t[a], tb = t[b];
+        if (ta < tb) return -1;
+        if (ta > tb) return 1;
+        const float wa = w[a], wb = w[b];
+        if (wa < wb) return -1;
+        if (wa > wb) return 1;
+        return 0;
+    }
+    __device__ __forceinline__ void swp(int a, int b) const {
+        uint32_t i0 = idx[a]; idx[a] = idx[b]; idx[b] = i0;
+        float t0 = t[a]; t[a] = t[b]; t[b] = t0;
+        float w0 = w[a]; w[a] = w[b]; w[b] = w0;
+    }
+};
+
+__device__ void fwsort_insertion(const CandView& v, int start, int size) {
+    for (int i = 1; i < size; i++) {
+        int j = start + i - 1;
+        while (j >= start && v.cmp(j, j + 1) > 0) {
+            v.swp(j, j + 1);
+            j--;
+        }
+    }
+}
+
+__device__ int fwsort_partition(const CandView& v, int low, int high) {
+    int l = low, c = (low + high) >> 1, h = high - 2;
+    if (v.cmp(l, h) > 0) {
+        int tmp = l; l = h; h = tmp;
+    }
+    if (v.cmp(l, c) > 0) c = l;
+    const int piv = (v.cmp(c, h) > 0) ? h : c;
+    v.swp(piv, high - 1);
+    int i = low - 1, j = high - 1;
+    for (;;) {
+        do i++; while (v.cmp(i, high - 1) < 0);
+        do j--; while (v.cmp(j, high - 1) > 0);
+        if (i >= j) break;
+        v.swp(i, j);
+    }
+    v.swp(i, high - 1);
+    return i;
+}
+
+__device__ void fwsort(const CandView& v, int start, int end) {
+    if (end - start < 2) return;
+    int low = start, high = end;
+    int stack[32];
+    int sp = 0;
+    stack[sp++] = high;
+    while (sp) {
+        high = stack[--sp];
+        if (high - low < 16 || sp + 2 > 32) {
+            fwsort_insertion(v, low, high - low);
+            low = high + 1;
+            continue;
+        }
+        const int i = fwsort_partition(v, low, high);
+        if (high - i > 2) stack[sp++] = high;
+        if (i - low > 1) stack[sp++] = i;
+        else low = i + 1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_sort(const int2* __restrict__ offcnt, int nil, uint32_t* cand,
+                                                      float* ct, float* cw) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nil) return;
+    const int2 oc = offcnt[k];
+    CandView v{cand + oc.x, ct + oc.x, cw + oc.x};
+    fwsort(v, 0, oc.y);
+}
+
+// ---------------------------------------------------------------------------
+// Per-sample tree data in layout order: points, bounds and t=0/t=1 boxes
+// (ReconstructionTreeBuilder.cpp:170-213).  Boxes of samples with an empty
+// time span are zero.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ cand, int nb, InSoA in,
+                                                   const float* __restrict__ rx, const float* __restrict__ ry,
+                                                   const float* __restrict__ rw, const float* __restrict__ rt,
+                                                   float W, float H, float cc0, float cc1, Points pts,
+                                                   Bounds* __restrict__ bounds, float4* __restrict__ boxes) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nb) return;
+    const uint32_t i = cand[k];
+    const float x = rx[i], y = ry[i], w = rw[i], t = rt[i];
+    const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
+    pts.x[k] = x; pts.y[k] = y; pts.w[k] = w; pts.t[k] = t;
+    pts.mvx[k] = mvx; pts.mvy[k] = mvy; pts.mvw[k] = mvw;
+    pts.r[k] = in.r[i]; pts.g[k] = in.g[i]; pts.b[k] = in.b[i];
+
+    Bounds b;
+    const float px = x * w - t * mvx, py = y * w - t * mvy, pw = w - t * mvw;
+    tux_bounds(px, py, pw, mvx, mvy, mvw, W, H, cc0, cc1, b);
+    float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
+    if (bounds_valid(b)) {
+        const float ta = b.ts0, tb = b.ts1;
+        const float ax = x * w + (ta - t) * mvx, ay = y * w + (ta - t) * mvy, aw = w + (ta - t) * mvw;
+        const float bx = x * w + (tb - t) * mvx, by = y * w + (tb - t) * mvy, bw = w + (tb - t) * mvw;
+        const float ca = rcp(aw), cb = rcp(bw);
+        const float xta = ax * ca, yta = ay * ca;
+        const float xtb = bx * cb, ytb = by * cb;
+        const float oodt = 1.0f / (tb - ta);
+        const float dxdt = (xtb - xta) * oodt;
+        const float dydt = (ytb - yta) * oodt;
+        const float x0 = xta - ta * dxdt, y0 = yta - ta * dydt;
+        const float x1 = xtb + (1.0f - tb) * dxdt, y1 = ytb + (1.0f - tb) * dydt;
+        const float cta = cc0 / aw + cc1;
+        const float ctb = cc0 / bw + cc1;
+        b0 = make_float4(x0 + cta, y0 + cta, x0 - cta, y0 - cta);   // boxa, boxb (t=0)
+        b1 = make_float4(x1 + ctb, y1 + ctb, x1 - ctb, y1 - ctb);   // boxaT1, boxbT1
+    }
+    bounds[k] = b;
+    boxes[2 * (size_t)k] = b0;
+    boxes[2 * (size_t)k + 1] = b1;
+}
+
+// Sequential surface scan of one initial leaf (ReconstructionTreeBuilder.cpp:
+// 222-340, keeping the cocboundst[0] typo at :334,336).  Marks leaf starts.
+__global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict__ offcnt, int nil,
+                                                         const float* __restrict__ pt,
+                                                         const float4* __restrict__ boxes,
+                                                         uint32_t* __restrict__ start_flag) {
+    const int li = blockIdx.x * kBlock + threadIdx.x;
+    if (li >= nil) return;
+    const int2 oc = offcnt[li];
+    const int base = oc.x, n = oc.y;
+    if (n <= 0) return;
+    const float4* B = boxes + 2 * (size_t)base;
+    const float* T = pt + base;
+    start_flag[base] = 1u;
+    int sb = 0;
+    bool reset = true;
+    float cb00 = 0, cb01 = 0, cb10 = 0, cb11 = 0, mn0 = 0, mn1 = 0, mx0 = 0, mx1 = 0, st = 0;
+    const float TH = 1 * 2.0f;
+    for (int k = 1; k < n; ++k) {
+        if (reset) {
+            const float4 a = B[2 * sb], c = B[2 * sb + 1];
+            cb00 = cb01 = 0.5f * (a.x - a.z);
+            cb10 = cb11 = 0.5f * (c.x - c.z);
+            mn0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
+            mn1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
+            mx0 = mn0; mx1 = mn1;
+            st = T[sb];
+            reset = false;
+        }
+        if (st != T[k]) {
+            sb = k;
+            start_flag[base + k] = 1u;
+            reset = true;
+            continue;
+        }
+        const float4 a = B[2 * k], c = B[2 * k + 1];
+        const float c0 = 0.5f * (a.x - a.z);
+        const float c1 = 0.5f * (c.x - c.z);
+        const float m0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
+        const float m1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
+        const float dmv = fmx(fmx(fabsf(mn0 - m0), fabsf(mn1 - m1)), fmx(fabsf(mx0 - m0), fabsf(mx1 - m1)));
+        const bool early = fmx(fabsf(c0 - cb00), fabsf(c0 - cb01)) < 0.5f * TH &&
+                           fmx(fabsf(c1 - cb10), fabsf(c1 - cb11)) < 0.5f * TH && dmv < TH;
+        bool conflict = false;
+        if (!early) {
+            for (int j = sb; j < k; ++j) {
+                const float4 ja = B[2 * j], jc = B[2 * j + 1];
+                const bool bX = tern_eq4(tern_cmp(ja.x, a.x, TH), tern_cmp(ja.z, a.z, TH), tern_cmp(jc.x, c.x, TH),
+                                         tern_cmp(jc.z, c.z, TH));
+                const bool bY = tern_eq4(tern_cmp(ja.y, a.y, TH), tern_cmp(ja.w, a.w, TH), tern_cmp(jc.y, c.y, TH),
+                                         tern_cmp(jc.w, c.w, TH));
+                if (!(bX && bY)) {
+                    sb = k;
+                    start_flag[base + k] = 1u;
+                    conflict = true;
+                    reset = true;
+                    break;
+                }
+            }
+        }
+        if (!conflict) {
+            cb00 = fmn(c0, cb00);
+            cb01 = fmx(c0, cb00);
+            cb10 = fmn(c1, cb10);
+            cb11 = fmx(c1, cb10);
+            mn0 = fmn(mn0, m0); mn1 = fmn(mn1, m1);
+            mx0 = fmx(mx0, m0); mx1 = fmx(mx1, m1);
+        }
+    }
+}
+
+// One thread per leaf start: leaf id = scanned flag, bounds folded in sample
+// order with node.tlb = TimeLensBounds(bounds[k], node.tlb).
+__global__ __launch_bounds__(kBlock) void k_leaf_emit(const uint32_t* __restrict__ start_flag,
+                                                      const uint32_t* __restrict__ leaf_pos, int nb,
+                                                      const Bounds* __restrict__ bounds, Points pts,
+                                                      LeafRec* __restrict__ leaves, Bounds* __restrict__ leaf_b,
+                                                      uint32_t* __restrict__ valid_flag) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nb || !start_flag[k]) return;
+    const uint32_t id = leaf_pos[k];
+    int e = k + 1;
+    while (e < nb && !start_flag[e]) e++;
+    Bounds acc;
+    bounds_zero(acc);
+    for (int j = k; j < e; j++) {
+        Bounds nb2;
+        bounds_merge(bounds[j], acc, nb2);
+        acc = nb2;
+    }
+    LeafRec r;
+    r.s0 = k;
+    r.s1 = e;
+    r.w0 = pts.w[k];
+    r.t0 = pts.t[k];
+    r.mvw0 = pts.mvw[k];
+    r.valid = bounds_valid(acc) ? 1 : 0;
+    r.pad[0] = r.pad[1] = 0.0f;
+    leaves[id] = r;
+    leaf_b[id] = acc;
+    valid_flag[id] = r.valid ? 1u : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// LBVH over valid leaves (Karras 2012).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quant16(float v, float lo, float sc) {
+    const float f = (v - lo) * sc;
+    if (!(f >= 0.0f)) return 0;
+    if (f >= 65535.0f) return 65535;
+    return (uint32_t)f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lbvh_keys(const uint32_t* __restrict__ valid_flag,
+                                                      const uint32_t* __restrict__ valid_pos, int L,
+                                                      const Bounds* __restrict__ leaf_b, const GridInfo* gi,
+                                                      uint32_t* __restrict__ codes, uint32_t* __restrict__ ids) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= L || !valid_flag[i]) return;
+    const Bounds& b = leaf_b[i];
+    const float cx = 0.5f * (plane_eval(b.p + 0, 0.5f, 0.0f) + plane_eval(b.p + 3, 0.5f, 0.0f));
+    const float cy = 0.5f * (plane_eval(b.p + 6, 0.5f, 0.0f) + plane_eval(b.p + 9, 0.5f, 0.0f));
+    const float bx0 = gi->bbmin[0], by0 = gi->bbmin[1];
+    const float ex = gi->bbmax[0] - bx0, ey = gi->bbmax[1] - by0;
+    const float sx = (ex > 0.0f) ? 65535.0f / ex : 0.0f;
+    const float sy = (ey > 0.0f) ? 65535.0f / ey : 0.0f;
+    const uint32_t code = spread16(quant16(cx, bx0, sx)) | (spread16(quant16(cy, by0, sy)) << 1);
+    const uint32_t p = valid_pos[i];
+    codes[p] = code;
+    ids[p] = (uint32_t)i;
+}
+
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ code, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = code[i], b = code[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+// Internal node i: children, parent links, range [lo,hi], split owner.
+__global__ __launch_bounds__(kBlock) void k_lbvh_nodes(const uint32_t* __restrict__ code, int n,
+                                                       int* __restrict__ child0, int* __restrict__ child1,
+                                                       int* __restrict__ parent, int2* __restrict__ range,
+                                                       int* __restrict__ split_owner) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (kdelta(code, n, i, i + 1) - kdelta(code, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(code, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(code, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(code, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dn = kdelta(code, n, i, j);
+    int s = 0;
+    int tt = l;
+    do {
+        tt = (tt + 1) >> 1;
+        if (kdelta(code, n, i, i + (s + tt) * d) > dn) s += tt;
+    } while (tt > 1);
+    const int g = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    // node ids: internal k -> k, leaf slot k -> (n-1)+k
+    const int c0 = (lo == g) ? (n - 1 + g) : g;
+    const int c1 = (hi == g + 1) ? (n - 1 + g + 1) : (g + 1);
+    child0[i] = c0;
+    child1[i] = c1;
+    parent[c0] = i;
+    parent[c1] = i;
+    range[i] = make_int2(lo, hi);
+    split_owner[g] = i;
+}
+
+// Bottom-up bounds: the second child to arrive merges (child0, child1).
+__global__ __launch_bounds__(kBlock) void k_lbvh_bounds(const uint32_t* __restrict__ ids, int n,
+                                                        const Bounds* leaf_b, const int* __restrict__ child0,
+                                                        const int* __restrict__ child1,
+                                                        const int* __restrict__ parent, Bounds* node_b,
+                                                        int* flags) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    int p = parent[n - 1 + k];
+    while (p >= 0) {
+        __threadfence();
+        const int old = atomicAdd(&flags[p], 1);
+        if (old == 0) return;
+        __threadfence();
+        const int c0 = child0[p], c1 = child1[p];
+        const Bounds* b0 = (c0 >= n - 1) ? &leaf_b[ids[c0 - (n - 1)]] : &node_b[c0];
+        const Bounds* b1 = (c1 >= n - 1) ? &leaf_b[ids[c1 - (n - 1)]] : &node_b[c1];
+        // The acquire fence above makes the other child's bounds visible.
+        const Bounds a = *b0, b = *b1;
+        Bounds r;
+        bounds_merge(a, b, r);
+        node_b[p] = r;
+        p = parent[p];
+    }
+}
+
+// Traversal records with escape links (next node after a subtree).
+__global__ __launch_bounds__(kBlock) void k_lbvh_trav(const uint32_t* __restrict__ ids, int n,
+                                                      const Bounds* __restrict__ leaf_b,
+                                                      const Bounds* __restrict__ node_b,
+                                                      const int* __restrict__ child0,
+                                                      const int* __restrict__ child1,
+                                                      const int2* __restrict__ range,
+                                                      const int* __restrict__ split_owner,
+                                                      TravNode* __restrict__ trav) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= 2 * n - 1) return;
+    TravNode t;
+    int hi;
+    const Bounds* b;
+    t.child1 = -1;
+    t.pad = 0;
+    if (k < n - 1) {
+        b = &node_b[k];
+        t.child = child0[k];
+        t.child1 = child1[k];
+        hi = range[k].y;
+    } else {
+        const int slot = k - (n - 1);
+        const uint32_t lid = ids[slot];
+        b = &leaf_b[lid];
+        t.child = -(int)lid - 1;
+        hi = slot;
+    }
+#pragma unroll
+    for (int q = 0; q < 12; q++) t.p[q] = b->p[q];
+    t.escape = (hi == n - 1) ? -1 : child1[split_owner[hi]];
+    trav[k] = t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+#define TLR_GRID(n) dim3((unsigned)(((n) + kBlock - 1) / kBlock)), dim3(kBlock)
+
+void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, float* rx, float* ry, float* rw,
+                      float* rt, int* hits, GridInfo* gi, hipStream_t s) {
+    if (n > 0) k_reproject<<<TLR_GRID(n), 0, s>>>(in, n, W, H, C0, C1, rx, ry, rw, rt, hits, gi);
+    const int nh = W * H;
+    unsigned blocks = (unsigned)std::min(4096, (nh + kBlock - 1) / kBlock);
+    if (blocks == 0) blocks = 1;
+    k_hits_max<<<blocks, kBlock, 0, s>>>(hits, nh, gi);
+    k_grid_setup<<<1, 64, 0, s>>>(gi);
+}
+
+void launch_cell_keys(const float* rx, const float* ry, const float* rw, int n, GridInfo* gi, uint32_t* keys,
+                      uint32_t* vals, uint32_t* counts, hipStream_t s) {
+    if (n > 0) k_cell_keys<<<TLR_GRID(n), 0, s>>>(rx, ry, rw, n, gi, keys, vals, counts);
+}
+
+void launch_sat(const uint32_t* counts, int gw, int gh, uint32_t* sat, hipStream_t s) {
+    k_sat_rows<<<gh, kBlock, 0, s>>>(counts, gw, gh, sat);
+    k_sat_cols<<<TLR_GRID(gw + 1), 0, s>>>(sat, gw, gh);
+}
+
+void launch_kd(const uint32_t* sat, int gw, int gh, int4* owner, uint32_t* own_flag, int* own_y1, hipStream_t s) {
+    k_kd_descend<<<TLR_GRID(gw * gh), 0, s>>>(sat, gw, gh, owner, own_flag, own_y1);
+}
+
+void launch_emit_initial(const int4* owner, const uint32_t* own_flag, const int* own_y1, const uint32_t* idx,
+                         const uint32_t* sat, int gw, int gh, int4* rect, int2* offcnt, hipStream_t s) {
+    k_emit_initial<<<TLR_GRID(gw * gh), 0, s>>>(owner, own_flag, own_y1, idx, sat, gw, gh, rect, offcnt);
+}
+
+void launch_place(const uint32_t* skeys, const uint32_t* svals, int nb, int gw, const int4* owner,
+                  const uint32_t* sat, const uint32_t* cell_start, const float* rt, const float* rw, uint32_t* cand,
+                  float* ct, float* cw, hipStream_t s) {
+    if (nb > 0) k_place<<<TLR_GRID(nb), 0, s>>>(skeys, svals, nb, gw, owner, sat, cell_start, rt, rw, cand, ct, cw);
+}
+
+void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, float* cw, hipStream_t s) {
+    if (nil > 0) k_leaf_sort<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, cand, ct, cw);
+}
+
+void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
+                   const float* rw, const float* rt, float W, float H, float cc0, float cc1, const Points& pts,
+                   Bounds* bounds, float4* boxes, hipStream_t s) {
+    if (nb > 0) k_points<<<TLR_GRID(nb), 0, s>>>(cand, nb, in, rx, ry, rw, rt, W, H, cc0, cc1, pts, bounds, boxes);
+}
+
+void launch_surface_scan(const int2* offcnt, int nil, const float* pt, const float4* boxes, uint32_t* flags,
+                         hipStream_t s) {
+    if (nil > 0) k_surface_scan<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, pt, boxes, flags);
+}
+
+void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
+                      const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s) {
+    if (nb > 0) k_leaf_emit<<<TLR_GRID(nb), 0, s>>>(flags, leaf_pos, nb, bounds, pts, leaves, leaf_b, valid_flag);
+}
+
+void launch_lbvh_keys(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
+                      const GridInfo* gi, uint32_t* codes, uint32_t* ids, hipStream_t s) {
+    if (L > 0) k_lbvh_keys<<<TLR_GRID(L), 0, s>>>(valid_flag, valid_pos, L, leaf_b, gi, codes, ids);
+}
+
+void launch_lbvh_nodes(const uint32_t* codes, int n, int* c0, int* c1, int* parent, int2* range, int* split_owner,
+                       hipStream_t s) {
+    if (n > 1) k_lbvh_nodes<<<TLR_GRID(n - 1), 0, s>>>(codes, n, c0, c1, parent, range, split_owner);
+}
+
+void launch_lbvh_bounds(const uint32_t* ids, int n, const Bounds* leaf_b, const int* c0, const int* c1,
+                        const int* parent, Bounds* node_b, int* flags, hipStream_t s) {
+    if (n > 1) k_lbvh_bounds<<<TLR_GRID(n), 0, s>>>(ids, n, leaf_b, c0, c1, parent, node_b, flags);
+}
+
+void launch_lbvh_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
+                      const int* c1, const int2* range, const int* split_owner, TravNode* trav, hipStream_t s) {
+    if (n > 0) k_lbvh_trav<<<TLR_GRID(2 * n - 1), 0, s>>>(ids, n, leaf_b, node_b, c0, c1, range, split_owner, trav);
+}
+
+}  // namespace tlr

@@ -1,0 +1,40 @@
+// Launchers of the build kernels (build.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "tlr_device.hpp"
+
+namespace tlr {
+
+void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, float* rx, float* ry, float* rw,
+                      float* rt, int* hits, GridInfo* gi, hipStream_t s);
+void launch_cell_keys(const float* rx, const float* ry, const float* rw, int n, GridInfo* gi, uint32_t* keys,
+                      uint32_t* vals, uint32_t* counts, hipStream_t s);
+void launch_sat(const uint32_t* counts, int gw, int gh, uint32_t* sat, hipStream_t s);
+void launch_kd(const uint32_t* sat, int gw, int gh, int4* owner, uint32_t* own_flag, int* own_y1, hipStream_t s);
+void launch_emit_initial(const int4* owner, const uint32_t* own_flag, const int* own_y1, const uint32_t* idx,
+                         const uint32_t* sat, int gw, int gh, int4* rect, int2* offcnt, hipStream_t s);
+void launch_place(const uint32_t* skeys, const uint32_t* svals, int nb, int gw, const int4* owner,
+                  const uint32_t* sat, const uint32_t* cell_start, const float* rt, const float* rw, uint32_t* cand,
+                  float* ct, float* cw, hipStream_t s);
+void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, float* cw, hipStream_t s);
+void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
+                   const float* rw, const float* rt, float W, float H, float cc0, float cc1, const Points& pts,
+                   Bounds* bounds, float4* boxes, hipStream_t s);
+void launch_surface_scan(const int2* offcnt, int nil, const float* pt, const float4* boxes, uint32_t* flags,
+                         hipStream_t s);
+void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
+                      const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s);
+void launch_lbvh_keys(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
+                      const GridInfo* gi, uint32_t* codes, uint32_t* ids, hipStream_t s);
+void launch_lbvh_nodes(const uint32_t* codes, int n, int* c0, int* c1, int* parent, int2* range, int* split_owner,
+                       hipStream_t s);
+void launch_lbvh_bounds(const uint32_t* ids, int n, const Bounds* leaf_b, const int* c0, const int* c1,
+                        const int* parent, Bounds* node_b, int* flags, hipStream_t s);
+void launch_lbvh_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
+                      const int* c1, const int2* range, const int* split_owner, TravNode* trav, hipStream_t s);
+
+}  // namespace tlr

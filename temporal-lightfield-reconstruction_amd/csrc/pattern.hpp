@@ -1,0 +1,11 @@
+// Host generation of the output sampling pattern (pattern.cpp).
+#pragma once
+#include <vector>
+
+namespace tlr {
+
+// Fills out5 with 64*M templates (x, y, u, v, t); returns M (1 when
+// override_uvt, else out_spp).
+int make_output_pattern(int out_spp, bool override_uvt, const float* uvt, std::vector<float>& out5);
+
+}  // namespace tlr

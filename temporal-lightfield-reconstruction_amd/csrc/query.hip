@@ -1,0 +1,552 @@
+// Output-sample reconstruction: one wavefront per output sample (x,y,u,v,t).
+//
+// Follows Filterer::reconstruct / collectInputSamples2
+// (ReconstructionOur.cpp:50-511) and FilterTask::process
+// (Reconstruction.cpp:642-720).  All decisions use the same FP32 values as
+// the sequential reference; lanes only split independent work:
+//   - hierarchy traversal: a wave-wide frontier over the LBVH (the set of
+//     leaves found does not depend on visiting order);
+//   - leaf ordering: rank sort by (key, leaf id);
+//   - per-leaf sample tests: one sample per lane, kept samples compacted in
+//     sample order with ballot + mbcnt;
+//   - sameSurface against earlier leaves: one leaf per lane, AND-reduced;
+//   - triangle coverage: one (k,l) vertex pair per lane for each m.
+// Sums that the reference accumulates sequentially (tent filter) are
+// accumulated in the same order by every lane.
+#include "query.hpp"
+
+namespace tlr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+enum { XPOS = 1, YPOS = 2, XNEG = 4, YNEG = 8 };
+
+__device__ __forceinline__ unsigned lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// Orders global-memory scratch writes of the wave before later reads.
+__device__ __forceinline__ void wave_sync_mem() { __threadfence_block(); }
+
+struct WaveMem {
+    int* stack;
+    float* lkey;
+    int* lid;
+    int* sid;
+    float *sx, *sy, *sd;
+    int *sidx, *srpos;
+    int *f_start, *f_size, *f_quad, *f_mini;
+    float* f_min;
+};
+
+__device__ __forceinline__ WaveMem wave_mem(float* base) {
+    WaveMem m;
+    char* p = reinterpret_cast<char*>(base);
+    auto take = [&](size_t words) {
+        char* r = p;
+        p += words * 4;
+        return r;
+    };
+    m.stack = reinterpret_cast<int*>(take(kStackCap));
+    m.lkey = reinterpret_cast<float*>(take(kLeafCap));
+    m.lid = reinterpret_cast<int*>(take(kLeafCap));
+    m.sid = reinterpret_cast<int*>(take(kLeafCap));
+    m.sx = reinterpret_cast<float*>(take(kSampleCap));
+    m.sy = reinterpret_cast<float*>(take(kSampleCap));
+    m.sd = reinterpret_cast<float*>(take(kSampleCap));
+    m.sidx = reinterpret_cast<int*>(take(kSampleCap));
+    m.srpos = reinterpret_cast<int*>(take(kSampleCap));
+    m.f_start = reinterpret_cast<int*>(take(kLeafCap));
+    m.f_size = reinterpret_cast<int*>(take(kLeafCap));
+    m.f_quad = reinterpret_cast<int*>(take(kLeafCap));
+    m.f_mini = reinterpret_cast<int*>(take(kLeafCap));
+    m.f_min = reinterpret_cast<float*>(take(kLeafCap));
+    return m;
+}
+
+struct OSample {
+    float x, y, u, v, t;
+};
+
+// Node::intersect: Reconstruction.hpp:204-216
+__device__ __forceinline__ bool node_hit(const float* p, const OSample& o, float R) {
+    const float mnx = plane_eval(p + 0, o.t, o.u) - R;
+    const float mny = plane_eval(p + 6, o.t, o.v) - R;
+    const float mxx = plane_eval(p + 3, o.t, o.u) + R;
+    const float mxy = plane_eval(p + 9, o.t, o.v) + R;
+    return (o.x >= mnx && o.y >= mny && o.x <= mxx && o.y <= mxy);
+}
+
+// Reprojection of tree sample k to the output's (u,v,t):
+// ReconstructionOur.cpp:438-451
+__device__ __forceinline__ void reproject_to(const Points& P, int k, const OSample& o, float cc0, float cc1,
+                                             float& x, float& y) {
+    const float sx = P.x[k], sy = P.y[k], sw = P.w[k], st = P.t[k];
+    const float mvx = P.mvx[k], mvy = P.mvy[k], mvw = P.mvw[k];
+    if (mvx != 0.0f || mvy != 0.0f || mvw != 0.0f) {
+        const float dt = o.t - st;
+        const float Px = sx * sw + dt * mvx;
+        const float Py = sy * sw + dt * mvy;
+        const float Pw = 1.0f * sw + dt * mvw;
+        const float c = rcp(Pw);
+        const float coc = cc0 / Pw + cc1;
+        x = Px * c + coc * o.u;
+        y = Py * c + coc * o.v;
+    } else {
+        const float coc = cc0 / sw + cc1;
+        x = sx + coc * o.u;
+        y = sy + coc * o.v;
+    }
+}
+
+struct Gather {
+    int total;     // gathered samples
+    int nsurf;     // surfaces
+    int used;      // sample slots in use (>= total after merges)
+    int overflow;
+};
+
+// collectInputSamples2 for one output sample at radius R (wave-uniform).
+__device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& o, float R,
+                         unsigned long long* st) {
+    const int lane = threadIdx.x & 63;
+    Gather g{0, 0, 0, 0};
+    int nl = 0;
+    // --- traversal: wave-wide frontier stack ---
+    if (Q.n_nodes > 0) {
+        int sp = 1;
+        if (lane == 0) m.stack[0] = 0;
+        wave_sync_mem();
+        while (sp > 0) {
+            const int take = sp < 64 ? sp : 64;
+            const int base = sp - take;
+            int node = -1;
+            if (lane < take) node = m.stack[base + lane];
+            bool hit = false, internal = false;
+            int c0 = 0, c1 = 0, lidv = 0;
+            float key = 0.0f;
+            if (node >= 0) {
+                const TravNode& tn = Q.trav[node];
+                float p[12];
+#pragma unroll
+                for (int q = 0; q < 12; q++) p[q] = tn.p[q];
+                hit = node_hit(p, o, R);
+                if (hit) {
+                    if (tn.child >= 0) {
+                        internal = true;
+                        c0 = tn.child;
+                        c1 = tn.child1;
+                    } else {
+                        lidv = -tn.child - 1;
+                        const LeafRec& lr = Q.leaves[lidv];
+                        key = lr.w0 + (o.t - lr.t0) * lr.mvw0;   // ReconstructionOur.cpp:384-385
+                    }
+                }
+            }
+            const uint64_t im = __ballot(hit && internal);
+            const uint64_t lm = __ballot(hit && !internal);
+            const int ni = __popcll(im), nlh = __popcll(lm);
+            if (base + 2 * ni > kStackCap || nl + nlh > kLeafCap) {
+                g.overflow = 1;
+                break;
+            }
+            if (hit && internal) {
+                const int r = (int)lanes_below(im);
+                m.stack[base + 2 * r] = c0;
+                m.stack[base + 2 * r + 1] = c1;
+            }
+            if (hit && !internal) {
+                const int r = nl + (int)lanes_below(lm);
+                m.lkey[r] = key;
+                m.lid[r] = lidv;
+            }
+            sp = base + 2 * ni;
+            nl += nlh;
+            wave_sync_mem();
+        }
+    }
+    if (g.overflow) return g;
+    st[ST_LEAF_NODES] += (unsigned long long)nl;
+    // --- front-to-back order: rank by (key, leaf id) ---
+    for (int j = lane; j < nl; j += 64) {
+        const float kj = m.lkey[j];
+        const int ij = m.lid[j];
+        int rank = 0;
+        for (int i = 0; i < nl; i++) {
+            const float ki = m.lkey[i];
+            const int ii = m.lid[i];
+            rank += (ki < kj || (!(kj < ki) && ii < ij)) ? 1 : 0;
+        }
+        m.sid[rank] = ij;
+    }
+    wave_sync_mem();
+    // --- per-leaf sample tests and surface assignment ---
+    const float RANGE = 1.0f / sqrtf(128.0f);
+    const float u0 = o.u - RANGE, v0 = o.v - RANGE, t0 = o.t - 0.5f * RANGE;
+    const float u1 = o.u + RANGE, v1 = o.v + RANGE, t1 = o.t + 0.5f * RANGE;
+    const float R2 = R * R;
+    int total = 0, nsurf = 0, sstart = 0;
+    int c_start = 0, c_size = 0, c_quad = 0, c_mini = -1;
+    float c_min = kFltMax;
+    unsigned long long in_leaves = 0;
+    for (int li = 0; li < nl; li++) {
+        const int lid = uni(m.sid[li]);
+        const LeafRec lr = Q.leaves[lid];
+        const int s0 = uni(lr.s0), s1 = uni(lr.s1);
+        in_leaves += (unsigned long long)(s1 - s0);
+        bool first = true;
+        for (int c = s0; c < s1; c += 64) {
+            const int k = c + lane;
+            bool kept = false;
+            float x = 0, y = 0, d2 = 0, dx = 0, dy = 0;
+            if (k < s1) {
+                reproject_to(Q.pts, k, o, Q.cc0, Q.cc1, x, y);
+                dx = x - o.x;
+                dy = y - o.y;
+                d2 = dx * dx + dy * dy;
+                kept = d2 <= R2;
+            }
+            const uint64_t km = __ballot(kept);
+            if (km == 0) continue;
+            if (first) {
+                first = false;
+                bool same = true;
+                const float* np = Q.leaf_b[lid].p;
+                for (int j0 = sstart; j0 < li && same; j0 += 64) {
+                    const int j = j0 + lane;
+                    bool ok = true;
+                    if (j < li) ok = same_surface(np, Q.leaf_b[m.sid[j]].p, u0, v0, t0, u1, v1, t1);
+                    same = __ballot(!ok) == 0;
+                }
+                if (!same || nsurf == 0) {
+                    if (nsurf > 0 && lane == 0) {
+                        m.f_start[nsurf - 1] = c_start;
+                        m.f_size[nsurf - 1] = c_size;
+                        m.f_quad[nsurf - 1] = c_quad;
+                        m.f_min[nsurf - 1] = c_min;
+                        m.f_mini[nsurf - 1] = c_mini;
+                    }
+                    sstart = li;
+                    nsurf++;
+                    c_start = total;
+                    c_size = 0;
+                    c_quad = 0;
+                    c_min = kFltMax;
+                    c_mini = -1;
+                }
+            }
+            const int cnt = __popcll(km);
+            if (total + cnt > kSampleCap / 2) {
+                g.overflow = 1;
+                return g;
+            }
+            const int rpos = (dx >= 0 ? XPOS : XNEG) | (dy >= 0 ? YPOS : YNEG);
+            const float dist = kept ? sqrtf(d2) : kFltMax;
+            if (kept) {
+                const int pos = total + (int)lanes_below(km);
+                m.sx[pos] = x;
+                m.sy[pos] = y;
+                m.sd[pos] = dist;
+                m.sidx[pos] = k;
+                m.srpos[pos] = rpos;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (__ballot(kept && (rpos & (XPOS | YPOS)) == q)) c_quad |= 1 << q;
+            // first occurrence of the chunk minimum (sequential strict-< rule)
+            float dm = dist;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const float od = __shfl_xor(dm, d, 64);
+                dm = od < dm ? od : dm;
+            }
+            const uint64_t am = __ballot(kept && dist == dm);
+            if (dm < c_min) {
+                const int first_lane = __ffsll((unsigned long long)am) - 1;
+                const int pos0 = total + (int)__popcll(km & ((1ull << first_lane) - 1ull));
+                c_min = dm;
+                c_mini = pos0 - c_start;
+            }
+            total += cnt;
+            c_size += cnt;
+        }
+    }
+    if (nsurf > 0 && lane == 0) {
+        m.f_start[nsurf - 1] = c_start;
+        m.f_size[nsurf - 1] = c_size;
+        m.f_quad[nsurf - 1] = c_quad;
+        m.f_min[nsurf - 1] = c_min;
+        m.f_mini[nsurf - 1] = c_mini;
+    }
+    wave_sync_mem();
+    st[ST_SAMPLES_IN_LEAVES] += in_leaves;
+    st[ST_FETCHED] += (unsigned long long)total;
+    g.total = total;
+    g.nsurf = nsurf;
+    g.used = total;
+    return g;
+}
+
+// Pair index p -> (a, b), a < b, pairs ordered b-major: (0,1),(0,2),(1,2),...
+__device__ __forceinline__ void pair_of(int p, int& a, int& b) {
+    int bb = (int)floorf((1.0f + sqrtf(1.0f + 8.0f * (float)p)) * 0.5f);
+    while (bb * (bb - 1) / 2 > p) bb--;
+    while ((bb + 1) * bb / 2 <= p) bb++;
+    b = bb;
+    a = p - bb * (bb - 1) / 2;
+}
+
+// Does any triangle of the surface's samples cover o?  Loops m like the
+// reference (ReconstructionOur.cpp:221-259); lanes take (k,l) pairs.
+__device__ bool triangle_search(const WaveMem& m, int start, int n, const OSample& o, float disp) {
+    const int lane = threadIdx.x & 63;
+    for (int mm = 0; mm < n - 2; mm++) {
+        const float r0x = m.sx[start + mm], r0y = m.sy[start + mm];
+        const int p0 = m.srpos[start + mm];
+        const int s = n - mm - 1;  // candidates for k,l
+        const int np = s * (s - 1) / 2;
+        for (int pb = 0; pb < np; pb += 64) {
+            const int p = pb + lane;
+            bool cov = false;
+            if (p < np) {
+                int a, b;
+                pair_of(p, a, b);
+                const int k = mm + 1 + a, l = mm + 1 + b;
+                const float r1x = m.sx[start + k], r1y = m.sy[start + k];
+                const float r2x = m.sx[start + l], r2y = m.sy[start + l];
+                const int rp = p0 | m.srpos[start + k] | m.srpos[start + l];
+                if (rp == (XPOS | XNEG | YPOS | YNEG)) {
+                    const float e01 = sqrtf((r0x - r1x) * (r0x - r1x) + (r0y - r1y) * (r0y - r1y));
+                    const float e02 = sqrtf((r0x - r2x) * (r0x - r2x) + (r0y - r2y) * (r0y - r2y));
+                    const float e12 = sqrtf((r1x - r2x) * (r1x - r2x) + (r1y - r2y) * (r1y - r2y));
+                    if (!(e01 > 2 * disp || e02 > 2 * disp || e12 > 2 * disp)) {
+                        const float d0 = sqrtf((r0x - o.x) * (r0x - o.x) + (r0y - o.y) * (r0y - o.y));
+                        const float d1 = sqrtf((r1x - o.x) * (r1x - o.x) + (r1y - o.y) * (r1y - o.y));
+                        const float d2 = sqrtf((r2x - o.x) * (r2x - o.x) + (r2y - o.y) * (r2y - o.y));
+                        if (!(d0 > disp && d1 > disp && d2 > disp))
+                            cov = tri_covers(r0x, r0y, r1x, r1y, r2x, r2y, o.x, o.y);
+                    }
+                }
+            }
+            if (__ballot(cov)) return true;
+        }
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const WaveMem m = wave_mem(Q.scratch + (size_t)wave * kWaveScratchWords);
+    unsigned long long st[kNumStats];
+#pragma unroll
+    for (int k = 0; k < kNumStats; k++) st[k] = 0;
+    const int M = Q.M;
+    for (;;) {
+        unsigned qi = 0;
+        if (lane == 0) qi = atomicAdd(Q.next_query, 1u);
+        qi = (unsigned)uni((int)qi);
+        if (qi >= Q.num_queries) break;
+        const unsigned pix = qi / (unsigned)M;
+        const int i = (int)(qi - pix * (unsigned)M);
+        const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+        const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
+        const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
+        OSample o;
+        o.x = tp[0] + (float)px;
+        o.y = tp[1] + (float)py;
+        o.u = tp[2];
+        o.v = tp[3];
+        o.t = tp[4];
+        st[ST_COUNT] += 1;
+
+        // Filterer::reconstruct: ReconstructionOur.cpp:50-159
+        float disp = dispersion_for_spp(Q.spp_pow2);
+        Gather g = gather(Q, m, o, disp, st);
+        int overflow = g.overflow;
+        const int n1 = g.total;
+        bool f2 = false;
+        if (!overflow) {
+            const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
+            f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
+        }
+        if (!overflow && f2) {
+            g = gather(Q, m, o, disp * 2, st);
+            overflow = g.overflow;
+            if (!overflow && g.total == 0) {
+                st[ST_AT_LEAST_ONE] += 1;
+                int nN = 0;
+                for (int k = 0; k < 3 && !nN && !overflow; k++) {
+                    disp *= 2.f;
+                    g = gather(Q, m, o, disp * 2, st);
+                    overflow = g.overflow;
+                    nN = g.total;
+                }
+            }
+        }
+        float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!overflow) {
+            st[ST_2R] += f2 ? 1 : 0;
+            st[ST_WITHIN_1R] += (unsigned long long)n1;
+            const int nsurf = g.nsurf;
+            int numSurfaces = nsurf;
+            int used = g.used;
+            // Tiny-surface merge: ReconstructionOur.cpp:137-156.  dst's list
+            // becomes dst ++ src (materialised at the end of the sample area);
+            // minIndex is copied as is (it then indexes the merged list).
+            if (f2) {
+                for (int si = 0; si < nsurf - 1; si++) {
+                    const int ssize = uni(m.f_size[si]);
+                    if (ssize < 4) {
+                        const int sst = uni(m.f_start[si]);
+                        const int dst = uni(m.f_start[si + 1]);
+                        const int dsize = uni(m.f_size[si + 1]);
+                        if (used + dsize + ssize > kSampleCap) {
+                            overflow = 1;
+                            break;
+                        }
+                        for (int j = lane; j < dsize + ssize; j += 64) {
+                            const int from = j < dsize ? dst + j : sst + (j - dsize);
+                            m.sx[used + j] = m.sx[from];
+                            m.sy[used + j] = m.sy[from];
+                            m.sd[used + j] = m.sd[from];
+                            m.sidx[used + j] = m.sidx[from];
+                            m.srpos[used + j] = m.srpos[from];
+                        }
+                        const float smin = unif(m.f_min[si]);
+                        const float dmin = unif(m.f_min[si + 1]);
+                        const int smini = uni(m.f_mini[si]);
+                        wave_sync_mem();
+                        if (lane == 0) {
+                            m.f_start[si + 1] = used;
+                            m.f_size[si + 1] = dsize + ssize;
+                            m.f_size[si] = 0;
+                            if (smin < dmin) {
+                                m.f_min[si + 1] = smin;
+                                m.f_mini[si + 1] = smini;
+                            }
+                        }
+                        wave_sync_mem();
+                        used += dsize + ssize;
+                        numSurfaces--;
+                    }
+                }
+            }
+            st[ST_SURFACES] += (unsigned long long)numSurfaces;
+            st[ST_MERGED] += (numSurfaces != nsurf) ? 1 : 0;
+
+            // Coverage and filtering, front to back: ReconstructionOur.cpp:165-302
+            bool found = false;
+            float cr = 0, cg = 0, cb = 0, ca = 0;
+            float nearest = kFltMax;
+            int nI0 = -1, nI1 = 1;
+            for (int si = 0; si < nsurf && !found && !overflow; si++) {
+                const int fs = uni(m.f_start[si]);
+                const int n = uni(m.f_size[si]);
+                if (n == 0) continue;
+                const float fmin = unif(m.f_min[si]);
+                const int fmini = uni(m.f_mini[si]);
+                if (fmin < nearest) {
+                    nearest = fmin;
+                    nI0 = si;
+                    nI1 = fmini;
+                }
+                if (!f2 || fmin < 0.f || si == nsurf - 1) found = true;
+                if (n >= 200) found = true;
+                if (!found) found = triangle_search(m, fs, n, o, disp);
+                if (found) {
+                    cr = cg = cb = ca = 0;
+                    for (int k = 0; k < n; k++) {
+                        const float rx = m.sx[fs + k], ry = m.sy[fs + k];
+                        const float dx = o.x - rx, dy = o.y - ry;
+                        const float wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
+                        if (wgt <= 0) continue;
+                        const int idx = m.sidx[fs + k];
+                        cr += wgt * Q.pts.r[idx];
+                        cg += wgt * Q.pts.g[idx];
+                        cb += wgt * Q.pts.b[idx];
+                        ca += wgt * 1.0f;
+                    }
+                    const float oow = rcp(ca);
+                    cr *= oow; cg *= oow; cb *= oow; ca *= oow;
+                    nI0 = si;
+                    nI1 = fmini;
+                }
+            }
+            if (!overflow && nI0 != -1) {
+                if (ca == 0) {
+                    const int fs = uni(m.f_start[nI0]);
+                    const int idx = uni(m.sidx[fs + nI1]);
+                    cr = Q.pts.r[idx];
+                    cg = Q.pts.g[idx];
+                    cb = Q.pts.b[idx];
+                    ca = 1.0f;
+                }
+                if (Q.lens_gauss) {
+                    // Gaussian(x, 0.5, 0): Reconstruction.hpp:378
+                    const float e = 2.718281828f;
+                    const float gu = powf(e, -((o.u - 0) * (o.u - 0)) / (2 * (0.5f * 0.5f)));
+                    const float gv = powf(e, -((o.v - 0) * (o.v - 0)) / (2 * (0.5f * 0.5f)));
+                    const float gg = gu * gv;
+                    cr *= gg; cg *= gg; cb *= gg; ca *= gg;
+                }
+                if (Q.time_gauss) {
+                    const float e = 2.718281828f;
+                    const float gg = powf(e, -((o.t - 0.5f) * (o.t - 0.5f)) / (2 * (0.25f * 0.25f)));
+                    cr *= gg; cg *= gg; cb *= gg; ca *= gg;
+                }
+                col = make_float4(cr, cg, cb, ca);
+            }
+        }
+        if (overflow) st[ST_OVERFLOW] += 1;
+        if (lane == 0) Q.qcol[qi] = col;
+        wave_sync_mem();
+    }
+    if (lane == 0 && Q.collect_stats) {
+#pragma unroll
+        for (int k = 0; k < kNumStats; k++)
+            if (st[k]) atomicAdd(&Q.stats[k], st[k]);
+    }
+    if (lane == 0 && !Q.collect_stats && st[ST_OVERFLOW]) atomicAdd(&Q.stats[ST_OVERFLOW], st[ST_OVERFLOW]);
+}
+
+// Pixel = sum of its output samples in order, normalised by alpha; red if
+// nothing was found (Reconstruction.cpp:665-719).
+__global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ qcol, int W, int H, int M,
+                                                    float* __restrict__ out) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= W * H) return;
+    float r = 0, g = 0, b = 0, a = 0;
+    for (int i = 0; i < M; i++) {
+        const float4 c = qcol[(size_t)p * M + i];
+        r += c.x; g += c.y; b += c.z; a += c.w;
+    }
+    if (a == 0) {
+        r = 1; g = 0; b = 0; a = 1;
+    }
+    const float rw = rcp(a);
+    float4 o = make_float4(r * rw, g * rw, b * rw, a * rw);
+    reinterpret_cast<float4*>(out)[p] = o;
+}
+
+}  // namespace
+
+hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
+    if (p.num_queries == 0) return hipSuccess;
+    const int blocks = p.num_waves / kWavesPerBlock;
+    k_query_general<<<blocks, kBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out, hipStream_t s) {
+    const int n = W * H;
+    if (n == 0) return hipSuccess;
+    k_combine<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(qcol, W, H, M, out);
+    return hipGetLastError();
+}
+
+}  // namespace tlr

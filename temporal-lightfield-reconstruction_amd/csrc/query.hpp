@@ -1,0 +1,49 @@
+// Per-output-sample reconstruction kernels (query.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tlr_device.hpp"
+
+namespace tlr {
+
+// Capacities of one wave's scratch (per query).  A query that exceeds them
+// is reported as an overflow and the call fails with TLR_E_CAPACITY.
+constexpr int kLeafCap = 4096;
+constexpr int kSampleCap = 32768;   // gathered samples + merged-surface copies
+constexpr int kStackCap = 8192;
+
+constexpr int kNumStats = 10;
+enum StatIdx {
+    ST_LEAF_NODES = 0, ST_SAMPLES_IN_LEAVES, ST_WITHIN_1R, ST_FETCHED, ST_SURFACES, ST_2R, ST_MERGED,
+    ST_AT_LEAST_ONE, ST_COUNT, ST_OVERFLOW
+};
+
+// Per-wave scratch words (int/float) for the general query kernel.
+constexpr size_t kWaveScratchWords =
+    (size_t)kStackCap + 3 * (size_t)kLeafCap + 5 * (size_t)kSampleCap + 5 * (size_t)kLeafCap;
+
+struct QueryParams {
+    int W, H, M;              // image size, output samples per pixel
+    int spp_pow2;             // input spp rounded up to a power of two
+    float cc0, cc1;           // adjusted CoC coefficients
+    int lens_gauss, time_gauss;
+    int collect_stats;
+    int n_nodes;              // traversal nodes (0: empty tree)
+    unsigned num_queries;
+    const float* pattern;     // 64*M templates: x,y,u,v,t
+    const TravNode* trav;
+    const LeafRec* leaves;
+    const Bounds* leaf_b;
+    Points pts;
+    float4* qcol;             // per query colour
+    unsigned long long* stats;
+    unsigned* next_query;     // work counter (zeroed before launch)
+    float* scratch;           // kWaveScratchWords per resident wave
+    int num_waves;
+};
+
+hipError_t launch_query_general(const QueryParams& p, hipStream_t s);
+hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
+
+}  // namespace tlr

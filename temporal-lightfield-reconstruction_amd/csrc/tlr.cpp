@@ -1,0 +1,612 @@
+// libtlr host side: context, device memory, build and reconstruct pipelines
+// behind the C ABI of include/tlr.h.
+//
+// TreeGather (Reconstruction.cpp:43-295) does its build eagerly in the
+// constructor and filters in reconstructDofMotion; tlr_build and
+// tlr_reconstruct are those two halves, running entirely on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tlr.h"
+#include "build.hpp"
+#include "pattern.hpp"
+#include "prims.hpp"
+#include "query.hpp"
+
+using namespace tlr;
+
+namespace {
+
+constexpr int kMaxGridSide = 4100;   // the bucketing grid is clamped to 4096 (+2)
+
+int bits_for(uint64_t n) {
+    int b = 0;
+    while (b < 32 && ((uint64_t)1 << b) < n) b++;
+    return b;
+}
+
+}  // namespace
+
+struct tlr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipDeviceProp_t prop;
+
+    // input
+    tlr_header h{};
+    int N = 0;
+    bool has_samples = false;
+    float* d_in = nullptr;          // 12 fields x N (x,y,w,u,v,t,r,g,b,mvx,mvy,mvw)
+    size_t d_in_cap = 0;
+
+    // build products (live until the next build)
+    DeviceArena arena;
+    bool built = false;
+    GridInfo* d_gi = nullptr;
+    GridInfo gi{};
+    int spp_pow2 = 1;
+    float cc0 = 0, cc1 = 0;
+    int nb = 0, nil = 0, L = 0, nv = 0;
+    float *rx = nullptr, *ry = nullptr, *rw = nullptr, *rt = nullptr;
+    uint32_t* cand = nullptr;
+    int4* ileaf_rect = nullptr;
+    int2* ileaf_oc = nullptr;
+    Points pts{};
+    LeafRec* leaves = nullptr;
+    Bounds* leaf_b = nullptr;
+    TravNode* trav = nullptr;
+
+    // reconstruct
+    DeviceArena qarena;
+    float* d_pattern = nullptr;
+    float4* qcol = nullptr;
+    size_t qcol_n = 0;
+    float* d_img = nullptr;
+    unsigned long long* d_stats = nullptr;
+    unsigned* d_counter = nullptr;
+    float* scratch = nullptr;
+    int num_waves = 0;
+    int last_M = 0;
+
+    hipEvent_t ev[8];
+    bool events = false;
+
+    tlr_status fail(tlr_status s, const char* fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return s;
+    }
+};
+
+#define HIPCHK(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t _e = (call);                                                                        \
+        if (_e != hipSuccess)                                                                          \
+            return (ctx)->fail(TLR_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                               __LINE__);                                                              \
+    } while (0)
+
+extern "C" {
+
+tlr_status tlr_create(tlr_ctx** out, int device) {
+    if (!out) return TLR_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return TLR_E_HIP;
+    if (device < 0 || device >= n) return TLR_E_INVALID;
+    tlr_ctx* c = new tlr_ctx;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&c->prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return TLR_E_HIP;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return TLR_E_HIP;
+        }
+    c->events = true;
+    if (hipMalloc(&c->d_gi, sizeof(GridInfo)) != hipSuccess ||
+        hipMalloc(&c->d_stats, sizeof(unsigned long long) * kNumStats) != hipSuccess ||
+        hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess) {
+        delete c;
+        return TLR_E_HIP;
+    }
+    *out = c;
+    return TLR_OK;
+}
+
+void tlr_destroy(tlr_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->arena.release();
+    c->qarena.release();
+    if (c->d_in) (void)hipFree(c->d_in);
+    if (c->d_gi) (void)hipFree(c->d_gi);
+    if (c->d_stats) (void)hipFree(c->d_stats);
+    if (c->d_counter) (void)hipFree(c->d_counter);
+    if (c->events)
+        for (auto& e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tlr_last_error(const tlr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static tlr_status check_header(tlr_ctx* c, const tlr_header* h) {
+    if (!h || h->width <= 0 || h->height <= 0 || h->spp <= 0)
+        return c->fail(TLR_E_INVALID, "invalid header");
+    const int64_t n = (int64_t)h->width * h->height * h->spp;
+    if (n >= (int64_t)1 << 31) return c->fail(TLR_E_INVALID, "sample count %lld exceeds 2^31", (long long)n);
+    return TLR_OK;
+}
+
+static tlr_status reserve_input(tlr_ctx* c, int N) {
+    const size_t need = (size_t)12 * N * sizeof(float);
+    if (need > c->d_in_cap) {
+        if (c->d_in) HIPCHK(c, hipFree(c->d_in));
+        c->d_in = nullptr;
+        c->d_in_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_in, need));
+        c->d_in_cap = need;
+    }
+    return TLR_OK;
+}
+
+// AoS records -> device SoA: the transpose runs on the host once per upload.
+tlr_status tlr_set_samples(tlr_ctx* c, const tlr_header* h, const float* rec, int64_t n) {
+    if (!c) return TLR_E_INVALID;
+    tlr_status s = check_header(c, h);
+    if (s != TLR_OK) return s;
+    if (!rec || n != (int64_t)h->width * h->height * h->spp)
+        return c->fail(TLR_E_INVALID, "record count does not match the header");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = (int)n;
+    s = reserve_input(c, N);
+    if (s != TLR_OK) return s;
+    static const int field[12] = {0, 1, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13};  // x,y,w,u,v,t,r,g,b,mv
+    std::vector<float> tmp((size_t)N);
+    for (int f = 0; f < 12; f++) {
+        const int src = field[f];
+        for (int k = 0; k < N; k++) tmp[k] = rec[(size_t)16 * k + src];
+        HIPCHK(c, hipMemcpy(c->d_in + (size_t)f * N, tmp.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    }
+    c->h = *h;
+    c->N = N;
+    c->has_samples = true;
+    c->built = false;
+    return TLR_OK;
+}
+
+tlr_status tlr_set_samples_soa(tlr_ctx* c, const tlr_header* h, const tlr_soa* soa, int on_device) {
+    if (!c) return TLR_E_INVALID;
+    tlr_status s = check_header(c, h);
+    if (s != TLR_OK) return s;
+    if (!soa) return c->fail(TLR_E_INVALID, "null soa");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = h->width * h->height * h->spp;
+    s = reserve_input(c, N);
+    if (s != TLR_OK) return s;
+    const float* src[12] = {soa->x, soa->y, soa->w, soa->u, soa->v, soa->t,
+                            soa->r, soa->g, soa->b, soa->mv_x, soa->mv_y, soa->mv_w};
+    for (int f = 0; f < 12; f++) {
+        if (!src[f]) return c->fail(TLR_E_INVALID, "null soa field %d", f);
+        HIPCHK(c, hipMemcpyAsync(c->d_in + (size_t)f * N, src[f], sizeof(float) * N,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h = *h;
+    c->N = N;
+    c->has_samples = true;
+    c->built = false;
+    return TLR_OK;
+}
+
+static InSoA in_view(tlr_ctx* c) {
+    const float* b = c->d_in;
+    const size_t N = (size_t)c->N;
+    InSoA in;
+    in.x = b; in.y = b + N; in.w = b + 2 * N; in.u = b + 3 * N; in.v = b + 4 * N; in.t = b + 5 * N;
+    in.r = b + 6 * N; in.g = b + 7 * N; in.b = b + 8 * N;
+    in.mvx = b + 9 * N; in.mvy = b + 10 * N; in.mvw = b + 11 * N;
+    return in;
+}
+
+static uint32_t read_u32(tlr_ctx* c, const uint32_t* d) {
+    uint32_t v = 0;
+    (void)hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    return v;
+}
+
+tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
+    if (!c) return TLR_E_INVALID;
+    if (!c->has_samples) return c->fail(TLR_E_STATE, "tlr_build before tlr_set_samples");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->built = false;
+    hipStream_t s = c->stream;
+    const int N = c->N, W = c->h.width, H = c->h.height;
+
+    // TreeGather::init preamble: Reconstruction.cpp:63-87
+    {
+        uint32_t v = (uint32_t)c->h.spp;
+        v--;
+        v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+        v++;
+        c->spp_pow2 = std::max(1, (int)v);
+        switch (c->spp_pow2) {
+            case 1: case 2: case 4: case 8: case 16: case 32: case 64: case 128: case 256: break;
+            default: return c->fail(TLR_E_UNSUPPORTED_SPP, "unsupported spp %d", c->h.spp);
+        }
+        float cx = c->h.coc_c0, cy = c->h.coc_c1;
+        float a = -cx;
+        float f = -cx * rcp(cy);
+        a *= aperture_adjust;
+        f *= focal_adjust;
+        c->cc0 = -a;
+        c->cc1 = a * rcp(f);
+    }
+
+    // ---- device memory for this build ----
+    const size_t cells_max = (size_t)kMaxGridSide * kMaxGridSide;
+    const size_t hist_words = radix_hist_words((size_t)N);
+    const size_t scan_words = scan_tmp_words(std::max<size_t>(std::max<size_t>((size_t)N, cells_max), hist_words));
+    size_t words = 0;
+    words += 4 * (size_t)N;           // rx ry rw rt
+    words += 4 * (size_t)N;           // radix ping-pong
+    words += 3 * (size_t)N;           // cand ct cw
+    words += 10 * (size_t)N;          // points
+    words += 14 * (size_t)N;          // per-sample bounds
+    words += 8 * (size_t)N;           // boxes
+    words += 2 * (size_t)N;           // start flags, leaf pos
+    words += 8 * (size_t)N;           // leaf records
+    words += 14 * (size_t)N;          // leaf bounds
+    words += 2 * (size_t)N;           // valid flag / pos
+    words += 4 * (size_t)N;           // lbvh keys ping-pong
+    words += 7 * (size_t)N;           // child0/1, parent(2N), range(2N), split owner
+    words += 14 * (size_t)N + (size_t)N;  // node bounds, flags
+    words += 32 * (size_t)N;          // traversal nodes (2N x 16 words)
+    words += (size_t)W * H;           // hits
+    words += 7 * cells_max + 2 * (size_t)kMaxGridSide + 16;  // counts, cell start, sat, owner(4), flags, y1
+    words += (size_t)N;               // initial-leaf idx (<= cells)
+    words += 6 * (size_t)N;           // initial leaf rect / offcnt
+    words += hist_words + scan_words + 4096;
+    const size_t bytes = words * 4 + 64 * 256;
+    HIPCHK(c, c->arena.reserve(bytes));
+    DeviceArena& A = c->arena;
+    A.reset();
+    auto need = [&](void* p) { return p != nullptr; };
+
+    c->rx = A.alloc<float>(N); c->ry = A.alloc<float>(N); c->rw = A.alloc<float>(N); c->rt = A.alloc<float>(N);
+    uint32_t* k0 = A.alloc<uint32_t>(N);
+    uint32_t* v0 = A.alloc<uint32_t>(N);
+    uint32_t* k1 = A.alloc<uint32_t>(N);
+    uint32_t* v1 = A.alloc<uint32_t>(N);
+    int* hits = A.alloc<int>((size_t)W * H);
+    uint32_t* hist = A.alloc<uint32_t>(hist_words);
+    uint32_t* stmp = A.alloc<uint32_t>(scan_words + 64);
+    if (!need(c->rt) || !need(v1) || !need(hits) || !need(stmp)) return c->fail(TLR_E_HIP, "arena exhausted");
+
+    // ---- reprojection + grid ----
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    GridInfo g0;
+    memset(&g0, 0, sizeof(g0));
+    g0.bbmin_key[0] = g0.bbmin_key[1] = 0x7FFFFFFF;
+    g0.bbmax_key[0] = g0.bbmax_key[1] = (int)0x80000000;
+    HIPCHK(c, hipMemcpyAsync(c->d_gi, &g0, sizeof(g0), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(hits, 0, sizeof(int) * (size_t)W * H, s));
+    const InSoA in = in_view(c);
+    launch_reproject(in, N, W, H, c->h.coc_c0, c->h.coc_c1, c->rx, c->ry, c->rw, c->rt, hits, c->d_gi, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&c->gi, c->d_gi, sizeof(GridInfo), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (c->gi.error == 1) return c->fail(TLR_E_INVALID, "%u samples reproject to non-finite positions", c->gi.num_bad);
+    if (c->gi.error == 3)
+        return c->fail(TLR_E_ALL_DISCARDED, "All samples were discarded in reprojection to (u,v,t)=0");
+    const int gw = c->gi.gw, gh = c->gi.gh;
+    if (gw <= 0 || gh <= 0 || gw > kMaxGridSide || gh > kMaxGridSide)
+        return c->fail(TLR_E_INVALID, "degenerate bucketing grid %dx%d", gw, gh);
+    const size_t cells = (size_t)gw * gh;
+
+    uint32_t* counts = A.alloc<uint32_t>(cells);
+    uint32_t* cell_start = A.alloc<uint32_t>(cells + 1);
+    uint32_t* sat = A.alloc<uint32_t>((size_t)(gw + 1) * (gh + 1));
+    int4* owner = A.alloc<int4>(cells);
+    uint32_t* own_flag = A.alloc<uint32_t>(cells);
+    int* own_y1 = A.alloc<int>(cells);
+    uint32_t* own_idx = A.alloc<uint32_t>(cells + 1);
+    if (!need(own_idx)) return c->fail(TLR_E_HIP, "arena exhausted (grid)");
+
+    // ---- bucketing: stable sort of (cell, input index) ----
+    HIPCHK(c, hipMemsetAsync(counts, 0, sizeof(uint32_t) * cells, s));
+    launch_cell_keys(c->rx, c->ry, c->rw, N, c->d_gi, k0, v0, counts, s);
+    HIPCHK(c, hipGetLastError());
+    bool first = true;
+    HIPCHK(c, radix_sort_pairs(k0, v0, k1, v1, (size_t)N, bits_for(cells + 1), hist, stmp, &first, s));
+    const uint32_t* skeys = first ? k0 : k1;
+    const uint32_t* svals = first ? v0 : v1;
+    HIPCHK(c, exclusive_scan_u32(counts, cell_start, cell_start + cells, cells, stmp, s));
+    const int nb = (int)read_u32(c, cell_start + cells);
+    c->nb = nb;
+
+    // ---- initial kd split ----
+    launch_sat(counts, gw, gh, sat, s);
+    launch_kd(sat, gw, gh, owner, own_flag, own_y1, s);
+    HIPCHK(c, exclusive_scan_u32(own_flag, own_idx, own_idx + cells, cells, stmp, s));
+    const int nil = (int)read_u32(c, own_idx + cells);
+    c->nil = nil;
+    c->ileaf_rect = A.alloc<int4>(std::max(nil, 1));
+    c->ileaf_oc = A.alloc<int2>(std::max(nil, 1));
+    c->cand = A.alloc<uint32_t>(std::max(nb, 1));
+    float* ct = A.alloc<float>(std::max(nb, 1));
+    float* cw = A.alloc<float>(std::max(nb, 1));
+    if (!need(cw)) return c->fail(TLR_E_HIP, "arena exhausted (candidates)");
+    launch_emit_initial(owner, own_flag, own_y1, own_idx, sat, gw, gh, c->ileaf_rect, c->ileaf_oc, s);
+    launch_place(skeys, svals, nb, gw, owner, sat, cell_start, c->rt, c->rw, c->cand, ct, cw, s);
+    launch_leaf_sort(c->ileaf_oc, nil, c->cand, ct, cw, s);
+    HIPCHK(c, hipGetLastError());
+
+    // ---- per-sample data in tree order ----
+    const int nb1 = std::max(nb, 1);
+    Points P;
+    P.x = A.alloc<float>(nb1); P.y = A.alloc<float>(nb1); P.w = A.alloc<float>(nb1); P.t = A.alloc<float>(nb1);
+    P.mvx = A.alloc<float>(nb1); P.mvy = A.alloc<float>(nb1); P.mvw = A.alloc<float>(nb1);
+    P.r = A.alloc<float>(nb1); P.g = A.alloc<float>(nb1); P.b = A.alloc<float>(nb1);
+    c->pts = P;
+    Bounds* sb = A.alloc<Bounds>(nb1);
+    float4* boxes = A.alloc<float4>(2 * (size_t)nb1);
+    uint32_t* sflag = A.alloc<uint32_t>(nb1 + 1);
+    uint32_t* lpos = A.alloc<uint32_t>(nb1 + 1);
+    if (!need(lpos)) return c->fail(TLR_E_HIP, "arena exhausted (points)");
+    launch_points(c->cand, nb, in, c->rx, c->ry, c->rw, c->rt, (float)W, (float)H, c->cc0, c->cc1, P, sb, boxes, s);
+    HIPCHK(c, hipMemsetAsync(sflag, 0, sizeof(uint32_t) * (nb1 + 1), s));
+    launch_surface_scan(c->ileaf_oc, nil, P.t, boxes, sflag, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, exclusive_scan_u32(sflag, lpos, lpos + nb, (size_t)nb, stmp, s));
+    const int L = nb > 0 ? (int)read_u32(c, lpos + nb) : 0;
+    c->L = L;
+    const int L1 = std::max(L, 1);
+    c->leaves = A.alloc<LeafRec>(L1);
+    c->leaf_b = A.alloc<Bounds>(L1);
+    uint32_t* vflag = A.alloc<uint32_t>(L1 + 1);
+    uint32_t* vpos = A.alloc<uint32_t>(L1 + 1);
+    if (!need(vpos)) return c->fail(TLR_E_HIP, "arena exhausted (leaves)");
+    launch_leaf_emit(sflag, lpos, nb, sb, P, c->leaves, c->leaf_b, vflag, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, exclusive_scan_u32(vflag, vpos, vpos + L, (size_t)L, stmp, s));
+    const int nv = L > 0 ? (int)read_u32(c, vpos + L) : 0;
+    c->nv = nv;
+
+    // ---- LBVH over valid leaves ----
+    const int nv1 = std::max(nv, 1);
+    uint32_t* codes = A.alloc<uint32_t>(nv1);
+    uint32_t* ids = A.alloc<uint32_t>(nv1);
+    uint32_t* codes2 = A.alloc<uint32_t>(nv1);
+    uint32_t* ids2 = A.alloc<uint32_t>(nv1);
+    int* ch0 = A.alloc<int>(nv1);
+    int* ch1 = A.alloc<int>(nv1);
+    int* parent = A.alloc<int>(2 * (size_t)nv1);
+    int2* range = A.alloc<int2>(nv1);
+    int* split_owner = A.alloc<int>(nv1);
+    Bounds* node_b = A.alloc<Bounds>(nv1);
+    int* bflags = A.alloc<int>(nv1);
+    c->trav = A.alloc<TravNode>(2 * (size_t)nv1);
+    if (!need(c->trav)) return c->fail(TLR_E_HIP, "arena exhausted (lbvh)");
+    if (nv > 0) {
+        launch_lbvh_keys(vflag, vpos, L, c->leaf_b, c->d_gi, codes, ids, s);
+        bool f2 = true;
+        HIPCHK(c, radix_sort_pairs(codes, ids, codes2, ids2, (size_t)nv, 32, hist, stmp, &f2, s));
+        const uint32_t* sc = f2 ? codes : codes2;
+        const uint32_t* si = f2 ? ids : ids2;
+        HIPCHK(c, hipMemsetAsync(parent, 0xFF, sizeof(int) * (2 * (size_t)nv), s));
+        HIPCHK(c, hipMemsetAsync(bflags, 0, sizeof(int) * nv1, s));
+        launch_lbvh_nodes(sc, nv, ch0, ch1, parent, range, split_owner, s);
+        launch_lbvh_bounds(si, nv, c->leaf_b, ch0, ch1, parent, node_b, bflags, s);
+        launch_lbvh_trav(si, nv, c->leaf_b, node_b, ch0, ch1, range, split_owner, c->trav, s);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipMemcpy(&c->gi, c->d_gi, sizeof(GridInfo), hipMemcpyDeviceToHost));
+    c->built = true;
+    return TLR_OK;
+}
+
+tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, int out_on_device, tlr_stats* stats) {
+    if (!c || !p) return TLR_E_INVALID;
+    if (!c->built) return c->fail(TLR_E_STATE, "tlr_reconstruct before tlr_build");
+    if (p->out_spp <= 0 && !p->override_uvt) return c->fail(TLR_E_INVALID, "out_spp must be positive");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    std::vector<float> pat;
+    const int M = make_output_pattern(p->out_spp, p->override_uvt != 0, p->uvt, pat);
+    const int W = c->h.width, H = c->h.height;
+    const size_t nq = (size_t)W * H * M;
+    if (nq >= ((size_t)1 << 32)) return c->fail(TLR_E_INVALID, "too many output samples");
+
+    // waves resident across the device: CUs x 8
+    const int waves = std::max(4, c->prop.multiProcessorCount * 8);
+    const size_t qbytes = pat.size() * 4 + 256 + nq * sizeof(float4) + 256 + (size_t)W * H * 16 + 256 +
+                          (size_t)waves * kWaveScratchWords * 4 + 4096;
+    HIPCHK(c, c->qarena.reserve(qbytes));
+    c->qarena.reset();
+    c->d_pattern = c->qarena.alloc<float>(pat.size());
+    c->qcol = c->qarena.alloc<float4>(nq);
+    c->d_img = c->qarena.alloc<float>((size_t)W * H * 4);
+    c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
+    if (!c->scratch) return c->fail(TLR_E_HIP, "query arena exhausted");
+    c->qcol_n = nq;
+    c->num_waves = waves;
+    c->last_M = M;
+    HIPCHK(c, hipMemcpyAsync(c->d_pattern, pat.data(), pat.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
+    HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
+
+    QueryParams Q;
+    Q.W = W; Q.H = H; Q.M = M;
+    Q.spp_pow2 = c->spp_pow2;
+    Q.cc0 = c->cc0; Q.cc1 = c->cc1;
+    Q.lens_gauss = p->lens_gaussian;
+    Q.time_gauss = p->time_gaussian;
+    Q.collect_stats = p->collect_stats;
+    Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
+    Q.num_queries = (unsigned)nq;
+    Q.pattern = c->d_pattern;
+    Q.trav = c->trav;
+    Q.leaves = c->leaves;
+    Q.leaf_b = c->leaf_b;
+    Q.pts = c->pts;
+    Q.qcol = c->qcol;
+    Q.stats = c->d_stats;
+    Q.next_query = c->d_counter;
+    Q.scratch = c->scratch;
+    Q.num_waves = waves;
+
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    HIPCHK(c, launch_query_general(Q, s));
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    float* dst = out_on_device ? out : c->d_img;
+    HIPCHK(c, launch_combine(c->qcol, W, H, M, dst, s));
+    HIPCHK(c, hipEventRecord(c->ev[4], s));
+    if (!out_on_device && out)
+        HIPCHK(c, hipMemcpyAsync(out, c->d_img, (size_t)W * H * 16, hipMemcpyDeviceToHost, s));
+    unsigned long long hs[kNumStats];
+    HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->leaf_nodes = (double)hs[ST_LEAF_NODES];
+        stats->samples_in_leaf_nodes = (double)hs[ST_SAMPLES_IN_LEAVES];
+        stats->samples_within_1r = (double)hs[ST_WITHIN_1R];
+        stats->samples_fetched = (double)hs[ST_FETCHED];
+        stats->surfaces = (double)hs[ST_SURFACES];
+        stats->num_2r = (double)hs[ST_2R];
+        stats->surfaces_merged = (double)hs[ST_MERGED];
+        stats->at_least_one = (double)hs[ST_AT_LEAST_ONE];
+        stats->count = (double)hs[ST_COUNT];
+        stats->overflow = (double)hs[ST_OVERFLOW];
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;
+        stats->total_ms = stats->build_ms + stats->query_ms + stats->combine_ms;
+        stats->num_leaves = c->L;
+        stats->num_samples = c->nb;
+    }
+    if (hs[ST_OVERFLOW])
+        return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",
+                       (unsigned long long)hs[ST_OVERFLOW]);
+    return TLR_OK;
+}
+
+tlr_status tlr_sync(tlr_ctx* c) {
+    if (!c) return TLR_E_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TLR_OK;
+}
+
+tlr_status tlr_get_build_info(tlr_ctx* c, tlr_build_info* o) {
+    if (!c || !o) return TLR_E_INVALID;
+    if (!c->built) return c->fail(TLR_E_STATE, "not built");
+    memset(o, 0, sizeof(*o));
+    o->num_input = c->N;
+    o->num_kept = (int)c->gi.num_kept;
+    o->num_bucketed = c->nb;
+    o->spp_pow2 = c->spp_pow2;
+    o->coc_adj[0] = c->cc0;
+    o->coc_adj[1] = c->cc1;
+    o->bbmin[0] = c->gi.bbmin[0]; o->bbmin[1] = c->gi.bbmin[1];
+    o->bbmax[0] = c->gi.bbmax[0]; o->bbmax[1] = c->gi.bbmax[1];
+    o->scale = c->gi.scale;
+    o->bbmin_int[0] = c->gi.bbmin_i[0]; o->bbmin_int[1] = c->gi.bbmin_i[1];
+    o->grid_w = c->gi.gw;
+    o->grid_h = c->gi.gh;
+    o->max_hits = c->gi.max_hits;
+    o->num_initial_leaves = c->nil;
+    o->num_leaves = c->L;
+    o->num_valid_leaves = c->nv;
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_reprojected(tlr_ctx* c, float* x, float* y, float* w, float* t) {
+    if (!c || !c->built) return TLR_E_STATE;
+    const size_t n = (size_t)c->N * 4;
+    if (x) HIPCHK(c, hipMemcpy(x, c->rx, n, hipMemcpyDeviceToHost));
+    if (y) HIPCHK(c, hipMemcpy(y, c->ry, n, hipMemcpyDeviceToHost));
+    if (w) HIPCHK(c, hipMemcpy(w, c->rw, n, hipMemcpyDeviceToHost));
+    if (t) HIPCHK(c, hipMemcpy(t, c->rt, n, hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_layout(tlr_ctx* c, int32_t* src) {
+    if (!c || !c->built || !src) return TLR_E_STATE;
+    if (c->nb) HIPCHK(c, hipMemcpy(src, c->cand, (size_t)c->nb * 4, hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_initial_leaves(tlr_ctx* c, int32_t* out6) {
+    if (!c || !c->built || !out6) return TLR_E_STATE;
+    std::vector<int4> r(c->nil);
+    std::vector<int2> oc(c->nil);
+    if (c->nil) {
+        HIPCHK(c, hipMemcpy(r.data(), c->ileaf_rect, sizeof(int4) * c->nil, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(oc.data(), c->ileaf_oc, sizeof(int2) * c->nil, hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < c->nil; k++) {
+        out6[6 * k + 0] = r[k].x; out6[6 * k + 1] = r[k].y;
+        out6[6 * k + 2] = r[k].z; out6[6 * k + 3] = r[k].w;
+        out6[6 * k + 4] = oc[k].x; out6[6 * k + 5] = oc[k].y;
+    }
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_leaves(tlr_ctx* c, int32_t* s01, float* b14) {
+    if (!c || !c->built) return TLR_E_STATE;
+    std::vector<LeafRec> lr(c->L);
+    std::vector<Bounds> lb(c->L);
+    if (c->L) {
+        HIPCHK(c, hipMemcpy(lr.data(), c->leaves, sizeof(LeafRec) * c->L, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(lb.data(), c->leaf_b, sizeof(Bounds) * c->L, hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < c->L; k++) {
+        if (s01) {
+            s01[2 * k] = lr[k].s0;
+            s01[2 * k + 1] = lr[k].s1;
+        }
+        if (b14) {
+            memcpy(b14 + 14 * (size_t)k, lb[k].p, 12 * 4);
+            b14[14 * (size_t)k + 12] = lb[k].ts0;
+            b14[14 * (size_t)k + 13] = lb[k].ts1;
+        }
+    }
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_query_colors(tlr_ctx* c, float* out4) {
+    if (!c || !c->qcol || !out4) return TLR_E_STATE;
+    HIPCHK(c, hipMemcpy(out4, c->qcol, c->qcol_n * sizeof(float4), hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+int tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, float* out5) {
+    std::vector<float> pat;
+    const float zero[3] = {0, 0, 0};
+    const int M = make_output_pattern(out_spp, override_uvt != 0, uvt ? uvt : zero, pat);
+    if (out5) memcpy(out5, pat.data(), pat.size() * 4);
+    return M;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Device-side data layout of one reconstruction context (all in HBM).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tlr_math.hpp"
+
+namespace tlr {
+
+// Input sample buffer, SoA, indexed (y*W+x)*spp+i: the fields of
+// UVTSampleBuffer the plain path reads (SampleBuffer.hpp:116-129,177-180).
+struct InSoA {
+    const float *x, *y, *w, *u, *v, *t, *r, *g, *b, *mvx, *mvy, *mvw;
+};
+
+// Scalars produced on the device during a build.
+struct GridInfo {
+    int bbmin_key[2];        // order-preserving int encodings of the bbox
+    int bbmax_key[2];
+    int max_hits;
+    unsigned num_kept;
+    unsigned num_bad;        // reprojected to a non-finite position
+    float bbmin[2], bbmax[2];
+    float scale;
+    int bbmin_i[2];
+    int gw, gh;
+    unsigned num_bucketed;
+    int error;               // tlr_status of the device-side checks
+    int pad[3];
+};
+
+// Tree sample layout (Node::s0..s1 ranges index it): SoA of the reprojected
+// samples in leaf order.  40 bytes per sample.
+struct Points {
+    float *x, *y, *w, *t, *mvx, *mvy, *mvw, *r, *g, *b;
+};
+
+// Leaf record, indexed by leaf id (ids follow the sample layout).
+struct LeafRec {
+    int s0, s1;
+    float w0, t0;            // w, t, mv.w of the leaf's first sample (the
+    float mvw0;              // query-time key of ReconstructionOur.cpp:384-385)
+    int valid;
+    float pad[2];
+};
+
+// Traversal node: 12 plane coefficients + children / leaf id + escape link.
+// Internal nodes [0, n-1), leaf slots [n-1, 2n-1).
+struct __align__(16) TravNode {
+    float p[12];
+    int child;               // >= 0: first child node index; < 0: -(leaf id + 1)
+    int escape;              // next node when this subtree is done; -1 = end
+    int child1;              // second child of an internal node
+    int pad;
+};
+
+TLR_HD int order_key(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
+TLR_HD float order_val(int k) {
+    int i = k >= 0 ? k : (k ^ 0x7FFFFFFF);
+    return __int_as_float(i);
+}
+
+}  // namespace tlr
