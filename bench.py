@@ -1,0 +1,164 @@
+"""Benchmark of the reconstruction hot path on one MI355X.
+
+One step = tlr_build (reprojection, bucketing, leaves, LBVH) + tlr_reconstruct
+(all output samples, per-pixel combine) over a synthetic sample buffer that is
+already resident in HBM.  Prints one JSON line.
+
+Workload (default): 960x540, 16 input spp (8,294,400 records), 16 output
+samples per pixel, scene type 2 (50 quads, defocus C0=-15.24 C1=19.05,
+perspective motion).  --config 3 selects 1920x1080 / 400 quads + 5000
+occluders / 32 output spp.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as ge  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    2: dict(preset=2, W=960, H=540, spp=16, out_spp=16,
+            name="960x540 16spp in, 16 out-spp, 50 quads, defocus+perspective motion"),
+    3: dict(preset=3, W=1920, H=1080, spp=16, out_spp=32,
+            name="1920x1080 16spp in, 32 out-spp, 400 quads + 5000 occluders, strong defocus, mixed motion"),
+    1: dict(preset=1, W=128, H=128, spp=4, out_spp=16,
+            name="128x128 4spp in, 16 out-spp, 3 quads, defocus only"),
+}
+
+
+def algorithmic_bytes(st, n_in, W, H):
+    """Bytes the reference algorithm must move (oracle Stats counters):
+    inputs 48 B/sample, tree points written+read 2x40 B/sample, leaves 2x64 B,
+    per query 64 B per leaf visited + 28 B per sample tested + 12 B per sample
+    fetched, output 16 B/pixel."""
+    b_in = 48.0 * n_in
+    b_build = 80.0 * st["num_samples"] + 128.0 * st["num_leaves"]
+    b_query = 64.0 * st["leaf_nodes"] + 28.0 * st["samples_in_leaf_nodes"] + 12.0 * st["samples_fetched"]
+    b_out = 16.0 * W * H
+    return b_in, b_build, b_query, b_out
+
+
+def read_traffic(kernel, workload_key):
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload_key))
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(cfg, sbuf, sample_rows):
+    orc = ge.load_oracle()
+    W, H = cfg["W"], cfg["H"]
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    o = orc.Oracle(W, H, cfg["spp"], sbuf.coc[0], sbuf.coc[1], sbuf.records)
+    t0 = time.perf_counter()
+    o.build(threads=threads)
+    t_build = time.perf_counter() - t0
+    stride = max(1, H // sample_rows)
+    t0 = time.perf_counter()
+    o.reconstruct(out_spp=cfg["out_spp"], threads=threads, row_stride=stride)
+    t_filter = time.perf_counter() - t0
+    rows = (H + stride - 1) // stride
+    est = t_build + t_filter * (H / rows)
+    o.close()
+    return {"value": W * H / est / 1e6, "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": "full build (%.2f s) + filter of every %d-th scanline (%d rows, %.2f s), "
+                      "frame time extrapolated to %.2f s" % (t_build, stride, rows, t_filter, est)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--seed", type=int, default=2011)
+    ap.add_argument("--cpu-rows", type=int, default=16, help="scanlines in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg = CONFIGS[args.config]
+    tlr = ge.load_package()
+    W, H, spp, M = cfg["W"], cfg["H"], cfg["spp"], cfg["out_spp"]
+    print("generating scene %d (%dx%d, %d spp)" % (args.config, W, H, spp), file=sys.stderr, flush=True)
+    sbuf = tlr.make_scene(cfg["preset"], W, H, spp, seed=args.seed)
+    n_in = W * H * spp
+    ctx = tlr.Ctx(0)
+    ctx.set_samples(sbuf)  # inputs resident in HBM before timing
+    img = np.empty((H, W, 4), np.float32)
+
+    def step():
+        ctx.build()
+        return ctx.reconstruct(out_spp=M, collect_stats=True, out=img)[1]
+
+    for i in range(args.warmup):
+        st = step()
+        print("warmup %d: build %.2f ms query %.2f ms" % (i, st["build_ms"], st["query_ms"]),
+              file=sys.stderr, flush=True)
+    ctx.build()
+    tlr.lib().tlr_sync(ctx.h)
+    sts = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sts.append(step())
+    tlr.lib().tlr_sync(ctx.h)
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    st = sts[-1]
+    q_ms = float(np.mean([s["query_ms"] for s in sts]))
+    b_ms = float(np.mean([s["build_ms"] for s in sts]))
+    b_in, b_build, b_query, b_out = algorithmic_bytes(st, n_in, W, H)
+    achieved = b_query / (q_ms * 1e-3) / 1e9
+    wk = "cfg%d" % args.config
+    traffic = read_traffic("k_query_general", wk)
+    res = {
+        "metric": "reconstructed Mpix/s @%d out-spp, 1 GPU (plain path)" % M,
+        "value": W * H / dt / 1e6,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded quad scene, generated on the host, uploaded before timing)",
+        "config": {"workload": cfg["name"], "width": W, "height": H, "input_spp": spp, "out_spp": M,
+                   "records": n_in, "seed": args.seed},
+        "output_samples_per_s": W * H * M / dt,
+        "device_ms": {"build": b_ms, "query": q_ms, "combine": float(np.mean([s["combine_ms"] for s in sts]))},
+        "stats_per_output_sample": {k: st[k] / st["count"] for k in (
+            "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched", "surfaces",
+            "num_2r", "surfaces_merged", "at_least_one")},
+        "roofline": {"bound": "hbm", "kernel": "k_query_general", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": b_query},
+        "path_roofline": {"algorithmic_bytes": b_in + b_build + b_query + b_out,
+                          "roofline_ms": (b_in + b_build + b_query + b_out) / (HBM_PEAK_GBS * 1e9) * 1e3,
+                          "frac": (b_in + b_build + b_query + b_out) / (HBM_PEAK_GBS * 1e9) / dt},
+    }
+    if not args.no_cpu:
+        print("cpu baseline ...", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(cfg, sbuf, args.cpu_rows)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -55,8 +55,10 @@ typedef struct tlr_recon_params {
     int   lens_gaussian;       /* CameraParams::lensFilter == LENS_GAUSSIAN         */
     int   time_gaussian;       /* CameraParams::timeFilter == TIME_GAUSSIAN         */
     int   collect_stats;       /* fill the Stats counters (slower)                  */
-    int   gate;                /* 0: plain method; 1: learned gate (tlr_gate_*)     */
-    int   reserved[4];
+    int   gate;                /* 0: plain method; 1: learned gate                  */
+    int   path;                /* 0: fast 1R kernel + general kernel for the rest;
+                                  1: general kernel for every output sample        */
+    int   reserved[3];
 } tlr_recon_params;
 
 /* TreeGather::Stats (Reconstruction.hpp:134-147) as sums over output samples,
@@ -66,6 +68,8 @@ typedef struct tlr_stats {
     double surfaces, num_2r, surfaces_merged, at_least_one, count, overflow;
     double build_ms, query_ms, combine_ms, total_ms;
     double num_leaves, num_samples;
+    double fast_ms, general_ms;  /* split of query_ms: 1R lane kernel, general kernel */
+    double general_queries;      /* output samples handled by the general kernel     */
 } tlr_stats;
 
 typedef struct tlr_build_info {

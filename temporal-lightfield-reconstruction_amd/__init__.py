@@ -52,14 +52,16 @@ class Soa(C.Structure):
 class ReconParams(C.Structure):
     _fields_ = [("out_spp", C.c_int), ("override_uvt", C.c_int), ("uvt", C.c_float * 3),
                 ("lens_gaussian", C.c_int), ("time_gaussian", C.c_int),
-                ("collect_stats", C.c_int), ("gate", C.c_int), ("reserved", C.c_int * 4)]
+                ("collect_stats", C.c_int), ("gate", C.c_int), ("path", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
         "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count", "overflow",
-        "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples")]
+        "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
+        "fast_ms", "general_ms", "general_queries")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -245,7 +247,7 @@ class Ctx:
         self._chk(lib().tlr_build(self.h, aperture_adjust, focal_adjust))
 
     def reconstruct(self, out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,
-                    collect_stats=False, gate=False, out=None):
+                    collect_stats=False, gate=False, path=0, out=None):
         p = ReconParams()
         p.out_spp = out_spp
         p.override_uvt = 1 if override_uvt is not None else 0
@@ -256,6 +258,7 @@ class Ctx:
         p.time_gaussian = int(time_gaussian)
         p.collect_stats = int(collect_stats)
         p.gate = int(gate)
+        p.path = int(path)
         img = out if out is not None else np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
         st = Stats()
         self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img), 0, C.byref(st)))

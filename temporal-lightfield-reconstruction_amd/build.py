@@ -52,8 +52,17 @@ def build_tlr():
     return out
 
 
+def build_facade_example():
+    """C++ caller of the facade (tests/cpp/facade_example.cpp), host compiler only."""
+    out = os.path.join(LIB, "facade_example")
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+          "-o", out, os.path.join(ROOT, "tests", "cpp", "facade_example.cpp"),
+          "-L", LIB, "-ltlr", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_all():
-    return [build_scene(), build_tlr()]
+    return [build_scene(), build_tlr(), build_facade_example()]
 
 
 if __name__ == "__main__":

@@ -401,9 +401,9 @@ __global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ 
     const uint32_t i = cand[k];
     const float x = rx[i], y = ry[i], w = rw[i], t = rt[i];
     const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
-    pts.x[k] = x; pts.y[k] = y; pts.w[k] = w; pts.t[k] = t;
-    pts.mvx[k] = mvx; pts.mvy[k] = mvy; pts.mvw[k] = mvw;
-    pts.r[k] = in.r[i]; pts.g[k] = in.g[i]; pts.b[k] = in.b[i];
+    pts.pa[k] = make_float4(x, y, w, t);
+    pts.pb[k] = make_float4(mvx, mvy, mvw, 0.0f);
+    pts.pc[k] = make_float4(in.r[i], in.g[i], in.b[i], 1.0f);
 
     Bounds b;
     const float px = x * w - t * mvx, py = y * w - t * mvy, pw = w - t * mvw;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ 
 // Sequential surface scan of one initial leaf (ReconstructionTreeBuilder.cpp:
 // 222-340, keeping the cocboundst[0] typo at :334,336).  Marks leaf starts.
 __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict__ offcnt, int nil,
-                                                         const float* __restrict__ pt,
+                                                         const float4* __restrict__ pa,
                                                          const float4* __restrict__ boxes,
                                                          uint32_t* __restrict__ start_flag) {
     const int li = blockIdx.x * kBlock + threadIdx.x;
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict_
     const int base = oc.x, n = oc.y;
     if (n <= 0) return;
     const float4* B = boxes + 2 * (size_t)base;
-    const float* T = pt + base;
+    const float4* T = pa + base;
     start_flag[base] = 1u;
     int sb = 0;
     bool reset = true;
@@ -457,10 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict_
             mn0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
             mn1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
             mx0 = mn0; mx1 = mn1;
-            st = T[sb];
+            st = T[sb].w;
             reset = false;
         }
-        if (st != T[k]) {
+        if (st != T[k].w) {
             sb = k;
             start_flag[base + k] = 1u;
             reset = true;
@@ -524,9 +524,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit(const uint32_t* __restrict
     LeafRec r;
     r.s0 = k;
     r.s1 = e;
-    r.w0 = pts.w[k];
-    r.t0 = pts.t[k];
-    r.mvw0 = pts.mvw[k];
+    const float4 a = pts.pa[k];
+    r.w0 = a.z;
+    r.t0 = a.w;
+    r.mvw0 = pts.pb[k].z;
     r.valid = bounds_valid(acc) ? 1 : 0;
     r.pad[0] = r.pad[1] = 0.0f;
     leaves[id] = r;
@@ -725,9 +726,9 @@ void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* r
     if (nb > 0) k_points<<<TLR_GRID(nb), 0, s>>>(cand, nb, in, rx, ry, rw, rt, W, H, cc0, cc1, pts, bounds, boxes);
 }
 
-void launch_surface_scan(const int2* offcnt, int nil, const float* pt, const float4* boxes, uint32_t* flags,
+void launch_surface_scan(const int2* offcnt, int nil, const float4* pa, const float4* boxes, uint32_t* flags,
                          hipStream_t s) {
-    if (nil > 0) k_surface_scan<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, pt, boxes, flags);
+    if (nil > 0) k_surface_scan<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, pa, boxes, flags);
 }
 
 void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,

@@ -24,7 +24,7 @@ void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, fl
 void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
                    const float* rw, const float* rt, float W, float H, float cc0, float cc1, const Points& pts,
                    Bounds* bounds, float4* boxes, hipStream_t s);
-void launch_surface_scan(const int2* offcnt, int nil, const float* pt, const float4* boxes, uint32_t* flags,
+void launch_surface_scan(const int2* offcnt, int nil, const float4* pa, const float4* boxes, uint32_t* flags,
                          hipStream_t s);
 void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
                       const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s);

@@ -85,8 +85,9 @@ __device__ __forceinline__ bool node_hit(const float* p, const OSample& o, float
 // ReconstructionOur.cpp:438-451
 __device__ __forceinline__ void reproject_to(const Points& P, int k, const OSample& o, float cc0, float cc1,
                                              float& x, float& y) {
-    const float sx = P.x[k], sy = P.y[k], sw = P.w[k], st = P.t[k];
-    const float mvx = P.mvx[k], mvy = P.mvy[k], mvw = P.mvw[k];
+    const float4 a = P.pa[k], b = P.pb[k];
+    const float sx = a.x, sy = a.y, sw = a.z, st = a.w;
+    const float mvx = b.x, mvy = b.y, mvw = b.z;
     if (mvx != 0.0f || mvy != 0.0f || mvw != 0.0f) {
         const float dt = o.t - st;
         const float Px = sx * sw + dt * mvx;
@@ -346,11 +347,13 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
 #pragma unroll
     for (int k = 0; k < kNumStats; k++) st[k] = 0;
     const int M = Q.M;
+    const unsigned nq = Q.qlist ? *Q.qcount : Q.num_queries;
     for (;;) {
         unsigned qi = 0;
         if (lane == 0) qi = atomicAdd(Q.next_query, 1u);
         qi = (unsigned)uni((int)qi);
-        if (qi >= Q.num_queries) break;
+        if (qi >= nq) break;
+        if (Q.qlist) qi = (unsigned)uni((int)Q.qlist[qi]);
         const unsigned pix = qi / (unsigned)M;
         const int i = (int)(qi - pix * (unsigned)M);
         const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
@@ -465,10 +468,10 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
                         const float dx = o.x - rx, dy = o.y - ry;
                         const float wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
                         if (wgt <= 0) continue;
-                        const int idx = m.sidx[fs + k];
-                        cr += wgt * Q.pts.r[idx];
-                        cg += wgt * Q.pts.g[idx];
-                        cb += wgt * Q.pts.b[idx];
+                        const float4 c = Q.pts.pc[m.sidx[fs + k]];
+                        cr += wgt * c.x;
+                        cg += wgt * c.y;
+                        cb += wgt * c.z;
                         ca += wgt * 1.0f;
                     }
                     const float oow = rcp(ca);
@@ -480,10 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
             if (!overflow && nI0 != -1) {
                 if (ca == 0) {
                     const int fs = uni(m.f_start[nI0]);
-                    const int idx = uni(m.sidx[fs + nI1]);
-                    cr = Q.pts.r[idx];
-                    cg = Q.pts.g[idx];
-                    cb = Q.pts.b[idx];
+                    const float4 c = Q.pts.pc[uni(m.sidx[fs + nI1])];
+                    cr = c.x;
+                    cg = c.y;
+                    cb = c.z;
                     ca = 1.0f;
                 }
                 if (Q.lens_gauss) {
@@ -514,6 +517,194 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
     if (lane == 0 && !Q.collect_stats && st[ST_OVERFLOW]) atomicAdd(&Q.stats[ST_OVERFLOW], st[ST_OVERFLOW]);
 }
 
+// ---------------------------------------------------------------------------
+// Fast path: one lane per output sample at 1R (collectInputSamples2 with
+// R = dispersion, ReconstructionOur.cpp:94).  The lane walks the LBVH
+// stacklessly (escape links), keeps its leaves sorted by (key, leaf id) in
+// LDS, then streams their samples in that order, assigning surfaces exactly
+// like the general kernel.  Surface 0's tent-weighted sum, quadrant mask and
+// nearest sample are accumulated on the fly in sample order.  The query is
+// finished here when the reference would not fetch 2R
+// (ReconstructionOur.cpp:99): one surface, or a second surface behind a
+// first one that covers all four quadrants; then the found surface is
+// surface 0 (:204) and its filtered colour is the result (:268-297).  Every
+// other query (no surface, a 2R fetch, or more than kFastLeafCap leaves) is
+// appended to fail_list for the general kernel.  STATS keeps walking all
+// leaves to count the reference's Stats; otherwise the lane stops as soon as
+// the second surface starts.
+// ---------------------------------------------------------------------------
+constexpr int kFastBlock = 128;
+
+template <bool STATS>
+__global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
+    __shared__ float lkey[kFastLeafCap][kFastBlock];
+    __shared__ int lid[kFastLeafCap][kFastBlock];
+    const int tid = threadIdx.x;
+    const unsigned q = blockIdx.x * kFastBlock + tid;
+    const bool active = q < Q.num_queries;
+    bool fail = false;
+    unsigned long long s_leaves = 0, s_in_leaves = 0, s_kept = 0, s_surf = 0;
+    if (active) {
+        const int M = Q.M;
+        const unsigned pix = q / (unsigned)M;
+        const int i = (int)(q - pix * (unsigned)M);
+        const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+        const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
+        const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
+        OSample o;
+        o.x = tp[0] + (float)px;
+        o.y = tp[1] + (float)py;
+        o.u = tp[2];
+        o.v = tp[3];
+        o.t = tp[4];
+        const float disp = dispersion_for_spp(Q.spp_pow2);
+        const float R = disp;
+        // --- stackless traversal ---
+        int n = 0;
+        int node = Q.n_nodes > 0 ? 0 : -1;
+        while (node >= 0) {
+            const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
+            const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
+            const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+            const int child = __float_as_int(d.x), escape = __float_as_int(d.y);
+            if (node_hit(p, o, R)) {
+                if (child >= 0) {
+                    node = child;
+                    continue;
+                }
+                if (n == kFastLeafCap) {
+                    fail = true;
+                    break;
+                }
+                const int id = -child - 1;
+                const LeafRec lr = Q.leaves[id];
+                const float key = lr.w0 + (o.t - lr.t0) * lr.mvw0;  // ReconstructionOur.cpp:384-385
+                int j = n;
+                while (j > 0) {
+                    const float kk = lkey[j - 1][tid];
+                    const int ii = lid[j - 1][tid];
+                    if (!(kk > key || (!(key > kk) && ii > id))) break;
+                    lkey[j][tid] = kk;
+                    lid[j][tid] = ii;
+                    j--;
+                }
+                lkey[j][tid] = key;
+                lid[j][tid] = id;
+                n++;
+            }
+            node = escape;
+        }
+        if (!fail) {
+            s_leaves = (unsigned long long)n;
+            const float RANGE = 1.0f / sqrtf(128.0f);
+            const float u0 = o.u - RANGE, v0 = o.v - RANGE, t0 = o.t - 0.5f * RANGE;
+            const float u1 = o.u + RANGE, v1 = o.v + RANGE, t1 = o.t + 0.5f * RANGE;
+            const float R2 = R * R;
+            int nsurf = 0, sstart = 0, quad0 = 0;
+            float ar = 0, ag = 0, ab = 0, aw = 0;
+            float min0 = kFltMax;
+            float4 near = make_float4(0, 0, 0, 0);
+            bool stop = false;
+            for (int li = 0; li < n && !stop; li++) {
+                const int id = lid[li][tid];
+                const LeafRec lr = Q.leaves[id];
+                s_in_leaves += (unsigned long long)(lr.s1 - lr.s0);
+                bool first = true;
+                for (int k = lr.s0; k < lr.s1; k++) {
+                    float x, y;
+                    reproject_to(Q.pts, k, o, Q.cc0, Q.cc1, x, y);
+                    const float dx = x - o.x, dy = y - o.y;
+                    const float d2 = dx * dx + dy * dy;
+                    if (!(d2 <= R2)) continue;
+                    if (first) {
+                        first = false;
+                        bool same = true;
+                        const float* np = Q.leaf_b[id].p;
+                        for (int j = sstart; j < li && same; ++j)
+                            same = same_surface(np, Q.leaf_b[lid[j][tid]].p, u0, v0, t0, u1, v1, t1);
+                        if (!same || nsurf == 0) {
+                            if (nsurf == 1 && !STATS) {
+                                stop = true;  // surface 0 is complete
+                                break;
+                            }
+                            sstart = li;
+                            nsurf++;
+                        }
+                    }
+                    s_kept++;
+                    if (nsurf == 1) {
+                        quad0 |= 1 << ((dx >= 0 ? XPOS : 0) | (dy >= 0 ? YPOS : 0));
+                        const float dist = sqrtf(d2);
+                        if (dist < min0) {
+                            min0 = dist;
+                            near = Q.pts.pc[k];
+                        }
+                        const float wgt = 1 - dist / disp;
+                        if (wgt > 0) {
+                            const float4 col = Q.pts.pc[k];
+                            ar += wgt * col.x;
+                            ag += wgt * col.y;
+                            ab += wgt * col.z;
+                            aw += wgt * 1.0f;
+                        }
+                    }
+                }
+            }
+            const bool f2 = nsurf == 0 || (quad0 != 0xF && (nsurf > 1 || stop));
+            if (f2) {
+                fail = true;
+            } else {
+                s_surf = (unsigned long long)nsurf;
+                const float oow = rcp(aw);
+                float cr = ar * oow, cg = ag * oow, cb = ab * oow, ca = aw * oow;
+                if (ca == 0) {
+                    cr = near.x; cg = near.y; cb = near.z; ca = 1.0f;
+                }
+                if (Q.lens_gauss) {
+                    const float e = 2.718281828f;
+                    const float gu = powf(e, -((o.u - 0) * (o.u - 0)) / (2 * (0.5f * 0.5f)));
+                    const float gv = powf(e, -((o.v - 0) * (o.v - 0)) / (2 * (0.5f * 0.5f)));
+                    const float gg = gu * gv;
+                    cr *= gg; cg *= gg; cb *= gg; ca *= gg;
+                }
+                if (Q.time_gauss) {
+                    const float e = 2.718281828f;
+                    const float gg = powf(e, -((o.t - 0.5f) * (o.t - 0.5f)) / (2 * (0.25f * 0.25f)));
+                    cr *= gg; cg *= gg; cb *= gg; ca *= gg;
+                }
+                Q.qcol[q] = make_float4(cr, cg, cb, ca);
+            }
+        }
+    }
+    // hand failed queries to the general kernel (one atomic per wave)
+    const uint64_t fm = __ballot(active && fail);
+    if (fm) {
+        const int lane = tid & 63;
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(Q.fail_count, (unsigned)__popcll(fm));
+        base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
+        if (active && fail) Q.fail_list[base + lanes_below(fm)] = q;
+    }
+    if (STATS) {
+        const bool done = active && !fail;
+        unsigned long long v[6] = {done ? s_leaves : 0ull, done ? s_in_leaves : 0ull, done ? s_kept : 0ull,
+                                   done ? s_surf : 0ull, done ? 1ull : 0ull, 0ull};
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, 64);
+        }
+        if ((tid & 63) == 0 && v[4]) {
+            atomicAdd(&Q.stats[ST_LEAF_NODES], v[0]);
+            atomicAdd(&Q.stats[ST_SAMPLES_IN_LEAVES], v[1]);
+            atomicAdd(&Q.stats[ST_WITHIN_1R], v[2]);
+            atomicAdd(&Q.stats[ST_FETCHED], v[2]);
+            atomicAdd(&Q.stats[ST_SURFACES], v[3]);
+            atomicAdd(&Q.stats[ST_COUNT], v[4]);
+        }
+    }
+}
+
 // Pixel = sum of its output samples in order, normalised by alpha; red if
 // nothing was found (Reconstruction.cpp:665-719).
 __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ qcol, int W, int H, int M,
@@ -539,6 +730,16 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     const int blocks = p.num_waves / kWavesPerBlock;
     k_query_general<<<blocks, kBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
+    if (p.num_queries == 0) return hipSuccess;
+    const unsigned blocks = (p.num_queries + kFastBlock - 1) / kFastBlock;
+    if (stats)
+        k_query_fast<true><<<blocks, kFastBlock, 0, s>>>(p);
+    else
+        k_query_fast<false><<<blocks, kFastBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 

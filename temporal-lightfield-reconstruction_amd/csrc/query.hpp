@@ -41,9 +41,24 @@ struct QueryParams {
     unsigned* next_query;     // work counter (zeroed before launch)
     float* scratch;           // kWaveScratchWords per resident wave
     int num_waves;
+    // general kernel: optional list of query ids (qlist[0 .. *qcount))
+    const unsigned* qlist;
+    const unsigned* qcount;
+    // fast kernel: queries it hands to the general kernel
+    unsigned* fail_list;
+    unsigned* fail_count;
 };
 
+// Capacity of the per-lane sorted leaf list of the fast kernel; queries with
+// more 1R leaves go to the general kernel.
+constexpr int kFastLeafCap = 32;
+
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s);
+// One lane per output sample, 1R only; completes every query whose result
+// is decided by the first surface without a 2R fetch
+// (ReconstructionOur.cpp:99,204: single surface, or a first surface covering
+// all four quadrants) and appends the others to fail_list.
+hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
 
 }  // namespace tlr

@@ -71,6 +71,8 @@ struct tlr_ctx {
     float* d_img = nullptr;
     unsigned long long* d_stats = nullptr;
     unsigned* d_counter = nullptr;
+    unsigned* d_fail_count = nullptr;
+    unsigned* fail_list = nullptr;
     float* scratch = nullptr;
     int num_waves = 0;
     int last_M = 0;
@@ -120,7 +122,8 @@ tlr_status tlr_create(tlr_ctx** out, int device) {
     c->events = true;
     if (hipMalloc(&c->d_gi, sizeof(GridInfo)) != hipSuccess ||
         hipMalloc(&c->d_stats, sizeof(unsigned long long) * kNumStats) != hipSuccess ||
-        hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess) {
+        hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->d_fail_count, sizeof(unsigned)) != hipSuccess) {
         delete c;
         return TLR_E_HIP;
     }
@@ -138,6 +141,7 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->d_gi) (void)hipFree(c->d_gi);
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_counter) (void)hipFree(c->d_counter);
+    if (c->d_fail_count) (void)hipFree(c->d_fail_count);
     if (c->events)
         for (auto& e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -268,7 +272,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     words += 4 * (size_t)N;           // rx ry rw rt
     words += 4 * (size_t)N;           // radix ping-pong
     words += 3 * (size_t)N;           // cand ct cw
-    words += 10 * (size_t)N;          // points
+    words += 12 * (size_t)N;          // points (3 x float4)
     words += 14 * (size_t)N;          // per-sample bounds
     words += 8 * (size_t)N;           // boxes
     words += 2 * (size_t)N;           // start flags, leaf pos
@@ -362,9 +366,9 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     // ---- per-sample data in tree order ----
     const int nb1 = std::max(nb, 1);
     Points P;
-    P.x = A.alloc<float>(nb1); P.y = A.alloc<float>(nb1); P.w = A.alloc<float>(nb1); P.t = A.alloc<float>(nb1);
-    P.mvx = A.alloc<float>(nb1); P.mvy = A.alloc<float>(nb1); P.mvw = A.alloc<float>(nb1);
-    P.r = A.alloc<float>(nb1); P.g = A.alloc<float>(nb1); P.b = A.alloc<float>(nb1);
+    P.pa = A.alloc<float4>(nb1);
+    P.pb = A.alloc<float4>(nb1);
+    P.pc = A.alloc<float4>(nb1);
     c->pts = P;
     Bounds* sb = A.alloc<Bounds>(nb1);
     float4* boxes = A.alloc<float4>(2 * (size_t)nb1);
@@ -373,7 +377,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     if (!need(lpos)) return c->fail(TLR_E_HIP, "arena exhausted (points)");
     launch_points(c->cand, nb, in, c->rx, c->ry, c->rw, c->rt, (float)W, (float)H, c->cc0, c->cc1, P, sb, boxes, s);
     HIPCHK(c, hipMemsetAsync(sflag, 0, sizeof(uint32_t) * (nb1 + 1), s));
-    launch_surface_scan(c->ileaf_oc, nil, P.t, boxes, sflag, s);
+    launch_surface_scan(c->ileaf_oc, nil, P.pa, boxes, sflag, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, exclusive_scan_u32(sflag, lpos, lpos + nb, (size_t)nb, stmp, s));
     const int L = nb > 0 ? (int)read_u32(c, lpos + nb) : 0;
@@ -440,12 +444,13 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     // waves resident across the device: CUs x 8
     const int waves = std::max(4, c->prop.multiProcessorCount * 8);
     const size_t qbytes = pat.size() * 4 + 256 + nq * sizeof(float4) + 256 + (size_t)W * H * 16 + 256 +
-                          (size_t)waves * kWaveScratchWords * 4 + 4096;
+                          nq * 4 + 256 + (size_t)waves * kWaveScratchWords * 4 + 4096;
     HIPCHK(c, c->qarena.reserve(qbytes));
     c->qarena.reset();
     c->d_pattern = c->qarena.alloc<float>(pat.size());
     c->qcol = c->qarena.alloc<float4>(nq);
     c->d_img = c->qarena.alloc<float>((size_t)W * H * 4);
+    c->fail_list = c->qarena.alloc<unsigned>(nq);
     c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
     if (!c->scratch) return c->fail(TLR_E_HIP, "query arena exhausted");
     c->qcol_n = nq;
@@ -454,6 +459,7 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     HIPCHK(c, hipMemcpyAsync(c->d_pattern, pat.data(), pat.size() * 4, hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
     HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
 
     QueryParams Q;
     Q.W = W; Q.H = H; Q.M = M;
@@ -474,9 +480,23 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     Q.next_query = c->d_counter;
     Q.scratch = c->scratch;
     Q.num_waves = waves;
+    Q.qlist = nullptr;
+    Q.qcount = nullptr;
+    Q.fail_list = c->fail_list;
+    Q.fail_count = c->d_fail_count;
 
     HIPCHK(c, hipEventRecord(c->ev[2], s));
-    HIPCHK(c, launch_query_general(Q, s));
+    if (p->path == 1) {
+        HIPCHK(c, launch_query_general(Q, s));
+        HIPCHK(c, hipEventRecord(c->ev[5], s));
+    } else {
+        HIPCHK(c, launch_query_fast(Q, p->collect_stats != 0, s));
+        HIPCHK(c, hipEventRecord(c->ev[5], s));
+        QueryParams G = Q;
+        G.qlist = c->fail_list;
+        G.qcount = c->d_fail_count;
+        HIPCHK(c, launch_query_general(G, s));
+    }
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     float* dst = out_on_device ? out : c->d_img;
     HIPCHK(c, launch_combine(c->qcol, W, H, M, dst, s));
@@ -484,7 +504,9 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     if (!out_on_device && out)
         HIPCHK(c, hipMemcpyAsync(out, c->d_img, (size_t)W * H * 16, hipMemcpyDeviceToHost, s));
     unsigned long long hs[kNumStats];
+    unsigned nfail = 0;
     HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&nfail, c->d_fail_count, sizeof(nfail), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (stats) {
         memset(stats, 0, sizeof(*stats));
@@ -501,6 +523,9 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         float ms = 0;
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) stats->fast_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) stats->general_ms = ms;
+        stats->general_queries = p->path == 1 ? (double)nq : (double)nfail;
         if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;
         stats->total_ms = stats->build_ms + stats->query_ms + stats->combine_ms;
         stats->num_leaves = c->L;

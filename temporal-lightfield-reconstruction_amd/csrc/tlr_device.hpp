@@ -29,10 +29,14 @@ struct GridInfo {
     int pad[3];
 };
 
-// Tree sample layout (Node::s0..s1 ranges index it): SoA of the reprojected
-// samples in leaf order.  40 bytes per sample.
+// Tree sample layout (Node::s0..s1 ranges index it): the reprojected samples
+// in leaf order as 16-byte records, so that one lane walking a leaf reads
+// consecutive bytes and a wave reading one leaf is fully coalesced.
+//   pa[k] = (x, y, w, t)        position at the (u,v,t) centre
+//   pb[k] = (mv_x, mv_y, mv_w, 0) homogeneous motion
+//   pc[k] = (r, g, b, 1)        colour
 struct Points {
-    float *x, *y, *w, *t, *mvx, *mvy, *mvw, *r, *g, *b;
+    float4 *pa, *pb, *pc;
 };
 
 // Leaf record, indexed by leaf id (ids follow the sample layout).

@@ -105,3 +105,25 @@ def test_image_parity(tlr, orc, case):
     assert st["overflow"] == 0
     assert nbad == 0, "per-query colours differ at %d output samples" % nbad
     assert err.max() <= 2e-3 and err.mean() <= 1e-4
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fast_path_equals_general(tlr, orc, case):
+    """The 1R lane kernel (+ general kernel for the rest), with and without
+    stats, gives bit-identical output samples to the general kernel alone."""
+    sbuf, ctx, o = _pair(tlr, orc, case)
+    M = case[5]
+    _, st_g = ctx.reconstruct(out_spp=M, path=1, collect_stats=True)
+    qg = ctx.query_colors(M)
+    _, st_f = ctx.reconstruct(out_spp=M, path=0, collect_stats=True)
+    qf = ctx.query_colors(M)
+    _, st_n = ctx.reconstruct(out_spp=M, path=0, collect_stats=False)
+    qn = ctx.query_colors(M)
+    print("%s: general %.2f ms; fast %.2f + general %.2f ms for %d of %d queries" % (
+        case[0], st_g["query_ms"], st_n["fast_ms"], st_n["general_ms"], st_n["general_queries"],
+        st_n["count"] if st_n["count"] else qg.size // 4))
+    assert np.array_equal(_bits(qg), _bits(qf))
+    assert np.array_equal(_bits(qg), _bits(qn))
+    for k in ("leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
+              "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count"):
+        assert st_g[k] == st_f[k], k
