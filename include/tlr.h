@@ -70,6 +70,8 @@ typedef struct tlr_stats {
     double num_leaves, num_samples;
     double fast_ms, general_ms;  /* split of query_ms: 1R lane kernel, general kernel */
     double general_queries;      /* output samples handled by the general kernel     */
+    double direct_ms, gate_ms;   /* gated path: direct filter + features, MLP + routing */
+    double gate_rejected;        /* output samples the gate sent to the plain method */
 } tlr_stats;
 
 typedef struct tlr_build_info {
@@ -133,6 +135,37 @@ tlr_status  tlr_copy_leaves(tlr_ctx* ctx, int32_t* s01, float* bounds14);
 tlr_status  tlr_copy_query_colors(tlr_ctx* ctx, float* out4);
 /* output pattern: 64*M templates x,y,u,v,t (host computation) */
 int         tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, float* out5);
+
+/* ---- Learned gate (no reference counterpart) ------------------------------
+ * A 24-64-64-1 MLP (BF16 operands, FP32 accumulation on MFMA) scores each
+ * output sample from features of a streaming 1R gather; output samples it
+ * accepts take the direct tent filter of that gather, the others the plain
+ * method.  Parameter layout: W1[24][64], b1[64], W2[64][64], b2[64], w3[64],
+ * b3 (TLR_GATE_PARAMS floats, row-major with the input index first). */
+#define TLR_GATE_FEATURES 24
+#define TLR_GATE_PARAMS 5825
+tlr_status  tlr_gate_set_params(tlr_ctx* ctx, const float* params);
+tlr_status  tlr_gate_get_params(tlr_ctx* ctx, float* params);
+/* output samples with logit >= threshold take the direct filter */
+tlr_status  tlr_gate_set_threshold(tlr_ctx* ctx, float logit_threshold);
+/* host arrays: logits of n feature rows */
+tlr_status  tlr_gate_forward(tlr_ctx* ctx, const float* features, int64_t n, float* logits);
+/* host arrays: summed gradient of the weighted BCE loss over n rows */
+tlr_status  tlr_gate_grad(tlr_ctx* ctx, const float* features, const float* labels, int64_t n, float pos_weight,
+                          float* grad, float* loss);
+/* For every stride-th output sample of the current build (query order):
+ * features, label (1 = one surface at 1R, no 2R fetch), direct-filter and
+ * plain-method colours.  Buffers may be NULL; *n receives the row count. */
+tlr_status  tlr_gate_collect(tlr_ctx* ctx, const tlr_recon_params* p, int stride, float* features, float* labels,
+                             float* direct_rgba, float* plain_rgba, int64_t* n);
+/* Same rows appended to a device-resident training set. */
+tlr_status  tlr_gate_append_training(tlr_ctx* ctx, const tlr_recon_params* p, int stride);
+tlr_status  tlr_gate_clear_training(tlr_ctx* ctx);
+int64_t     tlr_gate_training_size(tlr_ctx* ctx);
+/* Adam on the device training set; epoch_loss (optional) gets the mean loss
+ * of each epoch. */
+tlr_status  tlr_gate_train(tlr_ctx* ctx, int epochs, int batch, float lr, float pos_weight, uint64_t seed,
+                           float* epoch_loss);
 
 #ifdef __cplusplus
 }

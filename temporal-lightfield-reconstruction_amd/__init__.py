@@ -30,7 +30,12 @@ EXPORTED = [
     "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_sync",
     "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
     "tlr_copy_leaves", "tlr_copy_query_colors", "tlr_output_pattern",
+    "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
+    "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
+    "tlr_gate_training_size", "tlr_gate_train",
 ]
+GATE_FEATURES = 24
+GATE_PARAMS = 5825
 
 
 class TlrError(RuntimeError):
@@ -61,7 +66,7 @@ class Stats(C.Structure):
         "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
         "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count", "overflow",
         "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
-        "fast_ms", "general_ms", "general_queries")]
+        "fast_ms", "general_ms", "general_queries", "direct_ms", "gate_ms", "gate_rejected")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -124,6 +129,17 @@ def lib():
         L.tlr_copy_query_colors.argtypes = [V, V]
         L.tlr_output_pattern.restype = C.c_int
         L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
+        L.tlr_gate_set_params.argtypes = [V, V]
+        L.tlr_gate_get_params.argtypes = [V, V]
+        L.tlr_gate_set_threshold.argtypes = [V, C.c_float]
+        L.tlr_gate_forward.argtypes = [V, V, C.c_int64, V]
+        L.tlr_gate_grad.argtypes = [V, V, V, C.c_int64, C.c_float, V, V]
+        L.tlr_gate_collect.argtypes = [V, C.POINTER(ReconParams), C.c_int, V, V, V, V, C.POINTER(C.c_int64)]
+        L.tlr_gate_append_training.argtypes = [V, C.POINTER(ReconParams), C.c_int]
+        L.tlr_gate_clear_training.argtypes = [V]
+        L.tlr_gate_training_size.restype = C.c_int64
+        L.tlr_gate_training_size.argtypes = [V]
+        L.tlr_gate_train.argtypes = [V, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, V]
         _tlr = L
     return _tlr
 
@@ -263,6 +279,76 @@ class Ctx:
         st = Stats()
         self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img), 0, C.byref(st)))
         return img, st.as_dict()
+
+    @staticmethod
+    def _params(out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,
+                collect_stats=False, gate=False, path=0):
+        p = ReconParams()
+        p.out_spp = out_spp
+        p.override_uvt = 1 if override_uvt is not None else 0
+        if override_uvt is not None:
+            for k in range(3):
+                p.uvt[k] = override_uvt[k]
+        p.lens_gaussian = int(lens_gaussian)
+        p.time_gaussian = int(time_gaussian)
+        p.collect_stats = int(collect_stats)
+        p.gate = int(gate)
+        p.path = int(path)
+        return p
+
+    # ---- learned gate ----
+    def gate_set_params(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        assert params.size == GATE_PARAMS
+        self._chk(lib().tlr_gate_set_params(self.h, _p(params)))
+
+    def gate_get_params(self):
+        out = np.empty(GATE_PARAMS, np.float32)
+        self._chk(lib().tlr_gate_get_params(self.h, _p(out)))
+        return out
+
+    def gate_set_threshold(self, thr):
+        self._chk(lib().tlr_gate_set_threshold(self.h, float(thr)))
+
+    def gate_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, GATE_FEATURES)
+        out = np.empty(X.shape[0], np.float32)
+        self._chk(lib().tlr_gate_forward(self.h, _p(X), X.shape[0], _p(out)))
+        return out
+
+    def gate_grad(self, X, y, pos_weight=1.0):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, GATE_FEATURES)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(GATE_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(lib().tlr_gate_grad(self.h, _p(X), _p(y), X.shape[0], pos_weight, _p(g), _p(loss)))
+        return g, float(loss[0])
+
+    def gate_collect(self, out_spp=16, stride=1, **kw):
+        p = self._params(out_spp=out_spp, **kw)
+        M = 1 if p.override_uvt else out_spp
+        nq = self.sbuf.width * self.sbuf.height * M
+        rows = (nq + stride - 1) // stride
+        X = np.empty((rows, GATE_FEATURES), np.float32)
+        y = np.empty(rows, np.float32)
+        d = np.empty((rows, 4), np.float32)
+        pl = np.empty((rows, 4), np.float32)
+        n = C.c_int64(0)
+        self._chk(lib().tlr_gate_collect(self.h, C.byref(p), stride, _p(X), _p(y), _p(d), _p(pl), C.byref(n)))
+        return X, y, d, pl
+
+    def gate_append_training(self, out_spp=16, stride=1, **kw):
+        p = self._params(out_spp=out_spp, **kw)
+        self._chk(lib().tlr_gate_append_training(self.h, C.byref(p), stride))
+        return lib().tlr_gate_training_size(self.h)
+
+    def gate_clear_training(self):
+        self._chk(lib().tlr_gate_clear_training(self.h))
+
+    def gate_train(self, epochs, batch=4096, lr=1e-3, pos_weight=1.0, seed=0):
+        losses = np.zeros(max(epochs, 1), np.float32)
+        self._chk(lib().tlr_gate_train(self.h, epochs, batch, lr, pos_weight, C.c_uint64(seed), _p(losses)))
+        return losses[:epochs]
 
     def build_info(self):
         bi = BuildInfo()

@@ -540,9 +540,12 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     __shared__ float lkey[kFastLeafCap][kFastBlock];
     __shared__ int lid[kFastLeafCap][kFastBlock];
     const int tid = threadIdx.x;
-    const unsigned q = blockIdx.x * kFastBlock + tid;
-    const bool active = q < Q.num_queries;
+    const unsigned slot = blockIdx.x * kFastBlock + tid;
+    const unsigned nq = Q.fast_list ? *Q.qcount : Q.num_queries;
+    const bool active = slot < nq;
+    const unsigned q = active ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
     bool fail = false;
+    bool single = false;
     unsigned long long s_leaves = 0, s_in_leaves = 0, s_kept = 0, s_surf = 0;
     if (active) {
         const int M = Q.M;
@@ -651,6 +654,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
                 }
             }
             const bool f2 = nsurf == 0 || (quad0 != 0xF && (nsurf > 1 || stop));
+            single = nsurf == 1 && !stop;
             if (f2) {
                 fail = true;
             } else {
@@ -676,6 +680,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             }
         }
     }
+    if (Q.labels && active) Q.labels[q] = (!fail && single) ? 1.0f : 0.0f;
     // hand failed queries to the general kernel (one atomic per wave)
     const uint64_t fm = __ballot(active && fail);
     if (fm) {
@@ -703,6 +708,186 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             atomicAdd(&Q.stats[ST_COUNT], v[4]);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Direct filter + gate features (one lane per output sample, R = dispersion).
+// Leaves are visited in traversal order and every sample within R joins one
+// tent sum: exactly the plain result when the output sample sees a single
+// surface (ReconstructionOur.cpp:99,204,268-297), up to summation order.
+// Features (kGateIn floats, roughly unit scale):
+//   0 log1p(kept)  1 log1p(leaves)  2 log1p(samples in leaves)
+//   3 depth range / min depth  4 depth std / mean  5 log(mean depth)
+//   6 CoC range  7 mean |CoC| / 16  8,9 screen motion range x,y / 16
+//   10 range of sample t  11..14 quadrant fractions
+//   15 leaf key range / min key  16 fraction of leaves whose key differs by
+//   more than 5% from the first leaf met  17,18,19 output (u, v, t)
+//   20 nearest distance / R  21 mean distance / R  22 tent weight sum / kept
+//   23 fraction of gathered samples whose depth differs by more than 5% from
+//   the first gathered sample
+// ---------------------------------------------------------------------------
+constexpr int kDirectBlock = 256;
+
+__global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
+    const unsigned q = blockIdx.x * kDirectBlock + threadIdx.x;
+    if (q >= Q.num_queries) return;
+    const int M = Q.M;
+    const unsigned pix = q / (unsigned)M;
+    const int i = (int)(q - pix * (unsigned)M);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
+    const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
+    OSample o;
+    o.x = tp[0] + (float)px;
+    o.y = tp[1] + (float)py;
+    o.u = tp[2];
+    o.v = tp[3];
+    o.t = tp[4];
+    const float disp = dispersion_for_spp(Q.spp_pow2);
+    const float R = disp, R2 = R * R;
+    float ar = 0, ag = 0, ab = 0, aw = 0;
+    float dmin = kFltMax, dsum = 0;
+    float4 near = make_float4(0, 0, 0, 1);
+    int kept = 0, nleaf = 0, inleaf = 0;
+    float wmin = kFltMax, wmax = -kFltMax, wsum = 0, wsq = 0;
+    float cmin = kFltMax, cmax = -kFltMax, cabs = 0;
+    float mxmin = kFltMax, mxmax = -kFltMax, mymin = kFltMax, mymax = -kFltMax;
+    float tmin = kFltMax, tmax = -kFltMax;
+    float kmin = kFltMax, kmax = -kFltMax;
+    int quad[4] = {0, 0, 0, 0};
+    // depth separation relative to the first leaf / first kept sample met
+    float key_first = 0.0f, w_first = 0.0f;
+    int far_leaves = 0, off_depth = 0;
+    int node = Q.n_nodes > 0 ? 0 : -1;
+    while (node >= 0) {
+        const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
+        const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
+        const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+        const int child = __float_as_int(d.x), escape = __float_as_int(d.y);
+        if (node_hit(p, o, R)) {
+            if (child >= 0) {
+                node = child;
+                continue;
+            }
+            const LeafRec lr = Q.leaves[-child - 1];
+            const float key = lr.w0 + (o.t - lr.t0) * lr.mvw0;
+            if (nleaf == 0) key_first = key;
+            far_leaves += (nleaf > 0 && (key > 1.05f * key_first || key_first > 1.05f * key)) ? 1 : 0;
+            nleaf++;
+            inleaf += lr.s1 - lr.s0;
+            kmin = fminf(kmin, key);
+            kmax = fmaxf(kmax, key);
+            for (int k = lr.s0; k < lr.s1; k++) {
+                const float4 sa = Q.pts.pa[k], sb = Q.pts.pb[k];
+                float x, y, w;
+                if (sb.x != 0.0f || sb.y != 0.0f || sb.z != 0.0f) {
+                    const float dt = o.t - sa.w;
+                    const float Px = sa.x * sa.z + dt * sb.x;
+                    const float Py = sa.y * sa.z + dt * sb.y;
+                    const float Pw = 1.0f * sa.z + dt * sb.z;
+                    const float cr = rcp(Pw);
+                    const float coc = Q.cc0 / Pw + Q.cc1;
+                    x = Px * cr + coc * o.u;
+                    y = Py * cr + coc * o.v;
+                    w = Pw;
+                } else {
+                    const float coc = Q.cc0 / sa.z + Q.cc1;
+                    x = sa.x + coc * o.u;
+                    y = sa.y + coc * o.v;
+                    w = sa.z;
+                }
+                const float dx = x - o.x, dy = y - o.y;
+                const float d2 = dx * dx + dy * dy;
+                if (!(d2 <= R2)) continue;
+                const float dist = sqrtf(d2);
+                if (kept == 0) w_first = w;
+                off_depth += (w > 1.05f * w_first || w_first > 1.05f * w) ? 1 : 0;
+                kept++;
+                dsum += dist;
+                const float4 col = Q.pts.pc[k];
+                if (dist < dmin) {
+                    dmin = dist;
+                    near = col;
+                }
+                const float wgt = 1 - dist / disp;
+                if (wgt > 0) {
+                    ar += wgt * col.x;
+                    ag += wgt * col.y;
+                    ab += wgt * col.z;
+                    aw += wgt * 1.0f;
+                }
+                wmin = fminf(wmin, w);
+                wmax = fmaxf(wmax, w);
+                wsum += w;
+                wsq += w * w;
+                const float coc = Q.cc0 / w + Q.cc1;
+                cmin = fminf(cmin, coc);
+                cmax = fmaxf(cmax, coc);
+                cabs += fabsf(coc);
+                const float rw = rcp(w);
+                mxmin = fminf(mxmin, sb.x * rw);
+                mxmax = fmaxf(mxmax, sb.x * rw);
+                mymin = fminf(mymin, sb.y * rw);
+                mymax = fmaxf(mymax, sb.y * rw);
+                tmin = fminf(tmin, sa.w);
+                tmax = fmaxf(tmax, sa.w);
+                quad[(dx >= 0 ? 1 : 0) | (dy >= 0 ? 2 : 0)]++;
+            }
+        }
+        node = escape;
+    }
+    // colour: tent result, or the nearest sample when all weights vanish
+    const float oow = rcp(aw);
+    float cr = ar * oow, cg = ag * oow, cb = ab * oow, ca = aw * oow;
+    if (ca == 0 && kept > 0) {
+        cr = near.x; cg = near.y; cb = near.z; ca = 1.0f;
+    }
+    Q.qcol[q] = make_float4(cr, cg, cb, ca);
+    if (Q.features) {
+        float f[kGateFeatures];
+        const float kf = (float)(kept > 0 ? kept : 1);
+        const float wm = wsum / kf;
+        f[0] = log1pf((float)kept);
+        f[1] = log1pf((float)nleaf);
+        f[2] = log1pf((float)inleaf);
+        f[3] = kept ? (wmax - wmin) / wmin : 0.0f;
+        f[4] = kept ? sqrtf(fmaxf(wsq / kf - wm * wm, 0.0f)) / wm : 0.0f;
+        f[5] = kept ? logf(wm) : 0.0f;
+        f[6] = kept ? (cmax - cmin) : 0.0f;
+        f[7] = kept ? cabs / kf / 16.0f : 0.0f;
+        f[8] = kept ? (mxmax - mxmin) / 16.0f : 0.0f;
+        f[9] = kept ? (mymax - mymin) / 16.0f : 0.0f;
+        f[10] = kept ? (tmax - tmin) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) f[11 + k] = (float)quad[k] / kf;
+        f[15] = nleaf ? (kmax - kmin) / fmaxf(kmin, 1e-6f) : 0.0f;
+        f[16] = nleaf ? (float)far_leaves / (float)nleaf : 0.0f;
+        f[17] = o.u;
+        f[18] = o.v;
+        f[19] = o.t;
+        f[20] = kept ? dmin / R : 1.0f;
+        f[21] = kept ? dsum / kf / R : 1.0f;
+        f[22] = kept ? aw / kf : 0.0f;
+        f[23] = kept ? (float)off_depth / kf : 0.0f;
+        float* out = Q.features + (size_t)q * kGateFeatures;
+#pragma unroll
+        for (int k = 0; k < kGateFeatures; k++) out[k] = f[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gate_route(const float* __restrict__ logits,
+                                                    const float* __restrict__ features, unsigned n, float thr,
+                                                    unsigned* __restrict__ list, unsigned* __restrict__ count) {
+    const unsigned q = blockIdx.x * 256 + threadIdx.x;
+    const bool act = q < n;
+    // feature 0 is log1p(kept): no gathered sample -> plain path
+    const bool plain = act && (logits[q] < thr || features[(size_t)q * kGateFeatures] == 0.0f);
+    const uint64_t m = __ballot(plain);
+    if (!m) return;
+    unsigned base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
+    if (plain) list[base + lanes_below(m)] = q;
 }
 
 // Pixel = sum of its output samples in order, normalised by alpha; red if
@@ -735,11 +920,25 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
 
 hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
+    // list mode launches for the worst case; surplus lanes exit at once
     const unsigned blocks = (p.num_queries + kFastBlock - 1) / kFastBlock;
     if (stats)
         k_query_fast<true><<<blocks, kFastBlock, 0, s>>>(p);
     else
         k_query_fast<false><<<blocks, kFastBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_direct(const QueryParams& p, hipStream_t s) {
+    if (p.num_queries == 0) return hipSuccess;
+    k_query_direct<<<(p.num_queries + kDirectBlock - 1) / kDirectBlock, kDirectBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gate_route(const float* logits, const float* features, unsigned n, float thr, unsigned* list,
+                             unsigned* count, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_gate_route<<<(n + 255) / 256, 256, 0, s>>>(logits, features, n, thr, list, count);
     return hipGetLastError();
 }
 

@@ -47,11 +47,19 @@ struct QueryParams {
     // fast kernel: queries it hands to the general kernel
     unsigned* fail_list;
     unsigned* fail_count;
+    // fast kernel over a list: qlist/qcount above when fast_list != 0
+    int fast_list;
+    // fast kernel, optional: 1 where the output sample has exactly one
+    // surface at 1R and no 2R fetch (the gate's training label)
+    float* labels;
+    // direct kernel: per query features (kGateIn floats) and colours
+    float* features;
 };
 
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
 // more 1R leaves go to the general kernel.
 constexpr int kFastLeafCap = 32;
+constexpr int kGateFeatures = 24;
 
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s);
 // One lane per output sample, 1R only; completes every query whose result
@@ -59,6 +67,14 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s);
 // (ReconstructionOur.cpp:99,204: single surface, or a first surface covering
 // all four quadrants) and appends the others to fail_list.
 hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
+// Direct filter for the learned gate: one lane per output sample, 1R
+// gather in traversal order, tent filter over every gathered sample without
+// surface clustering, plus kGateIn features; writes qcol and features.
+hipError_t launch_query_direct(const QueryParams& p, hipStream_t s);
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
+// Routing after the gate: queries whose logit is below the threshold, or
+// with no gathered sample, are appended to list (for the plain path).
+hipError_t launch_gate_route(const float* logits, const float* features, unsigned n, float threshold,
+                             unsigned* list, unsigned* count, hipStream_t s);
 
 }  // namespace tlr

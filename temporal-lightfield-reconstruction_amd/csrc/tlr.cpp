@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <numeric>
+#include <random>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -15,6 +17,7 @@
 
 #include "../../include/tlr.h"
 #include "build.hpp"
+#include "gate.hpp"
 #include "pattern.hpp"
 #include "prims.hpp"
 #include "query.hpp"
@@ -76,6 +79,25 @@ struct tlr_ctx {
     float* scratch = nullptr;
     int num_waves = 0;
     int last_M = 0;
+    float* features = nullptr;      // gate: per query features
+    float* logits = nullptr;
+    float* labels = nullptr;
+    unsigned* route_list = nullptr;
+    unsigned* d_route_count = nullptr;
+
+    // learned gate
+    float* gate_p = nullptr;        // fp32 master parameters
+    float* gate_m = nullptr;
+    float* gate_v = nullptr;
+    float* gate_g = nullptr;
+    float* gate_loss = nullptr;
+    bool has_gate = false;
+    float gate_thr = 0.0f;
+    int gate_step = 0;
+    float* train_X = nullptr;
+    float* train_y = nullptr;
+    int* train_idx = nullptr;
+    int64_t train_n = 0, train_cap = 0;
 
     hipEvent_t ev[8];
     bool events = false;
@@ -123,7 +145,13 @@ tlr_status tlr_create(tlr_ctx** out, int device) {
     if (hipMalloc(&c->d_gi, sizeof(GridInfo)) != hipSuccess ||
         hipMalloc(&c->d_stats, sizeof(unsigned long long) * kNumStats) != hipSuccess ||
         hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess ||
-        hipMalloc(&c->d_fail_count, sizeof(unsigned)) != hipSuccess) {
+        hipMalloc(&c->d_fail_count, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->d_route_count, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->gate_p, sizeof(float) * kGateParams) != hipSuccess ||
+        hipMalloc(&c->gate_m, sizeof(float) * kGateParams) != hipSuccess ||
+        hipMalloc(&c->gate_v, sizeof(float) * kGateParams) != hipSuccess ||
+        hipMalloc(&c->gate_g, sizeof(float) * kGateParams) != hipSuccess ||
+        hipMalloc(&c->gate_loss, sizeof(float)) != hipSuccess) {
         delete c;
         return TLR_E_HIP;
     }
@@ -142,6 +170,10 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_counter) (void)hipFree(c->d_counter);
     if (c->d_fail_count) (void)hipFree(c->d_fail_count);
+    if (c->d_route_count) (void)hipFree(c->d_route_count);
+    for (float* p : {c->gate_p, c->gate_m, c->gate_v, c->gate_g, c->gate_loss, c->train_X, c->train_y})
+        if (p) (void)hipFree(p);
+    if (c->train_idx) (void)hipFree(c->train_idx);
     if (c->events)
         for (auto& e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -429,9 +461,9 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     return TLR_OK;
 }
 
-tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, int out_on_device, tlr_stats* stats) {
-    if (!c || !p) return TLR_E_INVALID;
-    if (!c->built) return c->fail(TLR_E_STATE, "tlr_reconstruct before tlr_build");
+// Allocates the per-reconstruction buffers and fills the query parameters.
+static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool gate_buffers, QueryParams& Q) {
+    if (!c->built) return c->fail(TLR_E_STATE, "reconstruct before tlr_build");
     if (p->out_spp <= 0 && !p->override_uvt) return c->fail(TLR_E_INVALID, "out_spp must be positive");
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -439,12 +471,12 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     const int M = make_output_pattern(p->out_spp, p->override_uvt != 0, p->uvt, pat);
     const int W = c->h.width, H = c->h.height;
     const size_t nq = (size_t)W * H * M;
-    if (nq >= ((size_t)1 << 32)) return c->fail(TLR_E_INVALID, "too many output samples");
-
+    if (nq >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "too many output samples");
     // waves resident across the device: CUs x 8
     const int waves = std::max(4, c->prop.multiProcessorCount * 8);
-    const size_t qbytes = pat.size() * 4 + 256 + nq * sizeof(float4) + 256 + (size_t)W * H * 16 + 256 +
-                          nq * 4 + 256 + (size_t)waves * kWaveScratchWords * 4 + 4096;
+    size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
+                    (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
+    if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 3) + 4 * 256;
     HIPCHK(c, c->qarena.reserve(qbytes));
     c->qarena.reset();
     c->d_pattern = c->qarena.alloc<float>(pat.size());
@@ -452,6 +484,15 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     c->d_img = c->qarena.alloc<float>((size_t)W * H * 4);
     c->fail_list = c->qarena.alloc<unsigned>(nq);
     c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
+    c->features = c->logits = c->labels = nullptr;
+    c->route_list = nullptr;
+    if (gate_buffers) {
+        c->features = c->qarena.alloc<float>(nq * kGateFeatures);
+        c->logits = c->qarena.alloc<float>(nq);
+        c->labels = c->qarena.alloc<float>(nq);
+        c->route_list = c->qarena.alloc<unsigned>(nq);
+        if (!c->route_list) return c->fail(TLR_E_HIP, "query arena exhausted");
+    }
     if (!c->scratch) return c->fail(TLR_E_HIP, "query arena exhausted");
     c->qcol_n = nq;
     c->num_waves = waves;
@@ -460,8 +501,8 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
     HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
     HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
-
-    QueryParams Q;
+    HIPCHK(c, hipMemsetAsync(c->d_route_count, 0, sizeof(unsigned), s));
+    memset(&Q, 0, sizeof(Q));
     Q.W = W; Q.H = H; Q.M = M;
     Q.spp_pow2 = c->spp_pow2;
     Q.cc0 = c->cc0; Q.cc1 = c->cc1;
@@ -480,23 +521,64 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     Q.next_query = c->d_counter;
     Q.scratch = c->scratch;
     Q.num_waves = waves;
-    Q.qlist = nullptr;
-    Q.qcount = nullptr;
     Q.fail_list = c->fail_list;
     Q.fail_count = c->d_fail_count;
+    return TLR_OK;
+}
+
+// Plain method over all queries (or over Q.qlist when list != nullptr):
+// fast 1R lane kernel, then the general kernel for what it hands over.
+static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* list, const unsigned* count,
+                            bool stats_mode, int path) {
+    hipStream_t s = c->stream;
+    if (path == 1 && !list) {
+        HIPCHK(c, launch_query_general(Q0, s));
+        HIPCHK(c, hipEventRecord(c->ev[5], s));
+        return TLR_OK;
+    }
+    QueryParams F = Q0;
+    if (list) {
+        F.fast_list = 1;
+        F.qlist = list;
+        F.qcount = count;
+    }
+    HIPCHK(c, launch_query_fast(F, stats_mode, s));
+    HIPCHK(c, hipEventRecord(c->ev[5], s));
+    QueryParams G = Q0;
+    G.qlist = c->fail_list;
+    G.qcount = c->d_fail_count;
+    G.labels = nullptr;
+    HIPCHK(c, launch_query_general(G, s));
+    return TLR_OK;
+}
+
+tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, int out_on_device, tlr_stats* stats) {
+    if (!c || !p) return TLR_E_INVALID;
+    if (p->gate && !c->has_gate) return c->fail(TLR_E_STATE, "gate requested but no gate parameters set");
+    QueryParams Q;
+    tlr_status st = prepare_queries(c, p, p->gate != 0, Q);
+    if (st != TLR_OK) return st;
+    hipStream_t s = c->stream;
+    const int W = c->h.width, H = c->h.height, M = c->last_M;
+    const size_t nq = c->qcol_n;
 
     HIPCHK(c, hipEventRecord(c->ev[2], s));
-    if (p->path == 1) {
-        HIPCHK(c, launch_query_general(Q, s));
-        HIPCHK(c, hipEventRecord(c->ev[5], s));
+    if (p->gate) {
+        // direct filter + features for every output sample, gate, then the
+        // plain method for the rejected ones
+        QueryParams D = Q;
+        D.features = c->features;
+        HIPCHK(c, launch_query_direct(D, s));
+        HIPCHK(c, hipEventRecord(c->ev[6], s));
+        HIPCHK(c, launch_gate_forward(c->features, (int)nq, c->gate_p, c->logits, s));
+        HIPCHK(c, launch_gate_route(c->logits, c->features, (unsigned)nq, c->gate_thr, c->route_list,
+                                    c->d_route_count, s));
+        HIPCHK(c, hipEventRecord(c->ev[7], s));
+        st = run_plain(c, Q, c->route_list, c->d_route_count, false, 0);
     } else {
-        HIPCHK(c, launch_query_fast(Q, p->collect_stats != 0, s));
-        HIPCHK(c, hipEventRecord(c->ev[5], s));
-        QueryParams G = Q;
-        G.qlist = c->fail_list;
-        G.qcount = c->d_fail_count;
-        HIPCHK(c, launch_query_general(G, s));
+        st = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0, p->path);
     }
+    if (st != TLR_OK) return st;
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     float* dst = out_on_device ? out : c->d_img;
     HIPCHK(c, launch_combine(c->qcol, W, H, M, dst, s));
@@ -504,9 +586,10 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     if (!out_on_device && out)
         HIPCHK(c, hipMemcpyAsync(out, c->d_img, (size_t)W * H * 16, hipMemcpyDeviceToHost, s));
     unsigned long long hs[kNumStats];
-    unsigned nfail = 0;
+    unsigned nfail = 0, nroute = 0;
     HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(&nfail, c->d_fail_count, sizeof(nfail), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&nroute, c->d_route_count, sizeof(nroute), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (stats) {
         memset(stats, 0, sizeof(*stats));
@@ -523,10 +606,17 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         float ms = 0;
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
-        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) stats->fast_ms = ms;
-        if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) stats->general_ms = ms;
-        stats->general_queries = p->path == 1 ? (double)nq : (double)nfail;
         if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;
+        if (p->gate) {
+            if (hipEventElapsedTime(&ms, c->ev[2], c->ev[6]) == hipSuccess) stats->direct_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) stats->gate_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev[7], c->ev[5]) == hipSuccess) stats->fast_ms = ms;
+            stats->gate_rejected = (double)nroute;
+        } else if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) {
+            stats->fast_ms = ms;
+        }
+        if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) stats->general_ms = ms;
+        stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)nfail;
         stats->total_ms = stats->build_ms + stats->query_ms + stats->combine_ms;
         stats->num_leaves = c->L;
         stats->num_samples = c->nb;
@@ -632,6 +722,201 @@ int tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, float* o
     const int M = make_output_pattern(out_spp, override_uvt != 0, uvt ? uvt : zero, pat);
     if (out5) memcpy(out5, pat.data(), pat.size() * 4);
     return M;
+}
+
+
+// ---------------------------------------------------------------------------
+// Learned gate
+// ---------------------------------------------------------------------------
+tlr_status tlr_gate_set_params(tlr_ctx* c, const float* params) {
+    if (!c || !params) return TLR_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(c->gate_p, params, sizeof(float) * kGateParams, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->gate_m, 0, sizeof(float) * kGateParams));
+    HIPCHK(c, hipMemset(c->gate_v, 0, sizeof(float) * kGateParams));
+    c->gate_step = 0;
+    c->has_gate = true;
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_get_params(tlr_ctx* c, float* params) {
+    if (!c || !params) return TLR_E_INVALID;
+    if (!c->has_gate) return c->fail(TLR_E_STATE, "no gate parameters");
+    HIPCHK(c, hipMemcpy(params, c->gate_p, sizeof(float) * kGateParams, hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_set_threshold(tlr_ctx* c, float thr) {
+    if (!c) return TLR_E_INVALID;
+    c->gate_thr = thr;
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_forward(tlr_ctx* c, const float* X, int64_t n, float* logits) {
+    if (!c || !X || !logits || n < 0) return TLR_E_INVALID;
+    if (!c->has_gate) return c->fail(TLR_E_STATE, "no gate parameters");
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dX = nullptr, *dz = nullptr;
+    HIPCHK(c, hipMalloc(&dX, sizeof(float) * kGateIn * std::max<int64_t>(n, 1)));
+    HIPCHK(c, hipMalloc(&dz, sizeof(float) * std::max<int64_t>(n, 1)));
+    hipError_t e = hipMemcpy(dX, X, sizeof(float) * kGateIn * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_gate_forward(dX, (int)n, c->gate_p, dz, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(logits, dz, sizeof(float) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dX);
+    (void)hipFree(dz);
+    if (e != hipSuccess) return c->fail(TLR_E_HIP, "gate forward: %s", hipGetErrorString(e));
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_grad(tlr_ctx* c, const float* X, const float* y, int64_t n, float pos_w, float* grad,
+                         float* loss) {
+    if (!c || !X || !y || !grad || n <= 0) return TLR_E_INVALID;
+    if (!c->has_gate) return c->fail(TLR_E_STATE, "no gate parameters");
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dX = nullptr, *dy = nullptr;
+    HIPCHK(c, hipMalloc(&dX, sizeof(float) * kGateIn * n));
+    HIPCHK(c, hipMalloc(&dy, sizeof(float) * n));
+    hipError_t e = hipMemcpy(dX, X, sizeof(float) * kGateIn * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dy, y, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c->gate_g, 0, sizeof(float) * kGateParams);
+    if (e == hipSuccess) e = hipMemset(c->gate_loss, 0, sizeof(float));
+    if (e == hipSuccess) e = launch_gate_grad(dX, dy, nullptr, (int)n, c->gate_p, c->gate_g, c->gate_loss, pos_w,
+                                              c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(grad, c->gate_g, sizeof(float) * kGateParams, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && loss) e = hipMemcpy(loss, c->gate_loss, sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dX);
+    (void)hipFree(dy);
+    if (e != hipSuccess) return c->fail(TLR_E_HIP, "gate grad: %s", hipGetErrorString(e));
+    return TLR_OK;
+}
+
+// Runs the plain method (with labels) and the direct filter (with features)
+// over every output sample of the current build.
+static tlr_status gate_pass(tlr_ctx* c, const tlr_recon_params* p, float* plain_dev) {
+    tlr_recon_params pp = *p;
+    pp.collect_stats = 0;
+    pp.gate = 0;
+    QueryParams Q;
+    tlr_status st = prepare_queries(c, &pp, true, Q);
+    if (st != TLR_OK) return st;
+    QueryParams L = Q;
+    L.labels = c->labels;
+    st = run_plain(c, L, nullptr, nullptr, false, 0);
+    if (st != TLR_OK) return st;
+    if (plain_dev)
+        HIPCHK(c, hipMemcpyAsync(plain_dev, c->qcol, c->qcol_n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    QueryParams D = Q;
+    D.features = c->features;
+    HIPCHK(c, launch_query_direct(D, c->stream));
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_collect(tlr_ctx* c, const tlr_recon_params* p, int stride, float* X, float* y, float* direct,
+                            float* plain, int64_t* n) {
+    if (!c || !p || stride <= 0) return TLR_E_INVALID;
+    float* plain_dev = nullptr;
+    if (plain) {
+        const size_t nq = (size_t)c->h.width * c->h.height * (p->override_uvt ? 1 : p->out_spp);
+        HIPCHK(c, hipMalloc(&plain_dev, nq * sizeof(float4)));
+    }
+    tlr_status st = gate_pass(c, p, plain_dev);
+    if (st != TLR_OK) {
+        if (plain_dev) (void)hipFree(plain_dev);
+        return st;
+    }
+    hipStream_t s = c->stream;
+    const size_t rows = (c->qcol_n + stride - 1) / stride;
+    hipError_t e = hipSuccess;
+    if (X && e == hipSuccess)
+        e = hipMemcpy2DAsync(X, kGateFeatures * 4, c->features, (size_t)stride * kGateFeatures * 4, kGateFeatures * 4,
+                             rows, hipMemcpyDeviceToHost, s);
+    if (y && e == hipSuccess)
+        e = hipMemcpy2DAsync(y, 4, c->labels, (size_t)stride * 4, 4, rows, hipMemcpyDeviceToHost, s);
+    if (direct && e == hipSuccess)
+        e = hipMemcpy2DAsync(direct, 16, c->qcol, (size_t)stride * 16, 16, rows, hipMemcpyDeviceToHost, s);
+    if (plain && e == hipSuccess)
+        e = hipMemcpy2DAsync(plain, 16, plain_dev, (size_t)stride * 16, 16, rows, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (plain_dev) (void)hipFree(plain_dev);
+    if (e != hipSuccess) return c->fail(TLR_E_HIP, "gate collect: %s", hipGetErrorString(e));
+    if (n) *n = (int64_t)rows;
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_append_training(tlr_ctx* c, const tlr_recon_params* p, int stride) {
+    if (!c || !p || stride <= 0) return TLR_E_INVALID;
+    tlr_status st = gate_pass(c, p, nullptr);
+    if (st != TLR_OK) return st;
+    const int64_t rows = (int64_t)((c->qcol_n + stride - 1) / stride);
+    if (c->train_n + rows > c->train_cap) {
+        const int64_t cap = std::max<int64_t>(c->train_n + rows, c->train_cap * 2);
+        float *nX = nullptr, *ny = nullptr;
+        HIPCHK(c, hipMalloc(&nX, sizeof(float) * kGateFeatures * cap));
+        HIPCHK(c, hipMalloc(&ny, sizeof(float) * cap));
+        if (c->train_n) {
+            HIPCHK(c, hipMemcpyAsync(nX, c->train_X, sizeof(float) * kGateFeatures * c->train_n,
+                                     hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(ny, c->train_y, sizeof(float) * c->train_n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->train_X) (void)hipFree(c->train_X);
+        if (c->train_y) (void)hipFree(c->train_y);
+        c->train_X = nX;
+        c->train_y = ny;
+        c->train_cap = cap;
+    }
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemcpy2DAsync(c->train_X + kGateFeatures * c->train_n, kGateFeatures * 4, c->features,
+                               (size_t)stride * kGateFeatures * 4, kGateFeatures * 4, rows, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpy2DAsync(c->train_y + c->train_n, 4, c->labels, (size_t)stride * 4, 4, rows,
+                               hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->train_n += rows;
+    return TLR_OK;
+}
+
+tlr_status tlr_gate_clear_training(tlr_ctx* c) {
+    if (!c) return TLR_E_INVALID;
+    c->train_n = 0;
+    return TLR_OK;
+}
+
+int64_t tlr_gate_training_size(tlr_ctx* c) { return c ? c->train_n : 0; }
+
+tlr_status tlr_gate_train(tlr_ctx* c, int epochs, int batch, float lr, float pos_w, uint64_t seed,
+                          float* epoch_loss) {
+    if (!c || epochs < 0 || batch <= 0) return TLR_E_INVALID;
+    if (!c->has_gate) return c->fail(TLR_E_STATE, "initialise the gate parameters first");
+    if (c->train_n == 0) return c->fail(TLR_E_STATE, "empty training set");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = c->train_n;
+    if (c->train_idx) (void)hipFree(c->train_idx);
+    HIPCHK(c, hipMalloc(&c->train_idx, sizeof(int) * n));
+    std::vector<int> perm((size_t)n);
+    std::mt19937_64 rng(seed);
+    for (int e = 0; e < epochs; e++) {
+        std::iota(perm.begin(), perm.end(), 0);
+        std::shuffle(perm.begin(), perm.end(), rng);
+        HIPCHK(c, hipMemcpyAsync(c->train_idx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipMemsetAsync(c->gate_loss, 0, sizeof(float), s));
+        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+            const int nb = (int)std::min<int64_t>(batch, n - b0);
+            HIPCHK(c, hipMemsetAsync(c->gate_g, 0, sizeof(float) * kGateParams, s));
+            HIPCHK(c, launch_gate_grad(c->train_X, c->train_y, c->train_idx + b0, nb, c->gate_p, c->gate_g,
+                                       c->gate_loss, pos_w, s));
+            c->gate_step++;
+            HIPCHK(c, launch_gate_adam(c->gate_p, c->gate_g, c->gate_m, c->gate_v, lr, 0.9f, 0.999f, 1e-8f,
+                                       c->gate_step, 1.0f / (float)nb, s));
+        }
+        float l = 0;
+        HIPCHK(c, hipMemcpyAsync(&l, c->gate_loss, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (epoch_loss) epoch_loss[e] = l / (float)n;
+    }
+    return TLR_OK;
 }
 
 }  // extern "C"
