@@ -131,6 +131,11 @@ tlr_status  tlr_copy_layout(tlr_ctx* ctx, int32_t* src);
 tlr_status  tlr_copy_initial_leaves(tlr_ctx* ctx, int32_t* out6);
 /* leaves: s0,s1 and 14 bound floats (12 plane coefficients, tspan) */
 tlr_status  tlr_copy_leaves(tlr_ctx* ctx, int32_t* s01, float* bounds14);
+/* hierarchy over the valid leaves (num_valid = n): leaf id of each slot (n),
+ * internal node children (2 per node, n-1 nodes; a leaf child is its leaf id
+ * with bit 31 set) and internal node bounds (14 floats each).  Root: internal
+ * node n/2-1 (n >= 2) or the single leaf. */
+tlr_status  tlr_copy_hierarchy(tlr_ctx* ctx, int32_t* leaf_order, int32_t* children2, float* bounds14);
 /* per output sample colour of the last reconstruct, (y,x,i) order, 4 floats */
 tlr_status  tlr_copy_query_colors(tlr_ctx* ctx, float* out4);
 /* output pattern: 64*M templates x,y,u,v,t (host computation) */

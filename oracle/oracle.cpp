@@ -846,107 +846,108 @@ void build_leaf_cell(orc_ctx* c, int inode, int offset, LeafScratch& sc, std::ve
     }
 }
 
-// Morton code of a leaf's centre at (t=0.5, u=v=0).
-inline uint32_t spread16(uint32_t v) {
-    v &= 0xFFFF;
-    v = (v | (v << 8)) & 0x00FF00FF;
-    v = (v | (v << 4)) & 0x0F0F0F0F;
-    v = (v | (v << 2)) & 0x33333333;
-    v = (v | (v << 1)) & 0x55555555;
-    return v;
+// Hierarchy over the valid leaves: a balanced binary tree of recursive
+// median splits.  Each node splits its leaves at the median of the feature
+// with the widest pixel extent among: centre x, centre y at (t=0.5, u=v=0),
+// 2x the CoC slope (u spans 2), and the t-slopes in x and y.  Grouping leaves
+// of similar slopes keeps the merged planes tight at u=+-1 and t=0,1.  The
+// order inside a node is a stable sort on the feature quantised to
+// kd_qbits(level) bits; internal node g is the one whose split falls between
+// sorted positions g and g+1.  (Replaces emitNodes, ReconstructionTreeBuilder
+// .cpp:409-451; which leaves a query finds does not depend on the hierarchy.)
+struct KdFeat {
+    float f[5];
+};
+KdFeat kd_feature(const TLB& b) {
+    KdFeat k;
+    k.f[0] = 0.5f * (b.eval(0, 0.5f, 0.0f) + b.eval(1, 0.5f, 0.0f));
+    k.f[1] = 0.5f * (b.eval(2, 0.5f, 0.0f) + b.eval(3, 0.5f, 0.0f));
+    k.f[2] = b.p[0][1] + b.p[1][1];
+    k.f[3] = 0.5f * (b.p[0][0] + b.p[1][0]);
+    k.f[4] = 0.5f * (b.p[2][0] + b.p[3][0]);
+    return k;
 }
-inline uint32_t quant16(float v, float lo, float sc) {
-    float f = (v - lo) * sc;
+inline int kd_qbits(int level) { return std::min(16, 32 - level); }
+inline uint32_t kd_quant(float v, float lo, float hi, int bits) {
+    if (!(hi > lo)) return 0;
+    const uint32_t m = (1u << bits) - 1u;
+    const float sc = (float)m / (hi - lo);
+    const float f = (v - lo) * sc;
     if (!(f >= 0.0f)) return 0;
-    if (f >= 65535.0f) return 65535;
+    if (f >= (float)m) return m;
     return (uint32_t)f;
 }
-uint32_t leaf_morton(const TLB& b, const float bbmin[2], const float bbmax[2]) {
-    float cx = 0.5f * (b.eval(0, 0.5f, 0.0f) + b.eval(1, 0.5f, 0.0f));
-    float cy = 0.5f * (b.eval(2, 0.5f, 0.0f) + b.eval(3, 0.5f, 0.0f));
-    float ex = bbmax[0] - bbmin[0], ey = bbmax[1] - bbmin[1];
-    float sx = (ex > 0.0f) ? 65535.0f / ex : 0.0f;
-    float sy = (ey > 0.0f) ? 65535.0f / ey : 0.0f;
-    return spread16(quant16(cx, bbmin[0], sx)) | (spread16(quant16(cy, bbmin[1], sy)) << 1);
+
+void kd_order(std::vector<int>& ord, const std::vector<KdFeat>& F, int lo, int hi, int level) {
+    if (hi - lo < 2) return;
+    float mn[5], mxv[5];
+    for (int d = 0; d < 5; d++) {
+        mn[d] = F[ord[lo]].f[d];
+        mxv[d] = mn[d];
+    }
+    for (int k = lo + 1; k < hi; k++)
+        for (int d = 0; d < 5; d++) {
+            mn[d] = std::min(mn[d], F[ord[k]].f[d]);
+            mxv[d] = std::max(mxv[d], F[ord[k]].f[d]);
+        }
+    int best = 0;
+    float bext = mxv[0] - mn[0];
+    for (int d = 1; d < 5; d++)
+        if (mxv[d] - mn[d] > bext) {
+            bext = mxv[d] - mn[d];
+            best = d;
+        }
+    const int bits = kd_qbits(level);
+    std::vector<uint32_t> q(hi - lo);
+    for (int k = lo; k < hi; k++) q[k - lo] = kd_quant(F[ord[k]].f[best], mn[best], mxv[best], bits);
+    std::vector<int> idx(hi - lo);
+    for (int k = 0; k < hi - lo; k++) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return q[a] < q[b]; });
+    std::vector<int> tmp(hi - lo);
+    for (int k = 0; k < hi - lo; k++) tmp[k] = ord[lo + idx[k]];
+    std::copy(tmp.begin(), tmp.end(), ord.begin() + lo);
+    const int mid = (lo + hi) / 2;
+    kd_order(ord, F, lo, mid, level + 1);
+    kd_order(ord, F, mid, hi, level + 1);
 }
 
-inline int clz32(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+// Internal node g <-> range [lo,hi) whose split mid-1 == g.
+void kd_link(orc_ctx* c, int lo, int hi, int n) {
+    const int mid = (lo + hi) / 2;
+    const int g = mid - 1;
+    Inner& nd = c->inner[g];
+    nd.c0 = (mid - lo == 1) ? (c->leaf_order[lo] | LEAF_TAG) : ((lo + mid) / 2 - 1);
+    nd.c1 = (hi - mid == 1) ? (c->leaf_order[mid] | LEAF_TAG) : ((mid + hi) / 2 - 1);
+    if (mid - lo > 1) kd_link(c, lo, mid, n);
+    if (hi - mid > 1) kd_link(c, mid, hi, n);
+    const TLB& a = (nd.c0 & LEAF_TAG) ? c->leaves[nd.c0 & ~LEAF_TAG].b : c->inner[nd.c0].b;
+    const TLB& b = (nd.c1 & LEAF_TAG) ? c->leaves[nd.c1 & ~LEAF_TAG].b : c->inner[nd.c1].b;
+    nd.b = merge_tlb(a, b);
+}
 
 void build_lbvh(orc_ctx* c) {
     c->leaf_order.clear();
     c->inner.clear();
     c->root = -1;
-    std::vector<uint32_t> code;
     std::vector<int> ids;
     for (int i = 0; i < (int)c->leaves.size(); i++)
         if (c->leaves[i].b.valid()) ids.push_back(i);
     const int n = (int)ids.size();
     if (n == 0) return;
-    std::vector<uint32_t> mc(n);
-    for (int k = 0; k < n; k++) mc[k] = leaf_morton(c->leaves[ids[k]].b, c->bbmin, c->bbmax);
-    std::vector<int> perm(n);
-    for (int k = 0; k < n; k++) perm[k] = k;
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return mc[a] < mc[b]; });
+    std::vector<KdFeat> F(n);
+    for (int k = 0; k < n; k++) F[k] = kd_feature(c->leaves[ids[k]].b);
+    std::vector<int> ord(n);
+    for (int k = 0; k < n; k++) ord[k] = k;
+    kd_order(ord, F, 0, n, 0);
     c->leaf_order.resize(n);
-    code.resize(n);
-    for (int k = 0; k < n; k++) {
-        c->leaf_order[k] = ids[perm[k]];
-        code[k] = mc[perm[k]];
-    }
+    for (int k = 0; k < n; k++) c->leaf_order[k] = ids[ord[k]];
     if (n == 1) {
         c->root = c->leaf_order[0] | LEAF_TAG;
         return;
     }
-    // Karras 2012 radix tree; equal codes are ordered by position.
-    auto delta = [&](int i, int j) -> int {
-        if (j < 0 || j >= n) return -1;
-        if (code[i] == code[j]) return 32 + clz32((uint32_t)i ^ (uint32_t)j);
-        return clz32(code[i] ^ code[j]);
-    };
     c->inner.assign(n - 1, Inner());
-    std::vector<int> parent_of_inner(n - 1, -1);
-    for (int i = 0; i < n - 1; i++) {
-        int d = (delta(i, i + 1) - delta(i, i - 1)) >= 0 ? 1 : -1;
-        int dmin = delta(i, i - d);
-        int lmax = 2;
-        while (delta(i, i + lmax * d) > dmin) lmax *= 2;
-        int l = 0;
-        for (int t = lmax / 2; t >= 1; t /= 2)
-            if (delta(i, i + (l + t) * d) > dmin) l += t;
-        int j = i + l * d;
-        int dn = delta(i, j);
-        int s = 0;
-        int tt = l;
-        do {
-            tt = (tt + 1) >> 1;
-            if (delta(i, i + (s + tt) * d) > dn) s += tt;
-        } while (tt > 1);
-        int g = i + s * d + std::min(d, 0);
-        int lo = std::min(i, j), hi = std::max(i, j);
-        c->inner[i].c0 = (lo == g) ? (c->leaf_order[g] | LEAF_TAG) : g;
-        c->inner[i].c1 = (hi == g + 1) ? (c->leaf_order[g + 1] | LEAF_TAG) : (g + 1);
-    }
-    // Bottom-up bounds: merge(child0, child1) in post-order.
-    std::vector<int> stack;
-    std::vector<char> done(n - 1, 0);
-    stack.push_back(0);
-    while (!stack.empty()) {
-        int k = stack.back();
-        Inner& nd = c->inner[k];
-        bool ready = true;
-        for (int ch : {nd.c0, nd.c1})
-            if (!(ch & LEAF_TAG) && !done[ch]) {
-                stack.push_back(ch);
-                ready = false;
-            }
-        if (!ready) continue;
-        stack.pop_back();
-        const TLB& a = (nd.c0 & LEAF_TAG) ? c->leaves[nd.c0 & ~LEAF_TAG].b : c->inner[nd.c0].b;
-        const TLB& b = (nd.c1 & LEAF_TAG) ? c->leaves[nd.c1 & ~LEAF_TAG].b : c->inner[nd.c1].b;
-        nd.b = merge_tlb(a, b);
-        done[k] = 1;
-    }
-    c->root = 0;
+    kd_link(c, 0, n, n);
+    c->root = n / 2 - 1;
 }
 
 // Node::intersect: Reconstruction.hpp:204-216
@@ -960,6 +961,8 @@ inline bool node_hit(const TLB& b, float ox, float oy, float ou, float ov, float
 
 struct OSample { float x, y, u, v, t; };
 
+long long g_nodes_tested = 0;  // instrumentation: hierarchy nodes tested
+
 void find_leaves(const orc_ctx* c, const OSample& o, float R, std::vector<int>& out) {
     out.clear();
     if (c->root == -1) return;
@@ -968,6 +971,7 @@ void find_leaves(const orc_ctx* c, const OSample& o, float R, std::vector<int>& 
     while (!st.empty()) {
         int id = st.back();
         st.pop_back();
+        g_nodes_tested++;
         if (id & LEAF_TAG) {
             int lf = id & ~LEAF_TAG;
             if (node_hit(c->leaves[lf].b, o.x, o.y, o.u, o.v, o.t, R)) out.push_back(lf);
@@ -1660,6 +1664,12 @@ int orc_query(orc_ctx* c, const orc_recon_params* p, int x, int y, int i, orc_qu
     out->dispersion = r.disp;
     for (int k = 0; k < 4; k++) out->color[k] = r.color[k];
     return 0;
+}
+
+long long orc_nodes_tested(int reset) {
+    long long v = g_nodes_tested;
+    if (reset) g_nodes_tested = 0;
+    return v;
 }
 
 int orc_leaves_bruteforce(orc_ctx* c, const float* o5, float R, int32_t* leaves, int max_leaves) {

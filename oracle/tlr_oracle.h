@@ -128,6 +128,10 @@ int orc_query(orc_ctx* c, const orc_recon_params* p, int x, int y, int i, orc_qu
  * checking that the LBVH traversal loses nothing. */
 int orc_leaves_bruteforce(orc_ctx* c, const float* o5, float R, int32_t* leaves, int max_leaves);
 
+/* Instrumentation: hierarchy nodes tested by orc_gather/orc_reconstruct so
+ * far (single-threaded use); reset != 0 clears the counter. */
+long long orc_nodes_tested(int reset);
+
 /* Primitive restatements exposed for known-answer tests. */
 uint32_t orc_hash_bits(uint32_t a, uint32_t b);
 void     orc_ranrot(uint32_t seed, int n, uint32_t* out);

@@ -29,7 +29,7 @@ EXPORTED = [
     "tlr_create", "tlr_destroy", "tlr_last_error", "tlr_load_sample_file", "tlr_save_sample_file",
     "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_sync",
     "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
-    "tlr_copy_leaves", "tlr_copy_query_colors", "tlr_output_pattern",
+    "tlr_copy_leaves", "tlr_copy_hierarchy", "tlr_copy_query_colors", "tlr_output_pattern",
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
@@ -126,6 +126,7 @@ def lib():
         L.tlr_copy_layout.argtypes = [V, V]
         L.tlr_copy_initial_leaves.argtypes = [V, V]
         L.tlr_copy_leaves.argtypes = [V, V, V]
+        L.tlr_copy_hierarchy.argtypes = [V, V, V, V]
         L.tlr_copy_query_colors.argtypes = [V, V]
         L.tlr_output_pattern.restype = C.c_int
         L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
@@ -378,6 +379,16 @@ class Ctx:
         b = np.empty((max(n, 1), 14), np.float32)
         self._chk(lib().tlr_copy_leaves(self.h, _p(s01), _p(b)))
         return s01[:n], b[:n]
+
+    def hierarchy(self):
+        """(leaf order, internal children (n-1, 2), internal bounds (n-1, 14));
+        leaf children carry bit 31."""
+        n = self.build_info().num_valid_leaves
+        order = np.empty(max(n, 1), np.int32)
+        ch = np.empty((max(n - 1, 1), 2), np.int32)
+        b = np.empty((max(n - 1, 1), 14), np.float32)
+        self._chk(lib().tlr_copy_hierarchy(self.h, _p(order), _p(ch), _p(b)))
+        return order[:n], ch[:max(n - 1, 0)], b[:max(n - 1, 0)]
 
     def query_colors(self, out_spp):
         out = np.empty((self.sbuf.height, self.sbuf.width, out_spp, 4), np.float32)

@@ -16,6 +16,8 @@
 //   k_lbvh_*                              replaces emitNodes (:409-451) and the
 //                                          task merge (Reconstruction.cpp:110-203)
 #include "build.hpp"
+#include "prims.hpp"
+#include <utility>
 
 namespace tlr {
 namespace {
@@ -536,82 +538,267 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit(const uint32_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// LBVH over valid leaves (Karras 2012).
+// Hierarchy over the valid leaves: balanced binary tree of recursive median
+// splits.  Every node splits its leaves at the median of the feature with the
+// widest pixel extent among: centre x, centre y at (t=0.5, u=v=0), 2x the CoC
+// slope (u spans 2), and the t-slopes in x and y; grouping leaves of similar
+// slopes keeps the merged planes tight at u=+-1 and t=0,1.  Built level by
+// level: per-node extents, then a stable radix sort of
+// (node at this level, feature quantised to kd_qbits(level) bits), then the
+// halving split.  Internal node g is the node whose split lies between sorted
+// positions g and g+1.  Replaces emitNodes (ReconstructionTreeBuilder.cpp:
+// 409-451); which leaves a query finds does not depend on the hierarchy.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t spread16(uint32_t v) {
-    v &= 0xFFFFu;
-    v = (v | (v << 8)) & 0x00FF00FFu;
-    v = (v | (v << 4)) & 0x0F0F0F0Fu;
-    v = (v | (v << 2)) & 0x33333333u;
-    v = (v | (v << 1)) & 0x55555555u;
-    return v;
-}
-__device__ __forceinline__ uint32_t quant16(float v, float lo, float sc) {
+__device__ __forceinline__ int kd_qbits(int level) { return level < 16 ? 16 : 32 - level; }
+
+__device__ __forceinline__ uint32_t kd_quant(float v, float lo, float hi, int bits) {
+    if (!(hi > lo)) return 0;
+    const uint32_t m = (1u << bits) - 1u;
+    const float sc = (float)m / (hi - lo);
     const float f = (v - lo) * sc;
     if (!(f >= 0.0f)) return 0;
-    if (f >= 65535.0f) return 65535;
+    if (f >= (float)m) return m;
     return (uint32_t)f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_lbvh_keys(const uint32_t* __restrict__ valid_flag,
-                                                      const uint32_t* __restrict__ valid_pos, int L,
-                                                      const Bounds* __restrict__ leaf_b, const GridInfo* gi,
-                                                      uint32_t* __restrict__ codes, uint32_t* __restrict__ ids) {
+// Node (index at `level`, virtual below early leaves) containing position p.
+__device__ __forceinline__ int kd_node_at(int n, int level, int p, int& lo, int& hi) {
+    lo = 0;
+    hi = n;
+    int node = 0;
+    for (int k = 0; k < level; k++) {
+        if (hi - lo < 2) return node << (level - k);
+        const int mid = (lo + hi) >> 1;
+        if (p < mid) {
+            hi = mid;
+            node = node * 2;
+        } else {
+            lo = mid;
+            node = node * 2 + 1;
+        }
+    }
+    return node;
+}
+
+// Range of node `node` at `level` (empty for virtual nodes).
+__device__ __forceinline__ void kd_range(int n, int level, int node, int& lo, int& hi) {
+    lo = 0;
+    hi = n;
+    for (int k = level - 1; k >= 0; k--) {
+        if (hi - lo < 2) {
+            if (node & ((1 << (k + 1)) - 1)) lo = hi;
+            return;
+        }
+        const int mid = (lo + hi) >> 1;
+        if ((node >> k) & 1) lo = mid;
+        else hi = mid;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_kd_features(const uint32_t* __restrict__ valid_flag,
+                                                        const uint32_t* __restrict__ valid_pos, int L,
+                                                        const Bounds* __restrict__ leaf_b, float* __restrict__ feat,
+                                                        int nv, uint32_t* __restrict__ vid_leaf,
+                                                        uint32_t* __restrict__ vals) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= L || !valid_flag[i]) return;
     const Bounds& b = leaf_b[i];
-    const float cx = 0.5f * (plane_eval(b.p + 0, 0.5f, 0.0f) + plane_eval(b.p + 3, 0.5f, 0.0f));
-    const float cy = 0.5f * (plane_eval(b.p + 6, 0.5f, 0.0f) + plane_eval(b.p + 9, 0.5f, 0.0f));
-    const float bx0 = gi->bbmin[0], by0 = gi->bbmin[1];
-    const float ex = gi->bbmax[0] - bx0, ey = gi->bbmax[1] - by0;
-    const float sx = (ex > 0.0f) ? 65535.0f / ex : 0.0f;
-    const float sy = (ey > 0.0f) ? 65535.0f / ey : 0.0f;
-    const uint32_t code = spread16(quant16(cx, bx0, sx)) | (spread16(quant16(cy, by0, sy)) << 1);
-    const uint32_t p = valid_pos[i];
-    codes[p] = code;
-    ids[p] = (uint32_t)i;
+    const uint32_t v = valid_pos[i];
+    feat[0 * (size_t)nv + v] = 0.5f * (plane_eval(b.p + 0, 0.5f, 0.0f) + plane_eval(b.p + 3, 0.5f, 0.0f));
+    feat[1 * (size_t)nv + v] = 0.5f * (plane_eval(b.p + 6, 0.5f, 0.0f) + plane_eval(b.p + 9, 0.5f, 0.0f));
+    feat[2 * (size_t)nv + v] = b.p[1] + b.p[4];
+    feat[3 * (size_t)nv + v] = 0.5f * (b.p[0] + b.p[3]);
+    feat[4 * (size_t)nv + v] = 0.5f * (b.p[6] + b.p[9]);
+    vid_leaf[v] = (uint32_t)i;
+    vals[v] = v;
 }
 
-__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ code, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    const uint32_t a = code[i], b = code[j];
-    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
-    return __clz((int)(a ^ b));
+// Per-node split dimension and its range: info[node] = (dim, lo, hi).
+__device__ __forceinline__ void kd_pick(const float mn[5], const float mx[5], float4& out) {
+    int best = 0;
+    float bext = mx[0] - mn[0];
+#pragma unroll
+    for (int d = 1; d < 5; d++)
+        if (mx[d] - mn[d] > bext) {
+            bext = mx[d] - mn[d];
+            best = d;
+        }
+    out = make_float4(__int_as_float(best), mn[best], mx[best], 0.0f);
 }
 
-// Internal node i: children, parent links, range [lo,hi], split owner.
-__global__ __launch_bounds__(kBlock) void k_lbvh_nodes(const uint32_t* __restrict__ code, int n,
-                                                       int* __restrict__ child0, int* __restrict__ child1,
-                                                       int* __restrict__ parent, int2* __restrict__ range,
-                                                       int* __restrict__ split_owner) {
+// Order-preserving float <-> uint32 map for atomic min/max.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__global__ __launch_bounds__(kBlock) void k_kd_ext_init(int nodes, uint32_t* __restrict__ emn,
+                                                        uint32_t* __restrict__ emx) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n - 1) return;
-    const int d = (kdelta(code, n, i, i + 1) - kdelta(code, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = kdelta(code, n, i, i - d);
-    int lmax = 2;
-    while (kdelta(code, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (kdelta(code, n, i, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dn = kdelta(code, n, i, j);
-    int s = 0;
-    int tt = l;
-    do {
-        tt = (tt + 1) >> 1;
-        if (kdelta(code, n, i, i + (s + tt) * d) > dn) s += tt;
-    } while (tt > 1);
-    const int g = i + s * d + min(d, 0);
-    const int lo = min(i, j), hi = max(i, j);
-    // node ids: internal k -> k, leaf slot k -> (n-1)+k
-    const int c0 = (lo == g) ? (n - 1 + g) : g;
-    const int c1 = (hi == g + 1) ? (n - 1 + g + 1) : (g + 1);
-    child0[i] = c0;
-    child1[i] = c1;
-    parent[c0] = i;
-    parent[c1] = i;
-    range[i] = make_int2(lo, hi);
-    split_owner[g] = i;
+    if (i >= 5 * nodes) return;
+    emn[i] = 0xFFFFFFFFu;
+    emx[i] = 0u;
+}
+
+// Large nodes: chunks of kKdChunk positions.  A chunk inside one node is
+// block-reduced (one atomic per dimension); a chunk that straddles nodes
+// falls back to per-position atomics.
+constexpr int kKdChunk = 2048;
+__global__ __launch_bounds__(kBlock) void k_kd_ext_atomic(const float* __restrict__ feat, int nv,
+                                                          const uint32_t* __restrict__ vals, int level,
+                                                          uint32_t* __restrict__ emn, uint32_t* __restrict__ emx) {
+    const int p0 = blockIdx.x * kKdChunk;
+    const int p1 = min(nv, p0 + kKdChunk);
+    int lo, hi;
+    const int nf = kd_node_at(nv, level, p0, lo, hi);
+    const bool whole = (p1 <= hi);  // chunk lies in node nf
+    if (whole) {
+        if (hi - lo < 2) return;
+        float mn[5], mx[5];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            mn[d] = kFltMax;
+            mx[d] = -kFltMax;
+        }
+        for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+            const uint32_t v = vals[p];
+#pragma unroll
+            for (int d = 0; d < 5; d++) {
+                const float f = feat[d * (size_t)nv + v];
+                mn[d] = fminf(mn[d], f);
+                mx[d] = fmaxf(mx[d], f);
+            }
+        }
+        __shared__ float smn[5][kBlock / 64], smx[5][kBlock / 64];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            mn[d] = wave_min(mn[d]);
+            mx[d] = wave_max(mx[d]);
+            if ((threadIdx.x & 63) == 0) {
+                smn[d][threadIdx.x >> 6] = mn[d];
+                smx[d][threadIdx.x >> 6] = mx[d];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 5) {
+            const int d = threadIdx.x;
+            float a = smn[d][0], b = smx[d][0];
+            for (int w = 1; w < kBlock / 64; w++) {
+                a = fminf(a, smn[d][w]);
+                b = fmaxf(b, smx[d][w]);
+            }
+            atomicMin(&emn[5 * nf + d], f2ord(a));
+            atomicMax(&emx[5 * nf + d], f2ord(b));
+        }
+        return;
+    }
+    for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        const int node = kd_node_at(nv, level, p, lo, hi);
+        if (hi - lo < 2) continue;
+        const uint32_t v = vals[p];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const uint32_t o = f2ord(feat[d * (size_t)nv + v]);
+            atomicMin(&emn[5 * node + d], o);
+            atomicMax(&emx[5 * node + d], o);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_kd_ext_final(int nv, int level, int nodes,
+                                                         const uint32_t* __restrict__ emn,
+                                                         const uint32_t* __restrict__ emx,
+                                                         float4* __restrict__ info) {
+    const int node = blockIdx.x * kBlock + threadIdx.x;
+    if (node >= nodes) return;
+    int lo, hi;
+    kd_range(nv, level, node, lo, hi);
+    if (hi - lo < 2) return;
+    float mn[5], mx[5];
+#pragma unroll
+    for (int d = 0; d < 5; d++) {
+        mn[d] = ord2f(emn[5 * node + d]);
+        mx[d] = ord2f(emx[5 * node + d]);
+    }
+    kd_pick(mn, mx, info[node]);
+}
+
+// Small nodes: one thread per node.
+__global__ __launch_bounds__(kBlock) void k_kd_extent_thread(const float* __restrict__ feat, int nv,
+                                                             const uint32_t* __restrict__ vals, int level,
+                                                             int nodes, float4* __restrict__ info) {
+    const int node = blockIdx.x * kBlock + threadIdx.x;
+    if (node >= nodes) return;
+    int lo, hi;
+    kd_range(nv, level, node, lo, hi);
+    if (hi - lo < 2) return;
+    float mn[5], mx[5];
+#pragma unroll
+    for (int d = 0; d < 5; d++) {
+        mn[d] = kFltMax;
+        mx[d] = -kFltMax;
+    }
+    for (int p = lo; p < hi; p++) {
+        const uint32_t v = vals[p];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const float f = feat[d * (size_t)nv + v];
+            mn[d] = fminf(mn[d], f);
+            mx[d] = fmaxf(mx[d], f);
+        }
+    }
+    kd_pick(mn, mx, info[node]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_kd_keys(const float* __restrict__ feat, int nv,
+                                                    const uint32_t* __restrict__ vals, int level,
+                                                    const float4* __restrict__ info, uint32_t* __restrict__ keys) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= nv) return;
+    int lo, hi;
+    const int node = kd_node_at(nv, level, p, lo, hi);
+    const int bits = kd_qbits(level);
+    uint32_t q = 0;
+    if (hi - lo >= 2) {
+        const float4 in = info[node];
+        const int d = __float_as_int(in.x);
+        q = kd_quant(feat[d * (size_t)nv + vals[p]], in.y, in.z, bits);
+    }
+    keys[p] = ((uint32_t)node << bits) | q;
+}
+
+// Leaf slot -> leaf id after the last level.
+__global__ __launch_bounds__(kBlock) void k_kd_order(const uint32_t* __restrict__ vals,
+                                                     const uint32_t* __restrict__ vid_leaf, int nv,
+                                                     uint32_t* __restrict__ ids) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < nv) ids[p] = vid_leaf[vals[p]];
+}
+
+// Internal node g: children, parent links, range, split owner.
+__global__ __launch_bounds__(kBlock) void k_kd_nodes(int n, int* __restrict__ child0, int* __restrict__ child1,
+                                                     int* __restrict__ parent, int2* __restrict__ range,
+                                                     int* __restrict__ split_owner) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n - 1) return;
+    int lo = 0, hi = n, mid = n >> 1;
+    for (;;) {
+        mid = (lo + hi) >> 1;
+        if (g == mid - 1) break;
+        if (g < mid - 1) hi = mid;
+        else lo = mid;
+    }
+    const int c0 = (mid - lo == 1) ? (n - 1 + lo) : ((lo + mid) >> 1) - 1;
+    const int c1 = (hi - mid == 1) ? (n - 1 + mid) : ((mid + hi) >> 1) - 1;
+    child0[g] = c0;
+    child1[g] = c1;
+    parent[c0] = g;
+    parent[c1] = g;
+    range[g] = make_int2(lo, hi - 1);
+    split_owner[g] = g;
 }
 
 // Bottom-up bounds: the second child to arrive merges (child0, child1).
@@ -736,14 +923,41 @@ void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, c
     if (nb > 0) k_leaf_emit<<<TLR_GRID(nb), 0, s>>>(flags, leaf_pos, nb, bounds, pts, leaves, leaf_b, valid_flag);
 }
 
-void launch_lbvh_keys(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
-                      const GridInfo* gi, uint32_t* codes, uint32_t* ids, hipStream_t s) {
-    if (L > 0) k_lbvh_keys<<<TLR_GRID(L), 0, s>>>(valid_flag, valid_pos, L, leaf_b, gi, codes, ids);
-}
-
-void launch_lbvh_nodes(const uint32_t* codes, int n, int* c0, int* c1, int* parent, int2* range, int* split_owner,
-                       hipStream_t s) {
-    if (n > 1) k_lbvh_nodes<<<TLR_GRID(n - 1), 0, s>>>(codes, n, c0, c1, parent, range, split_owner);
+hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
+                         int nv, float* feat, uint32_t* vid_leaf, uint32_t* k0, uint32_t* v0, uint32_t* k1,
+                         uint32_t* v1, float4* info, uint32_t* emn, uint32_t* emx, uint32_t* hist, uint32_t* tmp, uint32_t* ids, int* c0, int* c1,
+                         int* parent, int2* range, int* split_owner, hipStream_t s) {
+    if (nv <= 0) return hipSuccess;
+    k_kd_features<<<TLR_GRID(L), 0, s>>>(valid_flag, valid_pos, L, leaf_b, feat, nv, vid_leaf, v0);
+    uint32_t* vals = v0;
+    uint32_t* keys = k0;
+    uint32_t* ko = k1;
+    uint32_t* vo = v1;
+    int levels = 0;
+    while ((1 << levels) < nv) levels++;
+    for (int level = 0; level < levels; level++) {
+        const int nodes = 1 << level;
+        // average node size decides how extents are reduced
+        if (nv / nodes > 32) {
+            k_kd_ext_init<<<TLR_GRID(5 * nodes), 0, s>>>(nodes, emn, emx);
+            k_kd_ext_atomic<<<(nv + kKdChunk - 1) / kKdChunk, kBlock, 0, s>>>(feat, nv, vals, level, emn, emx);
+            k_kd_ext_final<<<TLR_GRID(nodes), 0, s>>>(nv, level, nodes, emn, emx, info);
+        } else {
+            k_kd_extent_thread<<<TLR_GRID(nodes), 0, s>>>(feat, nv, vals, level, nodes, info);
+        }
+        k_kd_keys<<<TLR_GRID(nv), 0, s>>>(feat, nv, vals, level, info, keys);
+        const int bits = level + (level < 16 ? 16 : 32 - level);
+        bool first = true;
+        hipError_t e = radix_sort_pairs(keys, vals, ko, vo, (size_t)nv, bits, hist, tmp, &first, s);
+        if (e != hipSuccess) return e;
+        if (!first) {
+            std::swap(keys, ko);
+            std::swap(vals, vo);
+        }
+    }
+    k_kd_order<<<TLR_GRID(nv), 0, s>>>(vals, vid_leaf, nv, ids);
+    if (nv > 1) k_kd_nodes<<<TLR_GRID(nv - 1), 0, s>>>(nv, c0, c1, parent, range, split_owner);
+    return hipGetLastError();
 }
 
 void launch_lbvh_bounds(const uint32_t* ids, int n, const Bounds* leaf_b, const int* c0, const int* c1,

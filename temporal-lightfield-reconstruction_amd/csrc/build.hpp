@@ -28,10 +28,12 @@ void launch_surface_scan(const int2* offcnt, int nil, const float4* pa, const fl
                          hipStream_t s);
 void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
                       const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s);
-void launch_lbvh_keys(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
-                      const GridInfo* gi, uint32_t* codes, uint32_t* ids, hipStream_t s);
-void launch_lbvh_nodes(const uint32_t* codes, int n, int* c0, int* c1, int* parent, int2* range, int* split_owner,
-                       hipStream_t s);
+// Median-split tree over the nv valid leaves: ids[k] = leaf id of slot k,
+// internal nodes g in [0, nv-1) with children / parent / range / split owner.
+hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
+                         int nv, float* feat, uint32_t* vid_leaf, uint32_t* k0, uint32_t* v0, uint32_t* k1,
+                         uint32_t* v1, float4* info, uint32_t* emn, uint32_t* emx, uint32_t* hist, uint32_t* tmp, uint32_t* ids, int* c0, int* c1,
+                         int* parent, int2* range, int* split_owner, hipStream_t s);
 void launch_lbvh_bounds(const uint32_t* ids, int n, const Bounds* leaf_b, const int* c0, const int* c1,
                         const int* parent, Bounds* node_b, int* flags, hipStream_t s);
 void launch_lbvh_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,

@@ -20,6 +20,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
+// Marks a query in the hand-over list whose 1R pass already decided on 2R.
+constexpr unsigned kNeeds2R = 0x80000000u;
 enum { XPOS = 1, YPOS = 2, XNEG = 4, YNEG = 8 };
 
 __device__ __forceinline__ unsigned lanes_below(uint64_t m) {
@@ -120,7 +122,7 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     // --- traversal: wave-wide frontier stack ---
     if (Q.n_nodes > 0) {
         int sp = 1;
-        if (lane == 0) m.stack[0] = 0;
+        if (lane == 0) m.stack[0] = Q.root;
         wave_sync_mem();
         while (sp > 0) {
             const int take = sp < 64 ? sp : 64;
@@ -353,7 +355,13 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
         if (lane == 0) qi = atomicAdd(Q.next_query, 1u);
         qi = (unsigned)uni((int)qi);
         if (qi >= nq) break;
-        if (Q.qlist) qi = (unsigned)uni((int)Q.qlist[qi]);
+        bool skip1r = false;
+        if (Q.qlist) {
+            qi = (unsigned)uni((int)Q.qlist[qi]);
+            // top bit: the fast kernel already found that 2R is needed
+            skip1r = (qi & kNeeds2R) != 0;
+            qi &= ~kNeeds2R;
+        }
         const unsigned pix = qi / (unsigned)M;
         const int i = (int)(qi - pix * (unsigned)M);
         const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
@@ -369,13 +377,19 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
 
         // Filterer::reconstruct: ReconstructionOur.cpp:50-159
         float disp = dispersion_for_spp(Q.spp_pow2);
-        Gather g = gather(Q, m, o, disp, st);
-        int overflow = g.overflow;
-        const int n1 = g.total;
-        bool f2 = false;
-        if (!overflow) {
-            const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
-            f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
+        Gather g{0, 0, 0, 0};
+        int overflow = 0;
+        int n1 = 0;
+        bool f2 = true;
+        if (!skip1r) {
+            g = gather(Q, m, o, disp, st);
+            overflow = g.overflow;
+            n1 = g.total;
+            f2 = false;
+            if (!overflow) {
+                const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
+                f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
+            }
         }
         if (!overflow && f2) {
             g = gather(Q, m, o, disp * 2, st);
@@ -546,6 +560,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     const unsigned q = active ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
     bool fail = false;
     bool single = false;
+    bool needs2r = false;  // failed because the reference fetches 2R
     unsigned long long s_leaves = 0, s_in_leaves = 0, s_kept = 0, s_surf = 0;
     if (active) {
         const int M = Q.M;
@@ -564,7 +579,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
         const float R = disp;
         // --- stackless traversal ---
         int n = 0;
-        int node = Q.n_nodes > 0 ? 0 : -1;
+        int node = Q.n_nodes > 0 ? Q.root : -1;
         while (node >= 0) {
             const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
             const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
@@ -657,6 +672,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             single = nsurf == 1 && !stop;
             if (f2) {
                 fail = true;
+                needs2r = true;
             } else {
                 s_surf = (unsigned long long)nsurf;
                 const float oow = rcp(aw);
@@ -688,18 +704,21 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
         unsigned base = 0;
         if (lane == 0) base = atomicAdd(Q.fail_count, (unsigned)__popcll(fm));
         base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-        if (active && fail) Q.fail_list[base + lanes_below(fm)] = q;
+        if (active && fail) Q.fail_list[base + lanes_below(fm)] = q | (needs2r ? kNeeds2R : 0u);
     }
     if (STATS) {
+        // finished queries report everything; queries handed over for a 2R
+        // fetch report their (complete) 1R pass, the general kernel the rest
         const bool done = active && !fail;
-        unsigned long long v[6] = {done ? s_leaves : 0ull, done ? s_in_leaves : 0ull, done ? s_kept : 0ull,
-                                   done ? s_surf : 0ull, done ? 1ull : 0ull, 0ull};
+        const bool one = done || (active && needs2r);
+        unsigned long long v[6] = {one ? s_leaves : 0ull, one ? s_in_leaves : 0ull, one ? s_kept : 0ull,
+                                   done ? s_surf : 0ull, done ? 1ull : 0ull, one ? 1ull : 0ull};
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < 6; k++) {
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, 64);
         }
-        if ((tid & 63) == 0 && v[4]) {
+        if ((tid & 63) == 0 && v[5]) {
             atomicAdd(&Q.stats[ST_LEAF_NODES], v[0]);
             atomicAdd(&Q.stats[ST_SAMPLES_IN_LEAVES], v[1]);
             atomicAdd(&Q.stats[ST_WITHIN_1R], v[2]);
@@ -758,7 +777,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
     // depth separation relative to the first leaf / first kept sample met
     float key_first = 0.0f, w_first = 0.0f;
     int far_leaves = 0, off_depth = 0;
-    int node = Q.n_nodes > 0 ? 0 : -1;
+    int node = Q.n_nodes > 0 ? Q.root : -1;
     while (node >= 0) {
         const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
         const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];

@@ -30,6 +30,7 @@ struct QueryParams {
     int lens_gauss, time_gauss;
     int collect_stats;
     int n_nodes;              // traversal nodes (0: empty tree)
+    int root;                 // traversal index of the root node
     unsigned num_queries;
     const float* pattern;     // 64*M templates: x,y,u,v,t
     const TravNode* trav;

@@ -64,6 +64,11 @@ struct tlr_ctx {
     Points pts{};
     LeafRec* leaves = nullptr;
     Bounds* leaf_b = nullptr;
+    // hierarchy: slot -> leaf id, internal children (leaf slot k is n-1+k), bounds
+    uint32_t* tree_ids = nullptr;
+    int* tree_c0 = nullptr;
+    int* tree_c1 = nullptr;
+    Bounds* tree_b = nullptr;
     TravNode* trav = nullptr;
 
     // reconstruct
@@ -311,7 +316,8 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     words += 8 * (size_t)N;           // leaf records
     words += 14 * (size_t)N;          // leaf bounds
     words += 2 * (size_t)N;           // valid flag / pos
-    words += 4 * (size_t)N;           // lbvh keys ping-pong
+    words += 4 * (size_t)N;           // hierarchy keys / values ping-pong
+    words += 21 * (size_t)N;          // leaf features (5), leaf of valid id, split info (4), extents (10), slot ids
     words += 7 * (size_t)N;           // child0/1, parent(2N), range(2N), split owner
     words += 14 * (size_t)N + (size_t)N;  // node bounds, flags
     words += 32 * (size_t)N;          // traversal nodes (2N x 16 words)
@@ -426,32 +432,35 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     const int nv = L > 0 ? (int)read_u32(c, vpos + L) : 0;
     c->nv = nv;
 
-    // ---- LBVH over valid leaves ----
+    // ---- median-split hierarchy over the valid leaves ----
     const int nv1 = std::max(nv, 1);
-    uint32_t* codes = A.alloc<uint32_t>(nv1);
-    uint32_t* ids = A.alloc<uint32_t>(nv1);
-    uint32_t* codes2 = A.alloc<uint32_t>(nv1);
-    uint32_t* ids2 = A.alloc<uint32_t>(nv1);
-    int* ch0 = A.alloc<int>(nv1);
-    int* ch1 = A.alloc<int>(nv1);
+    float* feat = A.alloc<float>(5 * (size_t)nv1);
+    uint32_t* vid_leaf = A.alloc<uint32_t>(nv1);
+    uint32_t* tk0 = A.alloc<uint32_t>(nv1);
+    uint32_t* tv0 = A.alloc<uint32_t>(nv1);
+    uint32_t* tk1 = A.alloc<uint32_t>(nv1);
+    uint32_t* tv1 = A.alloc<uint32_t>(nv1);
+    float4* kinfo = A.alloc<float4>(nv1);
+    uint32_t* kmn = A.alloc<uint32_t>(5 * (size_t)nv1);
+    uint32_t* kmx = A.alloc<uint32_t>(5 * (size_t)nv1);
+    c->tree_ids = A.alloc<uint32_t>(nv1);
+    c->tree_c0 = A.alloc<int>(nv1);
+    c->tree_c1 = A.alloc<int>(nv1);
     int* parent = A.alloc<int>(2 * (size_t)nv1);
     int2* range = A.alloc<int2>(nv1);
     int* split_owner = A.alloc<int>(nv1);
-    Bounds* node_b = A.alloc<Bounds>(nv1);
+    c->tree_b = A.alloc<Bounds>(nv1);
     int* bflags = A.alloc<int>(nv1);
     c->trav = A.alloc<TravNode>(2 * (size_t)nv1);
-    if (!need(c->trav)) return c->fail(TLR_E_HIP, "arena exhausted (lbvh)");
+    if (!need(c->trav)) return c->fail(TLR_E_HIP, "arena exhausted (hierarchy)");
     if (nv > 0) {
-        launch_lbvh_keys(vflag, vpos, L, c->leaf_b, c->d_gi, codes, ids, s);
-        bool f2 = true;
-        HIPCHK(c, radix_sort_pairs(codes, ids, codes2, ids2, (size_t)nv, 32, hist, stmp, &f2, s));
-        const uint32_t* sc = f2 ? codes : codes2;
-        const uint32_t* si = f2 ? ids : ids2;
         HIPCHK(c, hipMemsetAsync(parent, 0xFF, sizeof(int) * (2 * (size_t)nv), s));
         HIPCHK(c, hipMemsetAsync(bflags, 0, sizeof(int) * nv1, s));
-        launch_lbvh_nodes(sc, nv, ch0, ch1, parent, range, split_owner, s);
-        launch_lbvh_bounds(si, nv, c->leaf_b, ch0, ch1, parent, node_b, bflags, s);
-        launch_lbvh_trav(si, nv, c->leaf_b, node_b, ch0, ch1, range, split_owner, c->trav, s);
+        HIPCHK(c, launch_kdtree(vflag, vpos, L, c->leaf_b, nv, feat, vid_leaf, tk0, tv0, tk1, tv1, kinfo, kmn, kmx, hist, stmp,
+                                c->tree_ids, c->tree_c0, c->tree_c1, parent, range, split_owner, s));
+        launch_lbvh_bounds(c->tree_ids, nv, c->leaf_b, c->tree_c0, c->tree_c1, parent, c->tree_b, bflags, s);
+        launch_lbvh_trav(c->tree_ids, nv, c->leaf_b, c->tree_b, c->tree_c0, c->tree_c1, range, split_owner, c->trav,
+                         s);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[1], s));
@@ -510,6 +519,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     Q.time_gauss = p->time_gaussian;
     Q.collect_stats = p->collect_stats;
     Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
+    Q.root = c->nv > 1 ? c->nv / 2 - 1 : 0;
     Q.num_queries = (unsigned)nq;
     Q.pattern = c->d_pattern;
     Q.trav = c->trav;
@@ -684,6 +694,35 @@ tlr_status tlr_copy_initial_leaves(tlr_ctx* c, int32_t* out6) {
         out6[6 * k + 0] = r[k].x; out6[6 * k + 1] = r[k].y;
         out6[6 * k + 2] = r[k].z; out6[6 * k + 3] = r[k].w;
         out6[6 * k + 4] = oc[k].x; out6[6 * k + 5] = oc[k].y;
+    }
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_hierarchy(tlr_ctx* c, int32_t* order, int32_t* ch2, float* b14) {
+    if (!c || !c->built) return TLR_E_STATE;
+    const int n = c->nv;
+    if (n == 0) return TLR_OK;
+    std::vector<uint32_t> ids(n);
+    HIPCHK(c, hipMemcpy(ids.data(), c->tree_ids, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    if (order)
+        for (int k = 0; k < n; k++) order[k] = (int32_t)ids[k];
+    if (n < 2) return TLR_OK;
+    std::vector<int> c0(n - 1), c1(n - 1);
+    std::vector<Bounds> nb(n - 1);
+    HIPCHK(c, hipMemcpy(c0.data(), c->tree_c0, 4 * (size_t)(n - 1), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(c1.data(), c->tree_c1, 4 * (size_t)(n - 1), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(nb.data(), c->tree_b, sizeof(Bounds) * (size_t)(n - 1), hipMemcpyDeviceToHost));
+    auto enc = [&](int x) { return x >= n - 1 ? (int32_t)(ids[x - (n - 1)] | 0x80000000u) : (int32_t)x; };
+    for (int g = 0; g < n - 1; g++) {
+        if (ch2) {
+            ch2[2 * g] = enc(c0[g]);
+            ch2[2 * g + 1] = enc(c1[g]);
+        }
+        if (b14) {
+            memcpy(b14 + 14 * (size_t)g, nb[g].p, 12 * 4);
+            b14[14 * (size_t)g + 12] = nb[g].ts0;
+            b14[14 * (size_t)g + 13] = nb[g].ts1;
+        }
     }
     return TLR_OK;
 }

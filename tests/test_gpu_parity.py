@@ -88,6 +88,19 @@ def test_leaves_bitexact(tlr, orc, case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_hierarchy_bitexact(tlr, orc, case):
+    """Median-split hierarchy: same leaf order, same children, same merged
+    bounds as the oracle's recursive build."""
+    sbuf, ctx, o = _pair(tlr, orc, case)
+    go, gc, gb = ctx.hierarchy()
+    ro, rc, rb = o.lbvh()
+    assert go.size == ro.size == ctx.build_info().num_valid_leaves
+    assert np.array_equal(go, ro), "leaf order differs at %s" % np.nonzero(go != ro)[0][:5]
+    assert np.array_equal(gc, rc), "children differ"
+    assert np.array_equal(_bits(gb), _bits(rb)), "internal bounds differ"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_image_parity(tlr, orc, case):
     sbuf, ctx, o = _pair(tlr, orc, case)
     out_spp = case[5]

@@ -179,6 +179,49 @@ def test_lbvh_finds_every_intersecting_leaf(built, name):
     assert mism == 0
 
 
+@pytest.mark.parametrize("name", ["defocus", "motion", "strong"])
+def test_hierarchy_is_a_balanced_median_tree(built, name):
+    """Structure of the median-split hierarchy: every valid leaf appears once;
+    internal node g splits slots [lo, hi) at mid = (lo+hi)//2 with g = mid-1,
+    so the depth is ceil(log2 n); each internal bound holds its children at
+    the corners of the (t, u) domain."""
+    sb, o = built[name]
+    order, ch, b = o.lbvh()
+    s01, lb = o.leaves()
+    valid = np.nonzero(lb[:, 12] < lb[:, 13])[0]
+    n = order.size
+    assert sorted(order.tolist()) == valid.tolist()
+    seen = []
+    depth = 0
+    stack = [(0, n, 0)]
+    while stack:
+        lo, hi, d = stack.pop()
+        depth = max(depth, d)
+        if hi - lo == 1:
+            seen.append(lo)
+            continue
+        mid = (lo + hi) // 2
+        g = mid - 1
+        for k, (a, z) in enumerate(((lo, mid), (mid, hi))):
+            c = int(ch[g, k])
+            if z - a == 1:
+                assert c < 0  # leaf id tagged with bit 31
+                assert (c & 0x7FFFFFFF) == order[a]
+                cb = lb[order[a]]
+            else:
+                assert c == (a + z) // 2 - 1
+                cb = b[c]
+            for t in (0.0, 1.0):
+                for u in (-1.0, 1.0):
+                    ev = lambda p, j: p[3 * j] * t + p[3 * j + 1] * u + p[3 * j + 2]
+                    tol = 1e-3 * (1 + np.abs(cb[:12]).max())
+                    assert ev(b[g], 1) >= ev(cb, 1) - tol and ev(b[g], 3) >= ev(cb, 3) - tol
+                    assert ev(b[g], 0) <= ev(cb, 0) + tol and ev(b[g], 2) <= ev(cb, 2) + tol
+            stack.append((a, z, d + 1))
+    assert sorted(seen) == list(range(n))
+    assert depth == int(np.ceil(np.log2(n)))
+
+
 def test_error_paths(orc):
     # every sample behind the camera at t = 0.5, 0 and 1 -> all discarded
     rec = np.zeros((4, 4, 2, 16), np.float32)
