@@ -1,6 +1,6 @@
 """Benchmark of the reconstruction hot path on one MI355X.
 
-One step = tlr_build (reprojection, bucketing, leaves, LBVH) + tlr_reconstruct
+One step = tlr_build (reprojection, bucketing, leaves, hierarchy) + tlr_reconstruct
 (all output samples, per-pixel combine) over a synthetic sample buffer that is
 already resident in HBM.  Prints one JSON line.
 
@@ -65,7 +65,7 @@ def cpu_baseline(cfg, sbuf, sample_rows):
     t0 = time.perf_counter()
     o.build(threads=threads)
     t_build = time.perf_counter() - t0
-    stride = max(1, H // sample_rows)
+    stride = max(1, H // sample_rows) if sample_rows > 0 else 1
     t0 = time.perf_counter()
     o.reconstruct(out_spp=cfg["out_spp"], threads=threads, row_stride=stride)
     t_filter = time.perf_counter() - t0
@@ -73,8 +73,21 @@ def cpu_baseline(cfg, sbuf, sample_rows):
     est = t_build + t_filter * (H / rows)
     o.close()
     return {"value": W * H / est / 1e6, "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "full build (%.2f s) + filter of every %d-th scanline (%d rows, %.2f s), "
-                      "frame time extrapolated to %.2f s" % (t_build, stride, rows, t_filter, est)}
+            "sample": "full build (%.2f s) + filter of every %d-th scanline (%d of %d rows, %.2f s); "
+                      "frame time %.2f s" % (t_build, stride, rows, H, t_filter, est)}
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings to bytes (the copy reads and writes `mib` MiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
 def main():
@@ -84,13 +97,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=2011)
-    ap.add_argument("--cpu-rows", type=int, default=16, help="scanlines in the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="scanlines in the CPU baseline sample (0: the whole frame)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (profiler counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     cfg = CONFIGS[args.config]
     tlr = ge.load_package()
@@ -102,15 +121,14 @@ def main():
     ctx.set_samples(sbuf)  # inputs resident in HBM before timing
     img = np.empty((H, W, 4), np.float32)
 
-    def step():
+    def step(stats=False):
         ctx.build()
-        return ctx.reconstruct(out_spp=M, collect_stats=True, out=img)[1]
+        return ctx.reconstruct(out_spp=M, collect_stats=stats, out=img)[1]
 
     for i in range(args.warmup):
         st = step()
         print("warmup %d: build %.2f ms query %.2f ms" % (i, st["build_ms"], st["query_ms"]),
               file=sys.stderr, flush=True)
-    ctx.build()
     tlr.lib().tlr_sync(ctx.h)
     sts = []
     t0 = time.perf_counter()
@@ -118,13 +136,19 @@ def main():
         sts.append(step())
     tlr.lib().tlr_sync(ctx.h)
     dt = (time.perf_counter() - t0) / max(args.steps, 1)
-    st = sts[-1]
-    q_ms = float(np.mean([s["query_ms"] for s in sts]))
-    b_ms = float(np.mean([s["build_ms"] for s in sts]))
+    print("timed: %.2f ms/step" % (dt * 1e3), file=sys.stderr, flush=True)
+    # counters come from one more (untimed) pass with statistics enabled;
+    # they do not depend on timing
+    st = step(stats=True)
+    mean = lambda k: float(np.mean([s[k] for s in sts]))
+    q_ms, b_ms = mean("query_ms"), mean("build_ms")
+    fast_ms, gen_ms = mean("fast_ms"), mean("general_ms")
     b_in, b_build, b_query, b_out = algorithmic_bytes(st, n_in, W, H)
-    achieved = b_query / (q_ms * 1e-3) / 1e9
+    b_fast = (64.0 * st["fast_leaf_nodes"] + 28.0 * st["fast_samples_in_leaf_nodes"] +
+              12.0 * st["fast_samples_within_1r"])
+    b_gen = b_query - b_fast
     wk = "cfg%d" % args.config
-    traffic = read_traffic("k_query_general", wk)
+    achieved = b_fast / (fast_ms * 1e-3) / 1e9
     res = {
         "metric": "reconstructed Mpix/s @%d out-spp, 1 GPU (plain path)" % M,
         "value": W * H / dt / 1e6,
@@ -141,13 +165,19 @@ def main():
         "config": {"workload": cfg["name"], "width": W, "height": H, "input_spp": spp, "out_spp": M,
                    "records": n_in, "seed": args.seed},
         "output_samples_per_s": W * H * M / dt,
-        "device_ms": {"build": b_ms, "query": q_ms, "combine": float(np.mean([s["combine_ms"] for s in sts]))},
+        "device_ms": {"build": b_ms, "query": q_ms, "query_fast": fast_ms, "query_general": gen_ms,
+                      "combine": mean("combine_ms")},
+        "general_queries": st["general_queries"],
         "stats_per_output_sample": {k: st[k] / st["count"] for k in (
             "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched", "surfaces",
             "num_2r", "surfaces_merged", "at_least_one")},
-        "roofline": {"bound": "hbm", "kernel": "k_query_general", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": b_query},
+        "roofline": {"bound": "hbm", "kernel": "k_query_fast", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic("k_query_fast", wk),
+                     "algorithmic_bytes_per_launch": b_fast, "avg_launch_ms": fast_ms},
+        "roofline_general": {"kernel": "k_query_general", "algorithmic_bytes_per_launch": b_gen,
+                             "avg_launch_ms": gen_ms,
+                             "achieved": b_gen / max(gen_ms * 1e-3, 1e-9) / 1e9},
         "path_roofline": {"algorithmic_bytes": b_in + b_build + b_query + b_out,
                           "roofline_ms": (b_in + b_build + b_query + b_out) / (HBM_PEAK_GBS * 1e9) * 1e3,
                           "frac": (b_in + b_build + b_query + b_out) / (HBM_PEAK_GBS * 1e9) / dt},

@@ -72,6 +72,9 @@ typedef struct tlr_stats {
     double general_queries;      /* output samples handled by the general kernel     */
     double direct_ms, gate_ms;   /* gated path: direct filter + features, MLP + routing */
     double gate_rejected;        /* output samples the gate sent to the plain method */
+    /* the 1R lane kernel's share of leaf_nodes, samples_in_leaf_nodes and
+     * samples_within_1r (its algorithmic bytes) */
+    double fast_leaf_nodes, fast_samples_in_leaf_nodes, fast_samples_within_1r;
 } tlr_stats;
 
 typedef struct tlr_build_info {

@@ -18,7 +18,7 @@ import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
-TLR_LIB = os.path.join(LIB_DIR, "libtlr.so")
+TLR_LIB = os.environ.get("TLR_LIBRARY") or os.path.join(LIB_DIR, "libtlr.so")
 SCENE_LIB = os.path.join(LIB_DIR, "libtlr_scene.so")
 
 STATUS = {0: "TLR_OK", 1: "TLR_E_INVALID", 2: "TLR_E_IO", 3: "TLR_E_ALL_DISCARDED",
@@ -66,7 +66,8 @@ class Stats(C.Structure):
         "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
         "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count", "overflow",
         "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
-        "fast_ms", "general_ms", "general_queries", "direct_ms", "gate_ms", "gate_rejected")]
+        "fast_ms", "general_ms", "general_queries", "direct_ms", "gate_ms", "gate_rejected",
+        "fast_leaf_nodes", "fast_samples_in_leaf_nodes", "fast_samples_within_1r")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -1,5 +1,5 @@
 // Tree build kernels: reprojection to the (u,v,t) centre, bucketing grid,
-// initial kd split, per-cell sort / bounds / surface scan, leaves, LBVH.
+// initial kd split, per-cell sort / bounds / surface scan, leaves, hierarchy.
 //
 // Stage <-> reference:
 //   k_reproject, k_hits_max, k_grid_setup, k_cell_keys
@@ -13,10 +13,12 @@
 //   k_points                              ReconstructionTreeBuilder.cpp:170-213
 //   k_surface_scan                        ReconstructionTreeBuilder.cpp:222-340
 //   k_leaf_emit                           ReconstructionTreeBuilder.cpp:345-378
-//   k_lbvh_*                              replaces emitNodes (:409-451) and the
-//                                          task merge (Reconstruction.cpp:110-203)
+//   k_kd_*, k_tree_trav                   median-split hierarchy; replaces emitNodes
+//                                          (:409-451) and the task merge
+//                                          (Reconstruction.cpp:110-203)
 #include "build.hpp"
 #include "prims.hpp"
+#include <algorithm>
 #include <utility>
 
 namespace tlr {
@@ -47,17 +49,27 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
-// Reprojection (one thread per input sample).
+// Reprojection: grid-stride over the input samples; the bbox and counters are
+// reduced per block so each block issues one set of global atomics.
 // ---------------------------------------------------------------------------
+constexpr int kReduceBlocks = 2048;
+
 __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, int H, float C0, float C1,
                                                       float* __restrict__ rx, float* __restrict__ ry,
                                                       float* __restrict__ rw, float* __restrict__ rt,
                                                       int* __restrict__ hits, GridInfo* gi) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    bool ok = false, bad = false;
-    float ox = 0, oy = 0;
-    if (i < n) {
+    int kmin_x = 0x7FFFFFFF, kmin_y = 0x7FFFFFFF;
+    int kmax_x = (int)0x80000000, kmax_y = (int)0x80000000;
+    unsigned nok = 0, nbad = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        bool ok = false;
         const float x = in.x[i], y = in.y[i], w = in.w[i], u = in.u[i], v = in.v[i], t = in.t[i];
         const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
         const float coc = C0 / w + C1;
@@ -77,36 +89,49 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
         }
         if (Wt > 0.f) {
             const float c = rcp(Wt);
-            ox = X * c;
-            oy = Y * c;
+            const float ox = X * c;
+            const float oy = Y * c;
             if (isfinite(ox) && isfinite(oy)) {
                 ok = true;
                 rx[i] = ox; ry[i] = oy; rw[i] = Wt; rt[i] = nt;
                 const int sx = floor_i(ox), sy = floor_i(oy);
                 if (sx >= 0 && sy >= 0 && sx < W && sy < H) atomicAdd(&hits[sy * W + sx], 1);
+                kmin_x = min(kmin_x, order_key(ox));
+                kmin_y = min(kmin_y, order_key(oy));
+                kmax_x = max(kmax_x, order_key(ox));
+                kmax_y = max(kmax_y, order_key(oy));
+                nok++;
             } else {
-                bad = true;
+                nbad++;
             }
         }
         if (!ok) rw[i] = 0.0f;  // marks a discarded sample
     }
-    // bbox and counts: wave reduction, one atomic per wave
-    int kmin_x = ok ? order_key(ox) : 0x7FFFFFFF;
-    int kmin_y = ok ? order_key(oy) : 0x7FFFFFFF;
-    int kmax_x = ok ? order_key(ox) : (int)0x80000000;
-    int kmax_y = ok ? order_key(oy) : (int)0x80000000;
     kmin_x = wave_min(kmin_x); kmin_y = wave_min(kmin_y);
     kmax_x = wave_max(kmax_x); kmax_y = wave_max(kmax_y);
-    const uint64_t okm = __ballot(ok), badm = __ballot(bad);
+    nok = wave_sum(nok);
+    nbad = wave_sum(nbad);
+    __shared__ int red[kBlock / 64][6];
+    const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (okm) {
+        red[wv][0] = kmin_x; red[wv][1] = kmin_y; red[wv][2] = kmax_x; red[wv][3] = kmax_y;
+        red[wv][4] = (int)nok; red[wv][5] = (int)nbad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; k++) {
+            kmin_x = min(kmin_x, red[k][0]); kmin_y = min(kmin_y, red[k][1]);
+            kmax_x = max(kmax_x, red[k][2]); kmax_y = max(kmax_y, red[k][3]);
+            nok += (unsigned)red[k][4]; nbad += (unsigned)red[k][5];
+        }
+        if (nok) {
             atomicMin(&gi->bbmin_key[0], kmin_x);
             atomicMin(&gi->bbmin_key[1], kmin_y);
             atomicMax(&gi->bbmax_key[0], kmax_x);
             atomicMax(&gi->bbmax_key[1], kmax_y);
-            atomicAdd(&gi->num_kept, (unsigned)__popcll(okm));
+            atomicAdd(&gi->num_kept, nok);
         }
-        if (badm) atomicAdd(&gi->num_bad, (unsigned)__popcll(badm));
+        if (nbad) atomicAdd(&gi->num_bad, nbad);
     }
 }
 
@@ -146,30 +171,37 @@ __global__ void k_grid_setup(GridInfo* gi) {
     gi->gh = iy1 - iy0;
 }
 
-// Bucket index of each kept sample; counts per cell.
+// Bucket index of each kept sample; counts per cell (grid-stride, one
+// num_bucketed atomic per block).
 __global__ __launch_bounds__(kBlock) void k_cell_keys(const float* __restrict__ rx, const float* __restrict__ ry,
                                                       const float* __restrict__ rw, int n, GridInfo* gi,
                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                       uint32_t* __restrict__ counts) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    bool in = false;
-    if (i < n) {
+    const float scale = gi->scale;
+    const int bx = gi->bbmin_i[0], by = gi->bbmin_i[1], gw = gi->gw, gh = gi->gh;
+    unsigned nin = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         uint32_t key = 0xFFFFFFFFu;
         if (rw[i] > 0.0f) {
-            const float scale = gi->scale;
-            const int sx = floor_i(scale * rx[i]) - gi->bbmin_i[0];
-            const int sy = floor_i(scale * ry[i]) - gi->bbmin_i[1];
-            if (!(sx < 0 || sy < 0 || sx >= gi->gw || sy >= gi->gh)) {
-                key = (uint32_t)(sy * gi->gw + sx);
+            const int sx = floor_i(scale * rx[i]) - bx;
+            const int sy = floor_i(scale * ry[i]) - by;
+            if (!(sx < 0 || sy < 0 || sx >= gw || sy >= gh)) {
+                key = (uint32_t)(sy * gw + sx);
                 atomicAdd(&counts[key], 1u);
-                in = true;
+                nin++;
             }
         }
         keys[i] = key;
         vals[i] = (uint32_t)i;
     }
-    const uint64_t m = __ballot(in);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&gi->num_bucketed, (unsigned)__popcll(m));
+    nin = wave_sum(nin);
+    __shared__ unsigned red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; k++) nin += red[k];
+        if (nin) atomicAdd(&gi->num_bucketed, nin);
+    }
 }
 
 // Summed-area table over the cell counts, (gh+1) x (gw+1), zero first row/col.
@@ -403,8 +435,8 @@ __global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ 
     const uint32_t i = cand[k];
     const float x = rx[i], y = ry[i], w = rw[i], t = rt[i];
     const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
-    pts.pa[k] = make_float4(x, y, w, t);
-    pts.pb[k] = make_float4(mvx, mvy, mvw, 0.0f);
+    pts.pab[2 * (size_t)k] = make_float4(x, y, w, t);
+    pts.pab[2 * (size_t)k + 1] = make_float4(mvx, mvy, mvw, 0.0f);
     pts.pc[k] = make_float4(in.r[i], in.g[i], in.b[i], 1.0f);
 
     Bounds b;
@@ -436,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ 
 // Sequential surface scan of one initial leaf (ReconstructionTreeBuilder.cpp:
 // 222-340, keeping the cocboundst[0] typo at :334,336).  Marks leaf starts.
 __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict__ offcnt, int nil,
-                                                         const float4* __restrict__ pa,
+                                                         const float4* __restrict__ pab,
                                                          const float4* __restrict__ boxes,
                                                          uint32_t* __restrict__ start_flag) {
     const int li = blockIdx.x * kBlock + threadIdx.x;
@@ -445,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict_
     const int base = oc.x, n = oc.y;
     if (n <= 0) return;
     const float4* B = boxes + 2 * (size_t)base;
-    const float4* T = pa + base;
+    const float4* T = pab + 2 * (size_t)base;  // (x, y, w, t) at T[2k]
     start_flag[base] = 1u;
     int sb = 0;
     bool reset = true;
@@ -459,10 +491,10 @@ __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict_
             mn0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
             mn1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
             mx0 = mn0; mx1 = mn1;
-            st = T[sb].w;
+            st = T[2 * sb].w;
             reset = false;
         }
-        if (st != T[k].w) {
+        if (st != T[2 * k].w) {
             sb = k;
             start_flag[base + k] = 1u;
             reset = true;
@@ -526,10 +558,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit(const uint32_t* __restrict
     LeafRec r;
     r.s0 = k;
     r.s1 = e;
-    const float4 a = pts.pa[k];
+    const float4 a = pt_a(pts, k);
     r.w0 = a.z;
     r.t0 = a.w;
-    r.mvw0 = pts.pb[k].z;
+    r.mvw0 = pt_b(pts, k).z;
     r.valid = bounds_valid(acc) ? 1 : 0;
     r.pad[0] = r.pad[1] = 0.0f;
     leaves[id] = r;
@@ -643,9 +675,52 @@ __global__ __launch_bounds__(kBlock) void k_kd_ext_init(int nodes, uint32_t* __r
     emx[i] = 0u;
 }
 
-// Large nodes: chunks of kKdChunk positions.  A chunk inside one node is
-// block-reduced (one atomic per dimension); a chunk that straddles nodes
-// falls back to per-position atomics.
+// Min/max of the five features over positions [a, b) of node `node`, reduced
+// over the block, then one atomic per dimension.  Called by the whole block.
+__device__ void kd_block_extent(const float* __restrict__ feat, int nv, const uint32_t* __restrict__ vals, int a,
+                                int b, int node, uint32_t* __restrict__ emn, uint32_t* __restrict__ emx) {
+    __shared__ float smn[5][kBlock / 64], smx[5][kBlock / 64];
+    float mn[5], mx[5];
+#pragma unroll
+    for (int d = 0; d < 5; d++) {
+        mn[d] = kFltMax;
+        mx[d] = -kFltMax;
+    }
+    for (int p = a + threadIdx.x; p < b; p += kBlock) {
+        const uint32_t v = vals[p];
+#pragma unroll
+        for (int d = 0; d < 5; d++) {
+            const float f = feat[d * (size_t)nv + v];
+            mn[d] = fminf(mn[d], f);
+            mx[d] = fmaxf(mx[d], f);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 5; d++) {
+        mn[d] = wave_min(mn[d]);
+        mx[d] = wave_max(mx[d]);
+        if ((threadIdx.x & 63) == 0) {
+            smn[d][threadIdx.x >> 6] = mn[d];
+            smx[d][threadIdx.x >> 6] = mx[d];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 5 && b > a) {
+        const int d = threadIdx.x;
+        float lo = smn[d][0], hi = smx[d][0];
+        for (int w = 1; w < kBlock / 64; w++) {
+            lo = fminf(lo, smn[d][w]);
+            hi = fmaxf(hi, smx[d][w]);
+        }
+        atomicMin(&emn[5 * node + d], f2ord(lo));
+        atomicMax(&emx[5 * node + d], f2ord(hi));
+    }
+    __syncthreads();
+}
+
+// Nodes of at least kKdChunk leaves: one block per chunk of kKdChunk
+// positions; a chunk meets at most two nodes (node sizes at one level differ
+// by at most one).
 constexpr int kKdChunk = 2048;
 __global__ __launch_bounds__(kBlock) void k_kd_ext_atomic(const float* __restrict__ feat, int nv,
                                                           const uint32_t* __restrict__ vals, int level,
@@ -653,59 +728,45 @@ __global__ __launch_bounds__(kBlock) void k_kd_ext_atomic(const float* __restric
     const int p0 = blockIdx.x * kKdChunk;
     const int p1 = min(nv, p0 + kKdChunk);
     int lo, hi;
-    const int nf = kd_node_at(nv, level, p0, lo, hi);
-    const bool whole = (p1 <= hi);  // chunk lies in node nf
-    if (whole) {
-        if (hi - lo < 2) return;
-        float mn[5], mx[5];
-#pragma unroll
-        for (int d = 0; d < 5; d++) {
-            mn[d] = kFltMax;
-            mx[d] = -kFltMax;
-        }
-        for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
-            const uint32_t v = vals[p];
-#pragma unroll
-            for (int d = 0; d < 5; d++) {
-                const float f = feat[d * (size_t)nv + v];
-                mn[d] = fminf(mn[d], f);
-                mx[d] = fmaxf(mx[d], f);
-            }
-        }
-        __shared__ float smn[5][kBlock / 64], smx[5][kBlock / 64];
-#pragma unroll
-        for (int d = 0; d < 5; d++) {
-            mn[d] = wave_min(mn[d]);
-            mx[d] = wave_max(mx[d]);
-            if ((threadIdx.x & 63) == 0) {
-                smn[d][threadIdx.x >> 6] = mn[d];
-                smx[d][threadIdx.x >> 6] = mx[d];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 5) {
-            const int d = threadIdx.x;
-            float a = smn[d][0], b = smx[d][0];
-            for (int w = 1; w < kBlock / 64; w++) {
-                a = fminf(a, smn[d][w]);
-                b = fmaxf(b, smx[d][w]);
-            }
-            atomicMin(&emn[5 * nf + d], f2ord(a));
-            atomicMax(&emx[5 * nf + d], f2ord(b));
-        }
-        return;
+    const int n0 = kd_node_at(nv, level, p0, lo, hi);
+    const int cut = min(hi, p1);
+    kd_block_extent(feat, nv, vals, p0, cut, n0, emn, emx);
+    if (cut < p1) {
+        const int n1 = kd_node_at(nv, level, cut, lo, hi);
+        kd_block_extent(feat, nv, vals, cut, p1, n1, emn, emx);
     }
-    for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
-        const int node = kd_node_at(nv, level, p, lo, hi);
-        if (hi - lo < 2) continue;
+}
+
+// Mid-size nodes: one wave per node.
+__global__ __launch_bounds__(kBlock) void k_kd_extent_wave(const float* __restrict__ feat, int nv,
+                                                           const uint32_t* __restrict__ vals, int level, int nodes,
+                                                           float4* __restrict__ info) {
+    const int node = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (node >= nodes) return;
+    int lo, hi;
+    kd_range(nv, level, node, lo, hi);
+    if (hi - lo < 2) return;
+    float mn[5], mx[5];
+#pragma unroll
+    for (int d = 0; d < 5; d++) {
+        mn[d] = kFltMax;
+        mx[d] = -kFltMax;
+    }
+    for (int p = lo + (threadIdx.x & 63); p < hi; p += 64) {
         const uint32_t v = vals[p];
 #pragma unroll
         for (int d = 0; d < 5; d++) {
-            const uint32_t o = f2ord(feat[d * (size_t)nv + v]);
-            atomicMin(&emn[5 * node + d], o);
-            atomicMax(&emx[5 * node + d], o);
+            const float f = feat[d * (size_t)nv + v];
+            mn[d] = fminf(mn[d], f);
+            mx[d] = fmaxf(mx[d], f);
         }
     }
+#pragma unroll
+    for (int d = 0; d < 5; d++) {
+        mn[d] = wave_min(mn[d]);
+        mx[d] = wave_max(mx[d]);
+    }
+    if ((threadIdx.x & 63) == 0) kd_pick(mn, mx, info[node]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_kd_ext_final(int nv, int level, int nodes,
@@ -801,34 +862,30 @@ __global__ __launch_bounds__(kBlock) void k_kd_nodes(int n, int* __restrict__ ch
     split_owner[g] = g;
 }
 
-// Bottom-up bounds: the second child to arrive merges (child0, child1).
-__global__ __launch_bounds__(kBlock) void k_lbvh_bounds(const uint32_t* __restrict__ ids, int n,
-                                                        const Bounds* leaf_b, const int* __restrict__ child0,
-                                                        const int* __restrict__ child1,
-                                                        const int* __restrict__ parent, Bounds* node_b,
-                                                        int* flags) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    int p = parent[n - 1 + k];
-    while (p >= 0) {
-        __threadfence();
-        const int old = atomicAdd(&flags[p], 1);
-        if (old == 0) return;
-        __threadfence();
-        const int c0 = child0[p], c1 = child1[p];
-        const Bounds* b0 = (c0 >= n - 1) ? &leaf_b[ids[c0 - (n - 1)]] : &node_b[c0];
-        const Bounds* b1 = (c1 >= n - 1) ? &leaf_b[ids[c1 - (n - 1)]] : &node_b[c1];
-        // The acquire fence above makes the other child's bounds visible.
-        const Bounds a = *b0, b = *b1;
-        Bounds r;
-        bounds_merge(a, b, r);
-        node_b[p] = r;
-        p = parent[p];
-    }
+// Bounds of the median-split hierarchy, one launch per level from the
+// deepest: node `node` at `level` merges its two children (child0, child1),
+// which are leaves or nodes of the level below.
+__global__ __launch_bounds__(kBlock) void k_kd_bounds_level(const uint32_t* __restrict__ ids, int n, int level,
+                                                            const Bounds* __restrict__ leaf_b,
+                                                            const int* __restrict__ child0,
+                                                            const int* __restrict__ child1,
+                                                            Bounds* __restrict__ node_b) {
+    const int node = blockIdx.x * kBlock + threadIdx.x;
+    if (node >= (1 << level)) return;
+    int lo, hi;
+    kd_range(n, level, node, lo, hi);
+    if (hi - lo < 2) return;
+    const int g = ((lo + hi) >> 1) - 1;
+    const int c0 = child0[g], c1 = child1[g];
+    const Bounds a = (c0 >= n - 1) ? leaf_b[ids[c0 - (n - 1)]] : node_b[c0];
+    const Bounds b = (c1 >= n - 1) ? leaf_b[ids[c1 - (n - 1)]] : node_b[c1];
+    Bounds r;
+    bounds_merge(a, b, r);
+    node_b[g] = r;
 }
 
 // Traversal records with escape links (next node after a subtree).
-__global__ __launch_bounds__(kBlock) void k_lbvh_trav(const uint32_t* __restrict__ ids, int n,
+__global__ __launch_bounds__(kBlock) void k_tree_trav(const uint32_t* __restrict__ ids, int n,
                                                       const Bounds* __restrict__ leaf_b,
                                                       const Bounds* __restrict__ node_b,
                                                       const int* __restrict__ child0,
@@ -870,7 +927,8 @@ __global__ __launch_bounds__(kBlock) void k_lbvh_trav(const uint32_t* __restrict
 
 void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, float* rx, float* ry, float* rw,
                       float* rt, int* hits, GridInfo* gi, hipStream_t s) {
-    if (n > 0) k_reproject<<<TLR_GRID(n), 0, s>>>(in, n, W, H, C0, C1, rx, ry, rw, rt, hits, gi);
+    const unsigned rb = (unsigned)std::min<long long>(kReduceBlocks, ((long long)n + kBlock - 1) / kBlock);
+    if (n > 0) k_reproject<<<rb, kBlock, 0, s>>>(in, n, W, H, C0, C1, rx, ry, rw, rt, hits, gi);
     const int nh = W * H;
     unsigned blocks = (unsigned)std::min(4096, (nh + kBlock - 1) / kBlock);
     if (blocks == 0) blocks = 1;
@@ -880,7 +938,8 @@ void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, 
 
 void launch_cell_keys(const float* rx, const float* ry, const float* rw, int n, GridInfo* gi, uint32_t* keys,
                       uint32_t* vals, uint32_t* counts, hipStream_t s) {
-    if (n > 0) k_cell_keys<<<TLR_GRID(n), 0, s>>>(rx, ry, rw, n, gi, keys, vals, counts);
+    const unsigned cb = (unsigned)std::min<long long>(kReduceBlocks, ((long long)n + kBlock - 1) / kBlock);
+    if (n > 0) k_cell_keys<<<cb, kBlock, 0, s>>>(rx, ry, rw, n, gi, keys, vals, counts);
 }
 
 void launch_sat(const uint32_t* counts, int gw, int gh, uint32_t* sat, hipStream_t s) {
@@ -913,9 +972,9 @@ void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* r
     if (nb > 0) k_points<<<TLR_GRID(nb), 0, s>>>(cand, nb, in, rx, ry, rw, rt, W, H, cc0, cc1, pts, bounds, boxes);
 }
 
-void launch_surface_scan(const int2* offcnt, int nil, const float4* pa, const float4* boxes, uint32_t* flags,
+void launch_surface_scan(const int2* offcnt, int nil, const float4* pab, const float4* boxes, uint32_t* flags,
                          hipStream_t s) {
-    if (nil > 0) k_surface_scan<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, pa, boxes, flags);
+    if (nil > 0) k_surface_scan<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, pab, boxes, flags);
 }
 
 void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
@@ -926,7 +985,7 @@ void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, c
 hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
                          int nv, float* feat, uint32_t* vid_leaf, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                          uint32_t* v1, float4* info, uint32_t* emn, uint32_t* emx, uint32_t* hist, uint32_t* tmp, uint32_t* ids, int* c0, int* c1,
-                         int* parent, int2* range, int* split_owner, hipStream_t s) {
+                         int* parent, int2* range, int* split_owner, Bounds* node_b, hipStream_t s) {
     if (nv <= 0) return hipSuccess;
     k_kd_features<<<TLR_GRID(L), 0, s>>>(valid_flag, valid_pos, L, leaf_b, feat, nv, vid_leaf, v0);
     uint32_t* vals = v0;
@@ -938,10 +997,13 @@ hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, 
     for (int level = 0; level < levels; level++) {
         const int nodes = 1 << level;
         // average node size decides how extents are reduced
-        if (nv / nodes > 32) {
+        if (nv / nodes >= kKdChunk) {
             k_kd_ext_init<<<TLR_GRID(5 * nodes), 0, s>>>(nodes, emn, emx);
             k_kd_ext_atomic<<<(nv + kKdChunk - 1) / kKdChunk, kBlock, 0, s>>>(feat, nv, vals, level, emn, emx);
             k_kd_ext_final<<<TLR_GRID(nodes), 0, s>>>(nv, level, nodes, emn, emx, info);
+        } else if (nv / nodes > 64) {
+            k_kd_extent_wave<<<(nodes + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(feat, nv, vals, level,
+                                                                                        nodes, info);
         } else {
             k_kd_extent_thread<<<TLR_GRID(nodes), 0, s>>>(feat, nv, vals, level, nodes, info);
         }
@@ -957,17 +1019,14 @@ hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, 
     }
     k_kd_order<<<TLR_GRID(nv), 0, s>>>(vals, vid_leaf, nv, ids);
     if (nv > 1) k_kd_nodes<<<TLR_GRID(nv - 1), 0, s>>>(nv, c0, c1, parent, range, split_owner);
+    for (int level = levels - 1; level >= 0; level--)
+        k_kd_bounds_level<<<TLR_GRID(1 << level), 0, s>>>(ids, nv, level, leaf_b, c0, c1, node_b);
     return hipGetLastError();
 }
 
-void launch_lbvh_bounds(const uint32_t* ids, int n, const Bounds* leaf_b, const int* c0, const int* c1,
-                        const int* parent, Bounds* node_b, int* flags, hipStream_t s) {
-    if (n > 1) k_lbvh_bounds<<<TLR_GRID(n), 0, s>>>(ids, n, leaf_b, c0, c1, parent, node_b, flags);
-}
-
-void launch_lbvh_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
+void launch_tree_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
                       const int* c1, const int2* range, const int* split_owner, TravNode* trav, hipStream_t s) {
-    if (n > 0) k_lbvh_trav<<<TLR_GRID(2 * n - 1), 0, s>>>(ids, n, leaf_b, node_b, c0, c1, range, split_owner, trav);
+    if (n > 0) k_tree_trav<<<TLR_GRID(2 * n - 1), 0, s>>>(ids, n, leaf_b, node_b, c0, c1, range, split_owner, trav);
 }
 
 }  // namespace tlr

@@ -24,7 +24,7 @@ void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, fl
 void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
                    const float* rw, const float* rt, float W, float H, float cc0, float cc1, const Points& pts,
                    Bounds* bounds, float4* boxes, hipStream_t s);
-void launch_surface_scan(const int2* offcnt, int nil, const float4* pa, const float4* boxes, uint32_t* flags,
+void launch_surface_scan(const int2* offcnt, int nil, const float4* pab, const float4* boxes, uint32_t* flags,
                          hipStream_t s);
 void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
                       const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s);
@@ -33,10 +33,8 @@ void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, c
 hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,
                          int nv, float* feat, uint32_t* vid_leaf, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                          uint32_t* v1, float4* info, uint32_t* emn, uint32_t* emx, uint32_t* hist, uint32_t* tmp, uint32_t* ids, int* c0, int* c1,
-                         int* parent, int2* range, int* split_owner, hipStream_t s);
-void launch_lbvh_bounds(const uint32_t* ids, int n, const Bounds* leaf_b, const int* c0, const int* c1,
-                        const int* parent, Bounds* node_b, int* flags, hipStream_t s);
-void launch_lbvh_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
+                         int* parent, int2* range, int* split_owner, Bounds* node_b, hipStream_t s);
+void launch_tree_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
                       const int* c1, const int2* range, const int* split_owner, TravNode* trav, hipStream_t s);
 
 }  // namespace tlr

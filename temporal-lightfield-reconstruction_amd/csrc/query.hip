@@ -83,11 +83,10 @@ __device__ __forceinline__ bool node_hit(const float* p, const OSample& o, float
     return (o.x >= mnx && o.y >= mny && o.x <= mxx && o.y <= mxy);
 }
 
-// Reprojection of tree sample k to the output's (u,v,t):
+// Reprojection of a tree sample (pa, pb records) to the output's (u,v,t):
 // ReconstructionOur.cpp:438-451
-__device__ __forceinline__ void reproject_to(const Points& P, int k, const OSample& o, float cc0, float cc1,
-                                             float& x, float& y) {
-    const float4 a = P.pa[k], b = P.pb[k];
+__device__ __forceinline__ void reproject_point(const float4 a, const float4 b, const OSample& o, float cc0,
+                                                float cc1, float& x, float& y) {
     const float sx = a.x, sy = a.y, sw = a.z, st = a.w;
     const float mvx = b.x, mvy = b.y, mvw = b.z;
     if (mvx != 0.0f || mvy != 0.0f || mvw != 0.0f) {
@@ -104,6 +103,45 @@ __device__ __forceinline__ void reproject_to(const Points& P, int k, const OSamp
         x = sx + coc * o.u;
         y = sy + coc * o.v;
     }
+}
+
+// Early rejection for the per-sample test: the same reprojection with the
+// hardware reciprocal (v_rcp_f32, <= 1 ulp) instead of the two correctly
+// rounded divisions.  True only when the sample lies farther from the output
+// sample than R plus a bound on the difference to the exact computation
+// (1e-5 relative to the magnitudes involved, about 80 ulp), so the exact test
+// of ReconstructionOur.cpp:438-455 would reject it as well.  NaN results are
+// never rejected here and take the exact path.
+__device__ __forceinline__ bool surely_outside(const float4 a, const float4 b, const OSample& o, float cc0, float cc1,
+                                               float R) {
+    const float sx = a.x, sy = a.y, sw = a.z, st = a.w;
+    const float mvx = b.x, mvy = b.y, mvw = b.z;
+    float px, py, r;
+    if (mvx != 0.0f || mvy != 0.0f || mvw != 0.0f) {
+        const float dt = o.t - st;
+        const float Px = sx * sw + dt * mvx;
+        const float Py = sy * sw + dt * mvy;
+        const float Pw = 1.0f * sw + dt * mvw;
+        r = __builtin_amdgcn_rcpf(Pw);
+        px = Px * r;
+        py = Py * r;
+    } else {
+        r = __builtin_amdgcn_rcpf(sw);
+        px = sx;
+        py = sy;
+    }
+    const float c0r = cc0 * r;
+    const float coc = c0r + cc1;
+    const float dx = (px + coc * o.u) - o.x;
+    const float dy = (py + coc * o.v) - o.y;
+    const float err = 1e-5f * (fabsf(px) + fabsf(py) + fabsf(c0r) + fabsf(cc1) + fabsf(o.x) + fabsf(o.y)) + 1e-6f;
+    const float lim = R + err;
+    return dx * dx + dy * dy > lim * lim;
+}
+
+__device__ __forceinline__ void reproject_to(const Points& P, int k, const OSample& o, float cc0, float cc1,
+                                             float& x, float& y) {
+    reproject_point(pt_a(P, k), pt_b(P, k), o, cc0, cc1, x, y);
 }
 
 struct Gather {
@@ -549,24 +587,32 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
 // ---------------------------------------------------------------------------
 constexpr int kFastBlock = 128;
 
+// Key of leaf id at output time t (ReconstructionOur.cpp:384-385), from the
+// first 16 bytes of its record.
+__device__ __forceinline__ float leaf_key(const LeafRec* leaves, int id, float t) {
+    const float4 r = *reinterpret_cast<const float4*>(leaves + id);
+    return r.x + (t - r.y) * r.z;
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
-    __shared__ float lkey[kFastLeafCap][kFastBlock];
+    // per lane: leaf ids sorted by (key, id); keys are recomputed on compare
     __shared__ int lid[kFastLeafCap][kFastBlock];
     const int tid = threadIdx.x;
+    const int M = Q.M;
+    // Queries in scan order (pixel-major): the whole device sweeps one band
+    // of rows, whose samples stay resident in the last-level cache.
     const unsigned slot = blockIdx.x * kFastBlock + tid;
-    const unsigned nq = Q.fast_list ? *Q.qcount : Q.num_queries;
-    const bool active = slot < nq;
+    const bool active = slot < (Q.fast_list ? *Q.qcount : Q.num_queries);
     const unsigned q = active ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
+    const unsigned pix = q / (unsigned)M;
+    const int i = (int)(q - pix * (unsigned)M);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
     bool fail = false;
     bool single = false;
     bool needs2r = false;  // failed because the reference fetches 2R
     unsigned long long s_leaves = 0, s_in_leaves = 0, s_kept = 0, s_surf = 0;
     if (active) {
-        const int M = Q.M;
-        const unsigned pix = q / (unsigned)M;
-        const int i = (int)(q - pix * (unsigned)M);
-        const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
         const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
         const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
         OSample o;
@@ -595,18 +641,15 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
                     break;
                 }
                 const int id = -child - 1;
-                const LeafRec lr = Q.leaves[id];
-                const float key = lr.w0 + (o.t - lr.t0) * lr.mvw0;  // ReconstructionOur.cpp:384-385
+                const float key = leaf_key(Q.leaves, id, o.t);
                 int j = n;
                 while (j > 0) {
-                    const float kk = lkey[j - 1][tid];
                     const int ii = lid[j - 1][tid];
+                    const float kk = leaf_key(Q.leaves, ii, o.t);
                     if (!(kk > key || (!(key > kk) && ii > id))) break;
-                    lkey[j][tid] = kk;
                     lid[j][tid] = ii;
                     j--;
                 }
-                lkey[j][tid] = key;
                 lid[j][tid] = id;
                 n++;
             }
@@ -618,6 +661,8 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             const float u0 = o.u - RANGE, v0 = o.v - RANGE, t0 = o.t - 0.5f * RANGE;
             const float u1 = o.u + RANGE, v1 = o.v + RANGE, t1 = o.t + 0.5f * RANGE;
             const float R2 = R * R;
+            const float4* __restrict__ pab = Q.pts.pab;
+            const float4* __restrict__ pc = Q.pts.pc;
             int nsurf = 0, sstart = 0, quad0 = 0;
             float ar = 0, ag = 0, ab = 0, aw = 0;
             float min0 = kFltMax;
@@ -625,45 +670,71 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             bool stop = false;
             for (int li = 0; li < n && !stop; li++) {
                 const int id = lid[li][tid];
-                const LeafRec lr = Q.leaves[id];
-                s_in_leaves += (unsigned long long)(lr.s1 - lr.s0);
+                const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
+                s_in_leaves += (unsigned long long)(k1 - k0);
                 bool first = true;
-                for (int k = lr.s0; k < lr.s1; k++) {
-                    float x, y;
-                    reproject_to(Q.pts, k, o, Q.cc0, Q.cc1, x, y);
-                    const float dx = x - o.x, dy = y - o.y;
-                    const float d2 = dx * dx + dy * dy;
-                    if (!(d2 <= R2)) continue;
-                    if (first) {
-                        first = false;
-                        bool same = true;
-                        const float* np = Q.leaf_b[id].p;
-                        for (int j = sstart; j < li && same; ++j)
-                            same = same_surface(np, Q.leaf_b[lid[j][tid]].p, u0, v0, t0, u1, v1, t1);
-                        if (!same || nsurf == 0) {
-                            if (nsurf == 1 && !STATS) {
-                                stop = true;  // surface 0 is complete
-                                break;
-                            }
-                            sstart = li;
-                            nsurf++;
-                        }
+                // Samples in groups of four (128 contiguous bytes per lane and
+                // load round trip); slots past k1 belong to the next leaf or
+                // the zeroed padding and are not tested.
+                for (int kg = k0; kg < k1 && !stop; kg += 4) {
+                    const float4* g = pab + 2 * (size_t)kg;
+                    float4 ga[4], gb[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        ga[e] = g[2 * e];
+                        gb[e] = g[2 * e + 1];
                     }
-                    s_kept++;
-                    if (nsurf == 1) {
-                        quad0 |= 1 << ((dx >= 0 ? XPOS : 0) | (dy >= 0 ? YPOS : 0));
-                        const float dist = sqrtf(d2);
-                        if (dist < min0) {
-                            min0 = dist;
-                            near = Q.pts.pc[k];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int k = kg + e;
+                        if (k >= k1 || stop) break;
+                        if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
+                        float x, y;
+                        reproject_point(ga[e], gb[e], o, Q.cc0, Q.cc1, x, y);
+                        const float dx = x - o.x, dy = y - o.y;
+                        const float d2 = dx * dx + dy * dy;
+                        if (!(d2 <= R2)) continue;
+                        if (first) {
+                            first = false;
+                            bool same = true;
+                            if (sstart < li) {
+                                SurfSig a, b;
+                                surf_sig(Q.leaf_b[id].p, u0, v0, t0, u1, v1, t1, a);
+                                // planes of the next earlier leaf load while
+                                // the current pair is compared
+                                Bounds nb = Q.leaf_b[lid[sstart][tid]];
+                                for (int j = sstart; j < li && same; ++j) {
+                                    const Bounds cb = nb;
+                                    if (j + 1 < li) nb = Q.leaf_b[lid[j + 1][tid]];
+                                    surf_sig(cb.p, u0, v0, t0, u1, v1, t1, b);
+                                    same = same_surface_sig(a, b);
+                                }
+                            }
+                            if (!same || nsurf == 0) {
+                                if (nsurf == 1 && !STATS) {
+                                    stop = true;  // surface 0 is complete
+                                    break;
+                                }
+                                sstart = li;
+                                nsurf++;
+                            }
                         }
-                        const float wgt = 1 - dist / disp;
-                        if (wgt > 0) {
-                            const float4 col = Q.pts.pc[k];
-                            ar += wgt * col.x;
-                            ag += wgt * col.y;
-                            ab += wgt * col.z;
-                            aw += wgt * 1.0f;
+                        s_kept++;
+                        if (nsurf == 1) {
+                            quad0 |= 1 << ((dx >= 0 ? XPOS : 0) | (dy >= 0 ? YPOS : 0));
+                            const float dist = sqrtf(d2);
+                            const float4 col = pc[k];
+                            if (dist < min0) {
+                                min0 = dist;
+                                near = col;
+                            }
+                            const float wgt = 1 - dist / disp;
+                            if (wgt > 0) {
+                                ar += wgt * col.x;
+                                ag += wgt * col.y;
+                                ab += wgt * col.z;
+                                aw += wgt * 1.0f;
+                            }
                         }
                     }
                 }
@@ -725,6 +796,9 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             atomicAdd(&Q.stats[ST_FETCHED], v[2]);
             atomicAdd(&Q.stats[ST_SURFACES], v[3]);
             atomicAdd(&Q.stats[ST_COUNT], v[4]);
+            atomicAdd(&Q.stats[ST_FAST_LEAVES], v[0]);
+            atomicAdd(&Q.stats[ST_FAST_IN_LEAVES], v[1]);
+            atomicAdd(&Q.stats[ST_FAST_KEPT], v[2]);
         }
     }
 }
@@ -797,7 +871,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
             kmin = fminf(kmin, key);
             kmax = fmaxf(kmax, key);
             for (int k = lr.s0; k < lr.s1; k++) {
-                const float4 sa = Q.pts.pa[k], sb = Q.pts.pb[k];
+                const float4 sa = pt_a(Q.pts, k), sb = pt_b(Q.pts, k);
                 float x, y, w;
                 if (sb.x != 0.0f || sb.y != 0.0f || sb.z != 0.0f) {
                     const float dt = o.t - sa.w;

@@ -13,10 +13,12 @@ constexpr int kLeafCap = 4096;
 constexpr int kSampleCap = 32768;   // gathered samples + merged-surface copies
 constexpr int kStackCap = 8192;
 
-constexpr int kNumStats = 10;
+constexpr int kNumStats = 13;
 enum StatIdx {
     ST_LEAF_NODES = 0, ST_SAMPLES_IN_LEAVES, ST_WITHIN_1R, ST_FETCHED, ST_SURFACES, ST_2R, ST_MERGED,
-    ST_AT_LEAST_ONE, ST_COUNT, ST_OVERFLOW
+    ST_AT_LEAST_ONE, ST_COUNT, ST_OVERFLOW,
+    // the fast kernel's share of the first three (its 1R passes)
+    ST_FAST_LEAVES, ST_FAST_IN_LEAVES, ST_FAST_KEPT
 };
 
 // Per-wave scratch words (int/float) for the general query kernel.
@@ -59,7 +61,7 @@ struct QueryParams {
 
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
 // more 1R leaves go to the general kernel.
-constexpr int kFastLeafCap = 32;
+constexpr int kFastLeafCap = 24;
 constexpr int kGateFeatures = 24;
 
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s);

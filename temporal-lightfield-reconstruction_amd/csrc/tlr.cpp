@@ -309,17 +309,17 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     words += 4 * (size_t)N;           // rx ry rw rt
     words += 4 * (size_t)N;           // radix ping-pong
     words += 3 * (size_t)N;           // cand ct cw
-    words += 12 * (size_t)N;          // points (3 x float4)
-    words += 14 * (size_t)N;          // per-sample bounds
+    words += 12 * (size_t)N + 8 * kPointPad;  // points: pab (2 x float4, padded), pc
+    words += 16 * (size_t)N;          // per-sample bounds
     words += 8 * (size_t)N;           // boxes
     words += 2 * (size_t)N;           // start flags, leaf pos
     words += 8 * (size_t)N;           // leaf records
-    words += 14 * (size_t)N;          // leaf bounds
+    words += 16 * (size_t)N;          // leaf bounds
     words += 2 * (size_t)N;           // valid flag / pos
     words += 4 * (size_t)N;           // hierarchy keys / values ping-pong
     words += 21 * (size_t)N;          // leaf features (5), leaf of valid id, split info (4), extents (10), slot ids
     words += 7 * (size_t)N;           // child0/1, parent(2N), range(2N), split owner
-    words += 14 * (size_t)N + (size_t)N;  // node bounds, flags
+    words += 16 * (size_t)N + (size_t)N;  // node bounds, flags
     words += 32 * (size_t)N;          // traversal nodes (2N x 16 words)
     words += (size_t)W * H;           // hits
     words += 7 * cells_max + 2 * (size_t)kMaxGridSide + 16;  // counts, cell start, sat, owner(4), flags, y1
@@ -404,8 +404,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     // ---- per-sample data in tree order ----
     const int nb1 = std::max(nb, 1);
     Points P;
-    P.pa = A.alloc<float4>(nb1);
-    P.pb = A.alloc<float4>(nb1);
+    P.pab = A.alloc<float4>(2 * ((size_t)nb1 + kPointPad));
     P.pc = A.alloc<float4>(nb1);
     c->pts = P;
     Bounds* sb = A.alloc<Bounds>(nb1);
@@ -415,7 +414,8 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     if (!need(lpos)) return c->fail(TLR_E_HIP, "arena exhausted (points)");
     launch_points(c->cand, nb, in, c->rx, c->ry, c->rw, c->rt, (float)W, (float)H, c->cc0, c->cc1, P, sb, boxes, s);
     HIPCHK(c, hipMemsetAsync(sflag, 0, sizeof(uint32_t) * (nb1 + 1), s));
-    launch_surface_scan(c->ileaf_oc, nil, P.pa, boxes, sflag, s);
+    HIPCHK(c, hipMemsetAsync(P.pab + 2 * (size_t)nb, 0, sizeof(float4) * 2 * kPointPad, s));
+    launch_surface_scan(c->ileaf_oc, nil, P.pab, boxes, sflag, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, exclusive_scan_u32(sflag, lpos, lpos + nb, (size_t)nb, stmp, s));
     const int L = nb > 0 ? (int)read_u32(c, lpos + nb) : 0;
@@ -450,16 +450,13 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     int2* range = A.alloc<int2>(nv1);
     int* split_owner = A.alloc<int>(nv1);
     c->tree_b = A.alloc<Bounds>(nv1);
-    int* bflags = A.alloc<int>(nv1);
     c->trav = A.alloc<TravNode>(2 * (size_t)nv1);
     if (!need(c->trav)) return c->fail(TLR_E_HIP, "arena exhausted (hierarchy)");
     if (nv > 0) {
         HIPCHK(c, hipMemsetAsync(parent, 0xFF, sizeof(int) * (2 * (size_t)nv), s));
-        HIPCHK(c, hipMemsetAsync(bflags, 0, sizeof(int) * nv1, s));
         HIPCHK(c, launch_kdtree(vflag, vpos, L, c->leaf_b, nv, feat, vid_leaf, tk0, tv0, tk1, tv1, kinfo, kmn, kmx, hist, stmp,
-                                c->tree_ids, c->tree_c0, c->tree_c1, parent, range, split_owner, s));
-        launch_lbvh_bounds(c->tree_ids, nv, c->leaf_b, c->tree_c0, c->tree_c1, parent, c->tree_b, bflags, s);
-        launch_lbvh_trav(c->tree_ids, nv, c->leaf_b, c->tree_b, c->tree_c0, c->tree_c1, range, split_owner, c->trav,
+                                c->tree_ids, c->tree_c0, c->tree_c1, parent, range, split_owner, c->tree_b, s));
+        launch_tree_trav(c->tree_ids, nv, c->leaf_b, c->tree_b, c->tree_c0, c->tree_c1, range, split_owner, c->trav,
                          s);
         HIPCHK(c, hipGetLastError());
     }
@@ -481,8 +478,8 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     const int W = c->h.width, H = c->h.height;
     const size_t nq = (size_t)W * H * M;
     if (nq >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "too many output samples");
-    // waves resident across the device: CUs x 8
-    const int waves = std::max(4, c->prop.multiProcessorCount * 8);
+    // persistent general kernel: CUs x 16 waves (4 per SIMD, its VGPR limit)
+    const int waves = std::max(4, c->prop.multiProcessorCount * 16);
     size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
                     (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
     if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 3) + 4 * 256;
@@ -613,6 +610,9 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         stats->at_least_one = (double)hs[ST_AT_LEAST_ONE];
         stats->count = (double)hs[ST_COUNT];
         stats->overflow = (double)hs[ST_OVERFLOW];
+        stats->fast_leaf_nodes = (double)hs[ST_FAST_LEAVES];
+        stats->fast_samples_in_leaf_nodes = (double)hs[ST_FAST_IN_LEAVES];
+        stats->fast_samples_within_1r = (double)hs[ST_FAST_KEPT];
         float ms = 0;
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;

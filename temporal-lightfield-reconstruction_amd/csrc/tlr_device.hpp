@@ -30,20 +30,28 @@ struct GridInfo {
 };
 
 // Tree sample layout (Node::s0..s1 ranges index it): the reprojected samples
-// in leaf order as 16-byte records, so that one lane walking a leaf reads
-// consecutive bytes and a wave reading one leaf is fully coalesced.
-//   pa[k] = (x, y, w, t)        position at the (u,v,t) centre
-//   pb[k] = (mv_x, mv_y, mv_w, 0) homogeneous motion
-//   pc[k] = (r, g, b, 1)        colour
+// in leaf order.  What the per-sample test reads is one 32-byte record, so a
+// lane walking a leaf reads consecutive bytes (four samples per 128-byte
+// line); the colour, read only for samples within R, is separate.
+//   pab[2k]   = (x, y, w, t)          position at the (u,v,t) centre
+//   pab[2k+1] = (mv_x, mv_y, mv_w, 0) homogeneous motion
+//   pc[k]     = (r, g, b, 1)          colour
+// pab holds kPointPad extra zeroed records past the last sample so that
+// group loads of up to kPointPad+1 samples stay inside the allocation.
+constexpr int kPointPad = 4;
 struct Points {
-    float4 *pa, *pb, *pc;
+    float4 *pab, *pc;
 };
+__device__ __forceinline__ float4 pt_a(const Points& P, int k) { return P.pab[2 * (size_t)k]; }
+__device__ __forceinline__ float4 pt_b(const Points& P, int k) { return P.pab[2 * (size_t)k + 1]; }
 
-// Leaf record, indexed by leaf id (ids follow the sample layout).
-struct LeafRec {
-    int s0, s1;
-    float w0, t0;            // w, t, mv.w of the leaf's first sample (the
-    float mvw0;              // query-time key of ReconstructionOur.cpp:384-385)
+// Leaf record, indexed by leaf id (ids follow the sample layout).  The first
+// 16 bytes hold what ordering a leaf for a query needs (w, t, mv.w of the
+// leaf's first sample: the key of ReconstructionOur.cpp:384-385) and s0.
+struct __align__(16) LeafRec {
+    float w0, t0, mvw0;
+    int s0;
+    int s1;
     int valid;
     float pad[2];
 };

@@ -35,9 +35,11 @@ TLR_HD int floor_i(float f) { return f2i(floorf(f)); }
 // p[3k+0..2] = (A,B,C) of plane k (0 xmin, 1 xmax, 2 ymin, 3 ymax):
 // x = A*t + B*u + C.
 // ---------------------------------------------------------------------------
-struct Bounds {
+// 64-byte records so the 12 plane coefficients load as three 16-byte words.
+struct __align__(16) Bounds {
     float p[12];
     float ts0, ts1;
+    float pad[2];
 };
 
 TLR_HD bool bounds_valid(const Bounds& b) { return b.ts0 < b.ts1; }
@@ -231,6 +233,41 @@ TLR_HD bool same_surface(const float* n0, const float* n1, float u0, float v0, f
                     tern_cmp(plane_eval(n0 + 6, t1, 0.0f), plane_eval(n1 + 6, t1, 0.0f), T)) &&
            tern_eq2(tern_cmp(plane_eval(n0 + 9, t0, 0.0f), plane_eval(n1 + 9, t0, 0.0f), T),
                     tern_cmp(plane_eval(n0 + 9, t1, 0.0f), plane_eval(n1 + 9, t1, 0.0f), T));
+}
+
+// sameSurface split into a per-leaf part and a pair test.  surf_sig
+// evaluates the 16 plane values same_surface compares (each plane at the two
+// u (or v) ends at the mid time, then at u = 0 and the two ends of the time
+// range); same_surface_sig(a, b) == same_surface(na, nb) for the signatures
+// of the same planes and box.
+struct SurfSig {
+    float v[16];
+};
+TLR_HD void surf_sig(const float* n, float u0, float v0, float t0, float u1, float v1, float t1, SurfSig& s) {
+    const float tm = 0.5f * (t0 + t1);
+    s.v[0] = plane_eval(n + 0, tm, u0);
+    s.v[1] = plane_eval(n + 0, tm, u1);
+    s.v[2] = plane_eval(n + 6, tm, v0);
+    s.v[3] = plane_eval(n + 6, tm, v1);
+    s.v[4] = plane_eval(n + 3, tm, u0);
+    s.v[5] = plane_eval(n + 3, tm, u1);
+    s.v[6] = plane_eval(n + 9, tm, v0);
+    s.v[7] = plane_eval(n + 9, tm, v1);
+    s.v[8] = plane_eval(n + 0, t0, 0.0f);
+    s.v[9] = plane_eval(n + 0, t1, 0.0f);
+    s.v[10] = plane_eval(n + 3, t0, 0.0f);
+    s.v[11] = plane_eval(n + 3, t1, 0.0f);
+    s.v[12] = plane_eval(n + 6, t0, 0.0f);
+    s.v[13] = plane_eval(n + 6, t1, 0.0f);
+    s.v[14] = plane_eval(n + 9, t0, 0.0f);
+    s.v[15] = plane_eval(n + 9, t1, 0.0f);
+}
+TLR_HD bool same_surface_sig(const SurfSig& a, const SurfSig& b) {
+    const float T = 1 * 0.1f;
+    bool r = true;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) r = r && tern_eq2(tern_cmp(a.v[k], b.v[k], T), tern_cmp(a.v[k + 1], b.v[k + 1], T));
+    return r;
 }
 
 // ---------------------------------------------------------------------------
