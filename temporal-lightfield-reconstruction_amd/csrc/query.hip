@@ -341,14 +341,67 @@ __device__ __forceinline__ void pair_of(int p, int& a, int& b) {
     a = p - bb * (bb - 1) / 2;
 }
 
-// Does any triangle of the surface's samples cover o?  Loops m like the
-// reference (ReconstructionOur.cpp:221-259); lanes take (k,l) pairs.
+// Does any triangle of the surface's samples cover o?  The reference loops
+// over index triples m < k < l (ReconstructionOur.cpp:221-259); the answer is
+// a plain "any", so the visiting order does not matter, and every triple that
+// is tested is tested exactly as there (same vertex order, same FP32 edge
+// lengths and distances, FP64 edge functions).  The surface's samples
+// (n < 200, :219) are staged in LDS with their quadrant bits and whether they
+// lie within the dispersion.  Triples that cannot pass are skipped by exact
+// tests only:
+//   - no triangle can have all four quadrants unless the surface holds two
+//     samples in opposite quadrants (XPOS|YPOS with XNEG|YNEG, or XPOS|YNEG
+//     with XNEG|YPOS);
+//   - for each m, k and l are taken from the samples j > m whose edge to m
+//     is at most 2 * dispersion (the e01 / e02 tests of :236-238).
+constexpr int kTriCap = 200;
 __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSample& o, float disp) {
+    __shared__ float tx[kWavesPerBlock][kTriCap], ty[kWavesPerBlock][kTriCap];
+    __shared__ int tr[kWavesPerBlock][kTriCap];
+    __shared__ short tl[kWavesPerBlock][kTriCap];
     const int lane = threadIdx.x & 63;
-    for (int mm = 0; mm < n - 2; mm++) {
-        const float r0x = m.sx[start + mm], r0y = m.sy[start + mm];
-        const int p0 = m.srpos[start + mm];
-        const int s = n - mm - 1;  // candidates for k,l
+    const int w = threadIdx.x >> 6;
+    constexpr int kNear = 16;
+    constexpr int kAll = XPOS | XNEG | YPOS | YNEG;
+    int cls = 0;  // bit q: some sample has quadrant q (q = rpos & (XPOS|YPOS))
+    for (int j = lane; j < n; j += 64) {
+        const float x = m.sx[start + j], y = m.sy[start + j];
+        const float d = sqrtf((x - o.x) * (x - o.x) + (y - o.y) * (y - o.y));
+        const int rp = m.srpos[start + j];
+        tx[w][j] = x;
+        ty[w][j] = y;
+        tr[w][j] = rp | (!(d > disp) ? kNear : 0);
+        cls |= 1 << (rp & (XPOS | YPOS));
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) cls |= __shfl_xor(cls, d, 64);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // quadrant q = (x >= 0 ? XPOS : 0) | (y >= 0 ? YPOS : 0); opposite pairs
+    // are (XPOS|YPOS, 0) = (3, 0) and (XPOS, YPOS) = (1, 2)
+    const bool opposite = ((cls & (1 << 3)) && (cls & (1 << 0))) || ((cls & (1 << 1)) && (cls & (1 << 2)));
+    bool any = false;
+    for (int mm = 0; opposite && mm < n - 2 && !any; mm++) {
+        const float r0x = tx[w][mm], r0y = ty[w][mm];
+        const int p0 = tr[w][mm];
+        // candidates j > mm with |r0 - rj| <= 2 disp, in index order
+        int s = 0;
+        for (int jb = mm + 1; jb < n; jb += 64) {
+            const int j = jb + lane;
+            bool c = false;
+            if (j < n) {
+                const float rx = tx[w][j], ry = ty[w][j];
+                const float e = sqrtf((r0x - rx) * (r0x - rx) + (r0y - ry) * (r0y - ry));
+                c = !(e > 2 * disp);
+            }
+            const uint64_t cm = __ballot(c);
+            if (c) tl[w][s + (int)lanes_below(cm)] = (short)j;
+            s += __popcll(cm);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int np = s * (s - 1) / 2;
         for (int pb = 0; pb < np; pb += 64) {
             const int p = pb + lane;
@@ -356,27 +409,26 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
             if (p < np) {
                 int a, b;
                 pair_of(p, a, b);
-                const int k = mm + 1 + a, l = mm + 1 + b;
-                const float r1x = m.sx[start + k], r1y = m.sy[start + k];
-                const float r2x = m.sx[start + l], r2y = m.sy[start + l];
-                const int rp = p0 | m.srpos[start + k] | m.srpos[start + l];
-                if (rp == (XPOS | XNEG | YPOS | YNEG)) {
-                    const float e01 = sqrtf((r0x - r1x) * (r0x - r1x) + (r0y - r1y) * (r0y - r1y));
-                    const float e02 = sqrtf((r0x - r2x) * (r0x - r2x) + (r0y - r2y) * (r0y - r2y));
+                const int k = tl[w][a], l = tl[w][b];
+                const int p1 = tr[w][k], p2 = tr[w][l];
+                if (((p0 | p1 | p2) & kAll) == kAll && ((p0 | p1 | p2) & kNear)) {
+                    const float r1x = tx[w][k], r1y = ty[w][k];
+                    const float r2x = tx[w][l], r2y = ty[w][l];
                     const float e12 = sqrtf((r1x - r2x) * (r1x - r2x) + (r1y - r2y) * (r1y - r2y));
-                    if (!(e01 > 2 * disp || e02 > 2 * disp || e12 > 2 * disp)) {
-                        const float d0 = sqrtf((r0x - o.x) * (r0x - o.x) + (r0y - o.y) * (r0y - o.y));
-                        const float d1 = sqrtf((r1x - o.x) * (r1x - o.x) + (r1y - o.y) * (r1y - o.y));
-                        const float d2 = sqrtf((r2x - o.x) * (r2x - o.x) + (r2y - o.y) * (r2y - o.y));
-                        if (!(d0 > disp && d1 > disp && d2 > disp))
-                            cov = tri_covers(r0x, r0y, r1x, r1y, r2x, r2y, o.x, o.y);
-                    }
+                    if (!(e12 > 2 * disp)) cov = tri_covers(r0x, r0y, r1x, r1y, r2x, r2y, o.x, o.y);
                 }
             }
-            if (__ballot(cov)) return true;
+            if (__ballot(cov)) {
+                any = true;
+                break;
+            }
         }
+        // tl is rewritten for the next m
+        __builtin_amdgcn_wave_barrier();
     }
-    return false;
+    // the LDS area is rewritten by the next call
+    __builtin_amdgcn_wave_barrier();
+    return any;
 }
 
 __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
@@ -419,29 +471,26 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
         int overflow = 0;
         int n1 = 0;
         bool f2 = true;
-        if (!skip1r) {
-            g = gather(Q, m, o, disp, st);
+        // One gather call site (the gather is large; every inlined copy
+        // costs registers): pass 0 = 1R, 1 = 2R, 2..4 = emergency passes
+        // (dispersion doubled before each, gather at 2x dispersion).
+        int pass = skip1r ? 1 : 0;
+        for (;;) {
+            g = gather(Q, m, o, pass == 0 ? disp : disp * 2, st);
             overflow = g.overflow;
-            n1 = g.total;
-            f2 = false;
-            if (!overflow) {
+            if (overflow) break;
+            if (pass == 0) {
+                n1 = g.total;
                 const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
                 f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
+                if (!f2) break;
+                pass = 1;
+                continue;
             }
-        }
-        if (!overflow && f2) {
-            g = gather(Q, m, o, disp * 2, st);
-            overflow = g.overflow;
-            if (!overflow && g.total == 0) {
-                st[ST_AT_LEAST_ONE] += 1;
-                int nN = 0;
-                for (int k = 0; k < 3 && !nN && !overflow; k++) {
-                    disp *= 2.f;
-                    g = gather(Q, m, o, disp * 2, st);
-                    overflow = g.overflow;
-                    nN = g.total;
-                }
-            }
+            if (g.total != 0 || pass == 4) break;
+            if (pass == 1) st[ST_AT_LEAST_ONE] += 1;
+            disp *= 2.f;
+            pass++;
         }
         float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!overflow) {
@@ -515,16 +564,27 @@ __global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
                 if (!found) found = triangle_search(m, fs, n, o, disp);
                 if (found) {
                     cr = cg = cb = ca = 0;
-                    for (int k = 0; k < n; k++) {
-                        const float rx = m.sx[fs + k], ry = m.sy[fs + k];
-                        const float dx = o.x - rx, dy = o.y - ry;
-                        const float wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
-                        if (wgt <= 0) continue;
-                        const float4 c = Q.pts.pc[m.sidx[fs + k]];
-                        cr += wgt * c.x;
-                        cg += wgt * c.y;
-                        cb += wgt * c.z;
-                        ca += wgt * 1.0f;
+                    // weights and colours of 64 samples load in parallel; the
+                    // sum then runs in sample order over lane-held values
+                    for (int c0 = 0; c0 < n; c0 += 64) {
+                        const int k = c0 + lane;
+                        float wgt = 0.0f;
+                        float4 c = make_float4(0, 0, 0, 0);
+                        if (k < n) {
+                            const float rx = m.sx[fs + k], ry = m.sy[fs + k];
+                            const float dx = o.x - rx, dy = o.y - ry;
+                            wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
+                            if (wgt > 0) c = Q.pts.pc[m.sidx[fs + k]];
+                        }
+                        const int cnt = min(64, n - c0);
+                        for (int j = 0; j < cnt; j++) {
+                            const float wj = __shfl(wgt, j, 64);
+                            if (wj <= 0) continue;
+                            cr += wj * __shfl(c.x, j, 64);
+                            cg += wj * __shfl(c.y, j, 64);
+                            cb += wj * __shfl(c.z, j, 64);
+                            ca += wj * 1.0f;
+                        }
                     }
                     const float oow = rcp(ca);
                     cr *= oow; cg *= oow; cb *= oow; ca *= oow;
@@ -1003,6 +1063,14 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ q
 }
 
 }  // namespace
+
+int general_waves_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general, kBlock, 0) != hipSuccess ||
+        blocks < 1)
+        blocks = 1;
+    return blocks * kWavesPerBlock;
+}
 
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;

@@ -64,6 +64,8 @@ struct QueryParams {
 constexpr int kFastLeafCap = 24;
 constexpr int kGateFeatures = 24;
 
+// Resident waves of the general kernel per CU (its occupancy limit).
+int general_waves_per_cu();
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s);
 // One lane per output sample, 1R only; completes every query whose result
 // is decided by the first surface without a 2R fetch

@@ -83,6 +83,7 @@ struct tlr_ctx {
     unsigned* fail_list = nullptr;
     float* scratch = nullptr;
     int num_waves = 0;
+    int general_wpc = 0;  // resident general-kernel waves per CU
     int last_M = 0;
     float* features = nullptr;      // gate: per query features
     float* logits = nullptr;
@@ -478,8 +479,9 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     const int W = c->h.width, H = c->h.height;
     const size_t nq = (size_t)W * H * M;
     if (nq >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "too many output samples");
-    // persistent general kernel: CUs x 16 waves (4 per SIMD, its VGPR limit)
-    const int waves = std::max(4, c->prop.multiProcessorCount * 16);
+    // persistent general kernel: as many waves as can be resident
+    if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
+    const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
     size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
                     (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
     if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 3) + 4 * 256;
