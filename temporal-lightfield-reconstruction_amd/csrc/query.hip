@@ -257,7 +257,8 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                 first = false;
                 bool same = true;
                 const float* np = Q.leaf_b[lid].p;
-                for (int j0 = sstart; j0 < li && same; j0 += 64) {
+                // with no surface yet a new one opens whatever the test says
+                for (int j0 = sstart; nsurf > 0 && j0 < li && same; j0 += 64) {
                     const int j = j0 + lane;
                     bool ok = true;
                     if (j < li) ok = same_surface(np, Q.leaf_b[m.sid[j]].p, u0, v0, t0, u1, v1, t1);
@@ -431,7 +432,9 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
     return any;
 }
 
-__global__ __launch_bounds__(kBlock) void k_query_general(QueryParams Q) {
+// 7 waves per SIMD (the register limit this sets spills a little; the
+// kernel waits on memory and gains more from resident waves)
+__global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const WaveMem m = wave_mem(Q.scratch + (size_t)wave * kWaveScratchWords);
@@ -757,7 +760,9 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
                         if (first) {
                             first = false;
                             bool same = true;
-                            if (sstart < li) {
+                            // with no surface yet a new one opens whatever the
+                            // test says (ReconstructionOur.cpp:478)
+                            if (nsurf > 0 && sstart < li) {
                                 SurfSig a, b;
                                 surf_sig(Q.leaf_b[id].p, u0, v0, t0, u1, v1, t1, a);
                                 // planes of the next earlier leaf load while
