@@ -726,20 +726,84 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             const float R2 = R * R;
             const float4* __restrict__ pab = Q.pts.pab;
             const float4* __restrict__ pc = Q.pts.pc;
-            int nsurf = 0, sstart = 0, quad0 = 0;
-            float ar = 0, ag = 0, ab = 0, aw = 0;
-            float min0 = kFltMax;
-            float4 near = make_float4(0, 0, 0, 0);
-            bool stop = false;
-            for (int li = 0; li < n && !stop; li++) {
+            // Three passes, so that the per-leaf sameSurface work does not
+            // sit inside the sample loop, where some lane of the wave meets
+            // the first kept sample of some leaf at almost every step:
+            //  1. which leaves hold a sample within R (a leaf's scan stops at
+            //     its first such sample unless every sample is counted);
+            //  2. surfaces over those leaves, front to back
+            //     (ReconstructionOur.cpp:466-484);
+            //  3. surface 0's samples in order: quadrant mask, nearest sample
+            //     and tent-weighted sum (:486-499, :268-297).
+            uint32_t kmask = 0;
+            for (int li = 0; li < n; li++) {
                 const int id = lid[li][tid];
                 const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
                 s_in_leaves += (unsigned long long)(k1 - k0);
-                bool first = true;
-                // Samples in groups of four (128 contiguous bytes per lane and
-                // load round trip); slots past k1 belong to the next leaf or
-                // the zeroed padding and are not tested.
-                for (int kg = k0; kg < k1 && !stop; kg += 4) {
+                bool done_leaf = false;
+                for (int kg = k0; kg < k1 && !done_leaf; kg += 4) {
+                    const float4* g = pab + 2 * (size_t)kg;
+                    float4 ga[4], gb[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        ga[e] = g[2 * e];
+                        gb[e] = g[2 * e + 1];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if (kg + e >= k1) break;
+                        if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
+                        float x, y;
+                        reproject_point(ga[e], gb[e], o, Q.cc0, Q.cc1, x, y);
+                        const float dx = x - o.x, dy = y - o.y;
+                        if (!(dx * dx + dy * dy <= R2)) continue;
+                        kmask |= 1u << li;
+                        s_kept++;
+                        if (!STATS) {
+                            done_leaf = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            // 2. surfaces; surface 0 = kept leaves in [first0, end0)
+            int nsurf = 0, sstart = 0, first0 = n, end0 = n;
+            for (uint32_t km = kmask; km; km &= km - 1) {
+                const int li = __builtin_ctz(km);
+                if (nsurf == 0) {
+                    nsurf = 1;
+                    sstart = li;
+                    first0 = li;
+                    continue;
+                }
+                bool same = true;
+                SurfSig sa, sb;
+                surf_sig(Q.leaf_b[lid[li][tid]].p, u0, v0, t0, u1, v1, t1, sa);
+                Bounds nb = Q.leaf_b[lid[sstart][tid]];
+                for (int j = sstart; j < li && same; ++j) {
+                    const Bounds cb = nb;
+                    if (j + 1 < li) nb = Q.leaf_b[lid[j + 1][tid]];
+                    surf_sig(cb.p, u0, v0, t0, u1, v1, t1, sb);
+                    same = same_surface_sig(sa, sb);
+                }
+                if (!same) {
+                    if (nsurf == 1) end0 = li;
+                    nsurf++;
+                    sstart = li;
+                    if (!STATS) break;  // surface 0 is complete
+                }
+            }
+            // 3. surface 0
+            int quad0 = 0;
+            float ar = 0, ag = 0, ab = 0, aw = 0;
+            float min0 = kFltMax;
+            float4 near = make_float4(0, 0, 0, 0);
+            for (uint32_t km = nsurf > 0 ? (kmask & ((end0 < 32 ? (1u << end0) : 0u) - 1u)) : 0u; km;
+                 km &= km - 1) {
+                const int li = __builtin_ctz(km);
+                const int id = lid[li][tid];
+                const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
+                for (int kg = k0; kg < k1; kg += 4) {
                     const float4* g = pab + 2 * (size_t)kg;
                     float4 ga[4], gb[4];
 #pragma unroll
@@ -750,60 +814,31 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
                         const int k = kg + e;
-                        if (k >= k1 || stop) break;
+                        if (k >= k1) break;
                         if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
                         float x, y;
                         reproject_point(ga[e], gb[e], o, Q.cc0, Q.cc1, x, y);
                         const float dx = x - o.x, dy = y - o.y;
                         const float d2 = dx * dx + dy * dy;
                         if (!(d2 <= R2)) continue;
-                        if (first) {
-                            first = false;
-                            bool same = true;
-                            // with no surface yet a new one opens whatever the
-                            // test says (ReconstructionOur.cpp:478)
-                            if (nsurf > 0 && sstart < li) {
-                                SurfSig a, b;
-                                surf_sig(Q.leaf_b[id].p, u0, v0, t0, u1, v1, t1, a);
-                                // planes of the next earlier leaf load while
-                                // the current pair is compared
-                                Bounds nb = Q.leaf_b[lid[sstart][tid]];
-                                for (int j = sstart; j < li && same; ++j) {
-                                    const Bounds cb = nb;
-                                    if (j + 1 < li) nb = Q.leaf_b[lid[j + 1][tid]];
-                                    surf_sig(cb.p, u0, v0, t0, u1, v1, t1, b);
-                                    same = same_surface_sig(a, b);
-                                }
-                            }
-                            if (!same || nsurf == 0) {
-                                if (nsurf == 1 && !STATS) {
-                                    stop = true;  // surface 0 is complete
-                                    break;
-                                }
-                                sstart = li;
-                                nsurf++;
-                            }
+                        quad0 |= 1 << ((dx >= 0 ? XPOS : 0) | (dy >= 0 ? YPOS : 0));
+                        const float dist = sqrtf(d2);
+                        const float4 col = pc[k];
+                        if (dist < min0) {
+                            min0 = dist;
+                            near = col;
                         }
-                        s_kept++;
-                        if (nsurf == 1) {
-                            quad0 |= 1 << ((dx >= 0 ? XPOS : 0) | (dy >= 0 ? YPOS : 0));
-                            const float dist = sqrtf(d2);
-                            const float4 col = pc[k];
-                            if (dist < min0) {
-                                min0 = dist;
-                                near = col;
-                            }
-                            const float wgt = 1 - dist / disp;
-                            if (wgt > 0) {
-                                ar += wgt * col.x;
-                                ag += wgt * col.y;
-                                ab += wgt * col.z;
-                                aw += wgt * 1.0f;
-                            }
+                        const float wgt = 1 - dist / disp;
+                        if (wgt > 0) {
+                            ar += wgt * col.x;
+                            ag += wgt * col.y;
+                            ab += wgt * col.z;
+                            aw += wgt * 1.0f;
                         }
                     }
                 }
             }
+            const bool stop = nsurf > 1;
             const bool f2 = nsurf == 0 || (quad0 != 0xF && (nsurf > 1 || stop));
             single = nsurf == 1 && !stop;
             if (f2) {

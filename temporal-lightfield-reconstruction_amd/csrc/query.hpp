@@ -62,6 +62,7 @@ struct QueryParams {
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
 // more 1R leaves go to the general kernel.
 constexpr int kFastLeafCap = 24;
+static_assert(kFastLeafCap <= 32, "the fast kernel keeps a 32-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
 
 // Resident waves of the general kernel per CU (its occupancy limit).
