@@ -20,7 +20,7 @@
  *   leaf build                          partition invariants (each kept
  *       sample in exactly one leaf, one t per leaf, leaf inside one initial
  *       leaf, sorted by (t,w) inside)                    [test_oracle_build]
- *   LBVH                                leaf sets equal brute force over all
+ *   hierarchy                           leaf sets equal brute force over all
  *       leaves                                           [test_oracle_build]
  *   gather / clustering                 gathered set equals brute force over
  *       leaves + per-sample distance test; surfaces contiguous
@@ -569,7 +569,7 @@ struct orc_ctx {
     std::vector<int> ileaf_offset;
     std::vector<RS> samples;          // layout order
     std::vector<Leaf> leaves;         // layout order
-    // LBVH over valid leaves
+    // hierarchy over valid leaves
     std::vector<int> leaf_order;
     std::vector<Inner> inner;
     int root = -1;                    // internal id, leaf id | tag, or -1
@@ -925,7 +925,7 @@ void kd_link(orc_ctx* c, int lo, int hi, int n) {
     nd.b = merge_tlb(a, b);
 }
 
-void build_lbvh(orc_ctx* c) {
+void build_hierarchy(orc_ctx* c) {
     c->leaf_order.clear();
     c->inner.clear();
     c->root = -1;
@@ -1471,7 +1471,7 @@ int orc_build(orc_ctx* c, float ap_adj, float foc_adj, int threads) {
     for (auto& o : outs) c->leaves.insert(c->leaves.end(), o.begin(), o.end());
     std::sort(c->leaves.begin(), c->leaves.end(), [](const Leaf& a, const Leaf& b) { return a.s0 < b.s0; });
 
-    build_lbvh(c);
+    build_hierarchy(c);
     c->built = true;
     return 0;
 }
@@ -1544,7 +1544,7 @@ void orc_get_leaves(const orc_ctx* c, int32_t* s01, float* b14) {
     }
 }
 
-void orc_get_lbvh(const orc_ctx* c, int32_t* order, int32_t* ch, float* b14) {
+void orc_get_hierarchy(const orc_ctx* c, int32_t* order, int32_t* ch, float* b14) {
     for (size_t i = 0; i < c->leaf_order.size(); i++) order[i] = c->leaf_order[i];
     for (size_t i = 0; i < c->inner.size(); i++) {
         ch[2 * i] = c->inner[i].c0;

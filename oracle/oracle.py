@@ -79,7 +79,7 @@ def lib():
         L.orc_get_initial_leaves.argtypes = [P, ip]
         L.orc_get_layout.argtypes = [P, ip]
         L.orc_get_leaves.argtypes = [P, ip, fp]
-        L.orc_get_lbvh.argtypes = [P, ip, ip, fp]
+        L.orc_get_hierarchy.argtypes = [P, ip, ip, fp]
         L.orc_output_pattern.argtypes = [C.c_int, C.c_int, fp, C.c_void_p]
         L.orc_reconstruct.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
                                       C.POINTER(Stats)]
@@ -173,13 +173,13 @@ class Oracle:
         lib().orc_get_leaves(self.h, s01, b)
         return s01, b
 
-    def lbvh(self):
+    def hierarchy(self):
         bi = self.info()
         nvalid = bi.num_internal + 1 if bi.root != -1 else 0
         order = np.zeros(max(nvalid, 1), np.int32)
         ch = np.zeros((max(bi.num_internal, 1), 2), np.int32)
         b = np.zeros((max(bi.num_internal, 1), 14), np.float32)
-        lib().orc_get_lbvh(self.h, order, ch, b)
+        lib().orc_get_hierarchy(self.h, order, ch, b)
         return order[:nvalid], ch[:bi.num_internal], b[:bi.num_internal]
 
     @staticmethod

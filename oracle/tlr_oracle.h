@@ -20,8 +20,9 @@
  *   - leaves whose time span is empty are never gathered;
  *   - box values of samples with an empty time span are zero (the reference
  *     reads stale scratch contents there);
- *   - the hierarchy above the leaves is a Morton LBVH (the reference's is an
- *     agglomerative tree; the leaf set a query finds does not depend on it).
+ *   - the hierarchy above the leaves is a balanced median-split tree (the
+ *     reference's is an agglomerative tree; the leaf set a query finds does
+ *     not depend on it).
  *
  * Parity status of each function is listed in the header comment of
  * oracle.cpp.
@@ -41,7 +42,7 @@ orc_ctx* orc_create(int W, int H, int spp, float C0, float C1, const float* reco
 void     orc_destroy(orc_ctx* c);
 const char* orc_error(const orc_ctx* c);
 
-/* Build: spp rounding, CoC adjust, reprojection, grid, leaves, LBVH.
+/* Build: spp rounding, CoC adjust, reprojection, grid, leaves, hierarchy.
  * threads: worker threads for the leaf stage (<=0: hardware concurrency).
  * Returns 0 on success, else a TLR_E_* style code (see include/tlr.h). */
 int orc_build(orc_ctx* c, float aperture_adjust, float focal_adjust, int threads);
@@ -58,7 +59,7 @@ typedef struct orc_build_info {
     int   max_hits;
     int   num_initial_nodes, num_initial_leaves;
     int   num_leaves, num_internal;
-    int   root;                     /* LBVH root id (internal id, or leaf id | 0x80000000) */
+    int   root;                     /* hierarchy root (internal id, or leaf id | 0x80000000) */
 } orc_build_info;
 void orc_get_build_info(const orc_ctx* c, orc_build_info* out);
 
@@ -73,9 +74,9 @@ void orc_get_initial_leaves(const orc_ctx* c, int32_t* out6);
 void orc_get_layout(const orc_ctx* c, int32_t* src);
 /* Leaves in layout order: s0,s1 and 14 floats (12 plane coeffs, tspan). */
 void orc_get_leaves(const orc_ctx* c, int32_t* s01, float* bounds14);
-/* LBVH: sorted leaf order (leaf ids), internal nodes: child0, child1 (leaf ids
+/* Hierarchy: sorted leaf order (leaf ids), internal nodes: child0, child1 (leaf ids
  * tagged with 0x80000000) and 14 floats of bounds. */
-void orc_get_lbvh(const orc_ctx* c, int32_t* leaf_order, int32_t* children2, float* bounds14);
+void orc_get_hierarchy(const orc_ctx* c, int32_t* leaf_order, int32_t* children2, float* bounds14);
 
 /* Reconstruction parameters. */
 typedef struct orc_recon_params {
@@ -125,7 +126,7 @@ typedef struct orc_query_info {
 int orc_query(orc_ctx* c, const orc_recon_params* p, int x, int y, int i, orc_query_info* out);
 
 /* Brute-force leaf set (every valid leaf whose own bounds intersect), for
- * checking that the LBVH traversal loses nothing. */
+ * checking that the hierarchy traversal loses nothing. */
 int orc_leaves_bruteforce(orc_ctx* c, const float* o5, float R, int32_t* leaves, int max_leaves);
 
 /* Instrumentation: hierarchy nodes tested by orc_gather/orc_reconstruct so

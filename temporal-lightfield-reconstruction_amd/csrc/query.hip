@@ -4,7 +4,7 @@
 // (ReconstructionOur.cpp:50-511) and FilterTask::process
 // (Reconstruction.cpp:642-720).  All decisions use the same FP32 values as
 // the sequential reference; lanes only split independent work:
-//   - hierarchy traversal: a wave-wide frontier over the LBVH (the set of
+//   - hierarchy traversal: a wave-wide frontier over the tree (the set of
 //     leaves found does not depend on visiting order);
 //   - leaf ordering: rank sort by (key, leaf id);
 //   - per-leaf sample tests: one sample per lane, kept samples compacted in
@@ -634,19 +634,20 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
 
 // ---------------------------------------------------------------------------
 // Fast path: one lane per output sample at 1R (collectInputSamples2 with
-// R = dispersion, ReconstructionOur.cpp:94).  The lane walks the LBVH
-// stacklessly (escape links), keeps its leaves sorted by (key, leaf id) in
-// LDS, then streams their samples in that order, assigning surfaces exactly
-// like the general kernel.  Surface 0's tent-weighted sum, quadrant mask and
-// nearest sample are accumulated on the fly in sample order.  The query is
-// finished here when the reference would not fetch 2R
-// (ReconstructionOur.cpp:99): one surface, or a second surface behind a
-// first one that covers all four quadrants; then the found surface is
-// surface 0 (:204) and its filtered colour is the result (:268-297).  Every
-// other query (no surface, a 2R fetch, or more than kFastLeafCap leaves) is
-// appended to fail_list for the general kernel.  STATS keeps walking all
-// leaves to count the reference's Stats; otherwise the lane stops as soon as
-// the second surface starts.
+// R = dispersion, ReconstructionOur.cpp:94).  The lane walks the tree
+// stacklessly (escape links) and keeps the leaves it finds sorted by
+// (key, leaf id) in LDS.  Then, in three passes over them: which leaves hold a
+// sample within R; the surfaces over those leaves, front to back, exactly as
+// the general kernel forms them; surface 0's quadrant mask, nearest sample
+// and tent-weighted sum in sample order.  The query is finished here when the
+// reference would not fetch 2R (ReconstructionOur.cpp:99): one surface, or a
+// second surface behind a first one that covers all four quadrants; then the
+// found surface is surface 0 (:204) and its filtered colour is the result
+// (:268-297).  Every other query (no surface, a 2R fetch, or more than
+// kFastLeafCap leaves) is appended to fail_list for the general kernel.
+// STATS visits every leaf and sample to count the reference's Stats;
+// otherwise a leaf's scan stops at its first sample within R and the surface
+// pass stops when the second surface starts.
 // ---------------------------------------------------------------------------
 constexpr int kFastBlock = 128;
 

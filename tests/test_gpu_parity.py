@@ -93,7 +93,7 @@ def test_hierarchy_bitexact(tlr, orc, case):
     bounds as the oracle's recursive build."""
     sbuf, ctx, o = _pair(tlr, orc, case)
     go, gc, gb = ctx.hierarchy()
-    ro, rc, rb = o.lbvh()
+    ro, rc, rb = o.hierarchy()
     assert go.size == ro.size == ctx.build_info().num_valid_leaves
     assert np.array_equal(go, ro), "leaf order differs at %s" % np.nonzero(go != ro)[0][:5]
     assert np.array_equal(gc, rc), "children differ"

@@ -1,5 +1,5 @@
 """CPU pins of the oracle's build: reprojection closed forms, grid and leaf
-partition invariants, bounds containment/merge invariants, LBVH equivalence
+partition invariants, bounds containment/merge invariants, hierarchy equivalence
 with brute force.  No GPU."""
 import numpy as np
 import pytest
@@ -164,7 +164,7 @@ def test_merge_is_conservative(orc):
 
 
 @pytest.mark.parametrize("name", ["defocus", "motion", "strong"])
-def test_lbvh_finds_every_intersecting_leaf(built, name):
+def test_hierarchy_finds_every_intersecting_leaf(built, name):
     sb, o = built[name]
     rng = np.random.default_rng(12)
     disp = {1: 2.0, 2: 1.34, 4: 1.10, 8: 0.80, 16: 0.50}[o.info().spp_pow2]
@@ -186,7 +186,7 @@ def test_hierarchy_is_a_balanced_median_tree(built, name):
     so the depth is ceil(log2 n); each internal bound holds its children at
     the corners of the (t, u) domain."""
     sb, o = built[name]
-    order, ch, b = o.lbvh()
+    order, ch, b = o.hierarchy()
     s01, lb = o.leaves()
     valid = np.nonzero(lb[:, 12] < lb[:, 13])[0]
     n = order.size
