@@ -124,6 +124,45 @@ tlr_status  tlr_reconstruct(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba
 /* Blocks until the context's stream is idle. */
 tlr_status  tlr_sync(tlr_ctx* ctx);
 
+/* ---- Batched reconstruction (no reference counterpart: the reference
+ * builds and filters one TreeGather at a time) ------------------------------
+ * Many sample buffers of uneven size go through one build each and then one
+ * query phase for all of them: a single persistent launch of each query
+ * kernel over every buffer, captured in a hipGraph that is replayed while the
+ * batch's shape (items, sizes, output spp) stays the same.  The first batch
+ * of a context times the query kernels' launch variants on that batch and
+ * keeps the fastest (tlr_batch_retune starts over).  Buffers are uploaded
+ * once into numbered slots; an item names its slot.  One context runs one
+ * batch at a time. */
+typedef struct tlr_batch_item {
+    int    slot;                 /* filled by tlr_batch_set_samples           */
+    float  aperture_adjust;      /* TreeGather's apertureAdjust (1 = none)    */
+    float  focal_adjust;         /* TreeGather's focalDistanceAdjust          */
+    int    out_spp;              /* output samples per pixel                  */
+    float* rgba_out;             /* W*H*4 floats, or NULL to keep on device   */
+    int    out_on_device;        /* rgba_out is device memory                 */
+    float* build_ms;             /* optional: this item's build time          */
+} tlr_batch_item;
+
+typedef struct tlr_batch_stats {
+    double build_ms;             /* sum of the items' builds                  */
+    double query_ms;             /* the captured query phase, all items       */
+    double total_ms;
+    double output_samples;       /* over all items                            */
+    double general_queries;      /* handed to the general kernel              */
+    int    fast_leaf_cap, fast_block, general_blocks;  /* tuned launch shape  */
+    int    graph_reused;         /* 1: the previous call's graph was replayed */
+    int    autotuned;            /* 1: this call ran the autotuner            */
+} tlr_batch_stats;
+
+/* Uploads buffer `slot` (AoS records as for tlr_set_samples) to device
+ * memory, where it stays until overwritten or the context is destroyed. */
+tlr_status  tlr_batch_set_samples(tlr_ctx* ctx, int slot, const tlr_header* h, const float* records, int64_t n);
+/* Builds and reconstructs every item; images go to rgba_out. */
+tlr_status  tlr_reconstruct_batch(tlr_ctx* ctx, const tlr_batch_item* items, int n, tlr_batch_stats* stats);
+/* Forgets the tuned launch shape; the next batch tunes again. */
+tlr_status  tlr_batch_retune(tlr_ctx* ctx);
+
 /* ---- Introspection for parity checks (no reference counterpart) -------- */
 tlr_status  tlr_get_build_info(tlr_ctx* ctx, tlr_build_info* out);
 /* reprojected x,y,w,t per input sample (w = 0 if discarded) */

@@ -33,6 +33,7 @@ EXPORTED = [
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
+    "tlr_batch_set_samples", "tlr_reconstruct_batch", "tlr_batch_retune",
 ]
 GATE_FEATURES = 24
 GATE_PARAMS = 5825
@@ -68,6 +69,22 @@ class Stats(C.Structure):
         "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
         "fast_ms", "general_ms", "general_queries", "direct_ms", "gate_ms", "gate_rejected",
         "fast_leaf_nodes", "fast_samples_in_leaf_nodes", "fast_samples_within_1r")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("slot", C.c_int), ("aperture_adjust", C.c_float), ("focal_adjust", C.c_float),
+                ("out_spp", C.c_int), ("rgba_out", C.c_void_p), ("out_on_device", C.c_int),
+                ("build_ms", C.POINTER(C.c_float))]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("query_ms", C.c_double), ("total_ms", C.c_double),
+                ("output_samples", C.c_double), ("general_queries", C.c_double),
+                ("fast_leaf_cap", C.c_int), ("fast_block", C.c_int), ("general_blocks", C.c_int),
+                ("graph_reused", C.c_int), ("autotuned", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -128,6 +145,9 @@ def lib():
         L.tlr_copy_initial_leaves.argtypes = [V, V]
         L.tlr_copy_leaves.argtypes = [V, V, V]
         L.tlr_copy_hierarchy.argtypes = [V, V, V, V]
+        L.tlr_batch_set_samples.argtypes = [V, C.c_int, C.POINTER(Header), V, C.c_int64]
+        L.tlr_reconstruct_batch.argtypes = [V, C.POINTER(BatchItem), C.c_int, C.POINTER(BatchStats)]
+        L.tlr_batch_retune.argtypes = [V]
         L.tlr_copy_query_colors.argtypes = [V, V]
         L.tlr_output_pattern.restype = C.c_int
         L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
@@ -281,6 +301,43 @@ class Ctx:
         st = Stats()
         self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img), 0, C.byref(st)))
         return img, st.as_dict()
+
+    def batch_set_samples(self, slot, sbuf):
+        """Uploads a sample buffer into batch slot `slot` (device-resident)."""
+        self._chk(lib().tlr_batch_set_samples(self.h, slot, C.byref(sbuf.header()), _p(sbuf.records),
+                                              sbuf.records.shape[0]))
+        if not hasattr(self, "_slot_shapes"):
+            self._slot_shapes = {}
+        self._slot_shapes[slot] = (sbuf.height, sbuf.width)
+
+    def reconstruct_batch(self, jobs, outs=None):
+        """jobs: list of (slot, out_spp[, aperture_adjust, focal_adjust]);
+        outs: optional list of (H, W, 4) float32 arrays (host).  Returns
+        (images, per-item build ms, batch stats)."""
+        n = len(jobs)
+        items = (BatchItem * n)()
+        bms = (C.c_float * n)()
+        imgs = []
+        for k, job in enumerate(jobs):
+            slot, out_spp = job[0], job[1]
+            items[k].slot = slot
+            items[k].out_spp = out_spp
+            items[k].aperture_adjust = job[2] if len(job) > 2 else 1.0
+            items[k].focal_adjust = job[3] if len(job) > 3 else 1.0
+            img = outs[k] if outs is not None else None
+            if img is None and getattr(self, "_slot_shapes", {}).get(slot) is not None:
+                h, w = self._slot_shapes[slot]
+                img = np.empty((h, w, 4), np.float32)
+            imgs.append(img)
+            items[k].rgba_out = img.ctypes.data if img is not None else None
+            items[k].out_on_device = 0
+            items[k].build_ms = C.cast(C.addressof(bms) + 4 * k, C.POINTER(C.c_float))
+        st = BatchStats()
+        self._chk(lib().tlr_reconstruct_batch(self.h, items, n, C.byref(st)))
+        return imgs, [bms[k] for k in range(n)], st.as_dict()
+
+    def batch_retune(self):
+        self._chk(lib().tlr_batch_retune(self.h))
 
     @staticmethod
     def _params(out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,

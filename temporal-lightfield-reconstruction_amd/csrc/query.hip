@@ -432,8 +432,196 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
     return any;
 }
 
+// One output sample of the hand-off list, the whole algorithm
+// (Filterer::reconstruct, ReconstructionOur.cpp:50-358), wave-uniform.
+__device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMem& m, unsigned qi, bool skip1r,
+                                              unsigned long long* st) {
+    const int lane = threadIdx.x & 63;
+    const int M = Q.M;
+    const unsigned pix = qi / (unsigned)M;
+    const int i = (int)(qi - pix * (unsigned)M);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
+    const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
+    OSample o;
+    o.x = tp[0] + (float)px;
+    o.y = tp[1] + (float)py;
+    o.u = tp[2];
+    o.v = tp[3];
+    o.t = tp[4];
+    st[ST_COUNT] += 1;
+
+    // Filterer::reconstruct: ReconstructionOur.cpp:50-159
+    float disp = dispersion_for_spp(Q.spp_pow2);
+    Gather g{0, 0, 0, 0};
+    int overflow = 0;
+    int n1 = 0;
+    bool f2 = true;
+    // One gather call site (the gather is large; every inlined copy
+    // costs registers): pass 0 = 1R, 1 = 2R, 2..4 = emergency passes
+    // (dispersion doubled before each, gather at 2x dispersion).
+    int pass = skip1r ? 1 : 0;
+    for (;;) {
+        g = gather(Q, m, o, pass == 0 ? disp : disp * 2, st);
+        overflow = g.overflow;
+        if (overflow) break;
+        if (pass == 0) {
+            n1 = g.total;
+            const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
+            f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
+            if (!f2) break;
+            pass = 1;
+            continue;
+        }
+        if (g.total != 0 || pass == 4) break;
+        if (pass == 1) st[ST_AT_LEAST_ONE] += 1;
+        disp *= 2.f;
+        pass++;
+    }
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!overflow) {
+        st[ST_2R] += f2 ? 1 : 0;
+        st[ST_WITHIN_1R] += (unsigned long long)n1;
+        const int nsurf = g.nsurf;
+        int numSurfaces = nsurf;
+        int used = g.used;
+        // Tiny-surface merge: ReconstructionOur.cpp:137-156.  dst's list
+        // becomes dst ++ src (materialised at the end of the sample area);
+        // minIndex is copied as is (it then indexes the merged list).
+        if (f2) {
+            for (int si = 0; si < nsurf - 1; si++) {
+                const int ssize = uni(m.f_size[si]);
+                if (ssize < 4) {
+                    const int sst = uni(m.f_start[si]);
+                    const int dst = uni(m.f_start[si + 1]);
+                    const int dsize = uni(m.f_size[si + 1]);
+                    if (used + dsize + ssize > kSampleCap) {
+                        overflow = 1;
+                        break;
+                    }
+                    for (int j = lane; j < dsize + ssize; j += 64) {
+                        const int from = j < dsize ? dst + j : sst + (j - dsize);
+                        m.sx[used + j] = m.sx[from];
+                        m.sy[used + j] = m.sy[from];
+                        m.sd[used + j] = m.sd[from];
+                        m.sidx[used + j] = m.sidx[from];
+                        m.srpos[used + j] = m.srpos[from];
+                    }
+                    const float smin = unif(m.f_min[si]);
+                    const float dmin = unif(m.f_min[si + 1]);
+                    const int smini = uni(m.f_mini[si]);
+                    wave_sync_mem();
+                    if (lane == 0) {
+                        m.f_start[si + 1] = used;
+                        m.f_size[si + 1] = dsize + ssize;
+                        m.f_size[si] = 0;
+                        if (smin < dmin) {
+                            m.f_min[si + 1] = smin;
+                            m.f_mini[si + 1] = smini;
+                        }
+                    }
+                    wave_sync_mem();
+                    used += dsize + ssize;
+                    numSurfaces--;
+                }
+            }
+        }
+        st[ST_SURFACES] += (unsigned long long)numSurfaces;
+        st[ST_MERGED] += (numSurfaces != nsurf) ? 1 : 0;
+
+        // Coverage and filtering, front to back: ReconstructionOur.cpp:165-302
+        bool found = false;
+        float cr = 0, cg = 0, cb = 0, ca = 0;
+        float nearest = kFltMax;
+        int nI0 = -1, nI1 = 1;
+        for (int si = 0; si < nsurf && !found && !overflow; si++) {
+            const int fs = uni(m.f_start[si]);
+            const int n = uni(m.f_size[si]);
+            if (n == 0) continue;
+            const float fmin = unif(m.f_min[si]);
+            const int fmini = uni(m.f_mini[si]);
+            if (fmin < nearest) {
+                nearest = fmin;
+                nI0 = si;
+                nI1 = fmini;
+            }
+            if (!f2 || fmin < 0.f || si == nsurf - 1) found = true;
+            if (n >= 200) found = true;
+            if (!found) found = triangle_search(m, fs, n, o, disp);
+            if (found) {
+                cr = cg = cb = ca = 0;
+                // weights and colours of 64 samples load in parallel; the
+                // sum then runs in sample order over lane-held values
+                for (int c0 = 0; c0 < n; c0 += 64) {
+                    const int k = c0 + lane;
+                    float wgt = 0.0f;
+                    float4 c = make_float4(0, 0, 0, 0);
+                    if (k < n) {
+                        const float rx = m.sx[fs + k], ry = m.sy[fs + k];
+                        const float dx = o.x - rx, dy = o.y - ry;
+                        wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
+                        if (wgt > 0) c = Q.pts.pc[m.sidx[fs + k]];
+                    }
+                    const int cnt = min(64, n - c0);
+                    for (int j = 0; j < cnt; j++) {
+                        const float wj = __shfl(wgt, j, 64);
+                        if (wj <= 0) continue;
+                        cr += wj * __shfl(c.x, j, 64);
+                        cg += wj * __shfl(c.y, j, 64);
+                        cb += wj * __shfl(c.z, j, 64);
+                        ca += wj * 1.0f;
+                    }
+                }
+                const float oow = rcp(ca);
+                cr *= oow; cg *= oow; cb *= oow; ca *= oow;
+                nI0 = si;
+                nI1 = fmini;
+            }
+        }
+        if (!overflow && nI0 != -1) {
+            if (ca == 0) {
+                const int fs = uni(m.f_start[nI0]);
+                const float4 c = Q.pts.pc[uni(m.sidx[fs + nI1])];
+                cr = c.x;
+                cg = c.y;
+                cb = c.z;
+                ca = 1.0f;
+            }
+            if (Q.lens_gauss) {
+                // Gaussian(x, 0.5, 0): Reconstruction.hpp:378
+                const float e = 2.718281828f;
+                const float gu = powf(e, -((o.u - 0) * (o.u - 0)) / (2 * (0.5f * 0.5f)));
+                const float gv = powf(e, -((o.v - 0) * (o.v - 0)) / (2 * (0.5f * 0.5f)));
+                const float gg = gu * gv;
+                cr *= gg; cg *= gg; cb *= gg; ca *= gg;
+            }
+            if (Q.time_gauss) {
+                const float e = 2.718281828f;
+                const float gg = powf(e, -((o.t - 0.5f) * (o.t - 0.5f)) / (2 * (0.25f * 0.25f)));
+                cr *= gg; cg *= gg; cb *= gg; ca *= gg;
+            }
+            col = make_float4(cr, cg, cb, ca);
+        }
+    }
+    if (overflow) st[ST_OVERFLOW] += 1;
+    if (lane == 0) Q.qcol[qi] = col;
+    wave_sync_mem();
+}
+
+__device__ __forceinline__ void flush_stats(unsigned long long* dst, const unsigned long long* st, bool all) {
+    if ((threadIdx.x & 63) != 0) return;
+    if (all) {
+#pragma unroll
+        for (int k = 0; k < kNumStats; k++)
+            if (st[k]) atomicAdd(&dst[k], st[k]);
+    } else if (st[ST_OVERFLOW]) {
+        atomicAdd(&dst[ST_OVERFLOW], st[ST_OVERFLOW]);
+    }
+}
+
 // 7 waves per SIMD (the register limit this sets spills a little; the
-// kernel waits on memory and gains more from resident waves)
+// kernel waits on memory and gains more from resident waves).  Persistent:
+// waves take queries from an atomic counter until the list is done.
 __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -441,7 +629,6 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     unsigned long long st[kNumStats];
 #pragma unroll
     for (int k = 0; k < kNumStats; k++) st[k] = 0;
-    const int M = Q.M;
     const unsigned nq = Q.qlist ? *Q.qcount : Q.num_queries;
     for (;;) {
         unsigned qi = 0;
@@ -455,181 +642,34 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
             skip1r = (qi & kNeeds2R) != 0;
             qi &= ~kNeeds2R;
         }
-        const unsigned pix = qi / (unsigned)M;
-        const int i = (int)(qi - pix * (unsigned)M);
-        const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
-        const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
-        const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
-        OSample o;
-        o.x = tp[0] + (float)px;
-        o.y = tp[1] + (float)py;
-        o.u = tp[2];
-        o.v = tp[3];
-        o.t = tp[4];
-        st[ST_COUNT] += 1;
-
-        // Filterer::reconstruct: ReconstructionOur.cpp:50-159
-        float disp = dispersion_for_spp(Q.spp_pow2);
-        Gather g{0, 0, 0, 0};
-        int overflow = 0;
-        int n1 = 0;
-        bool f2 = true;
-        // One gather call site (the gather is large; every inlined copy
-        // costs registers): pass 0 = 1R, 1 = 2R, 2..4 = emergency passes
-        // (dispersion doubled before each, gather at 2x dispersion).
-        int pass = skip1r ? 1 : 0;
-        for (;;) {
-            g = gather(Q, m, o, pass == 0 ? disp : disp * 2, st);
-            overflow = g.overflow;
-            if (overflow) break;
-            if (pass == 0) {
-                n1 = g.total;
-                const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
-                f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
-                if (!f2) break;
-                pass = 1;
-                continue;
-            }
-            if (g.total != 0 || pass == 4) break;
-            if (pass == 1) st[ST_AT_LEAST_ONE] += 1;
-            disp *= 2.f;
-            pass++;
-        }
-        float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!overflow) {
-            st[ST_2R] += f2 ? 1 : 0;
-            st[ST_WITHIN_1R] += (unsigned long long)n1;
-            const int nsurf = g.nsurf;
-            int numSurfaces = nsurf;
-            int used = g.used;
-            // Tiny-surface merge: ReconstructionOur.cpp:137-156.  dst's list
-            // becomes dst ++ src (materialised at the end of the sample area);
-            // minIndex is copied as is (it then indexes the merged list).
-            if (f2) {
-                for (int si = 0; si < nsurf - 1; si++) {
-                    const int ssize = uni(m.f_size[si]);
-                    if (ssize < 4) {
-                        const int sst = uni(m.f_start[si]);
-                        const int dst = uni(m.f_start[si + 1]);
-                        const int dsize = uni(m.f_size[si + 1]);
-                        if (used + dsize + ssize > kSampleCap) {
-                            overflow = 1;
-                            break;
-                        }
-                        for (int j = lane; j < dsize + ssize; j += 64) {
-                            const int from = j < dsize ? dst + j : sst + (j - dsize);
-                            m.sx[used + j] = m.sx[from];
-                            m.sy[used + j] = m.sy[from];
-                            m.sd[used + j] = m.sd[from];
-                            m.sidx[used + j] = m.sidx[from];
-                            m.srpos[used + j] = m.srpos[from];
-                        }
-                        const float smin = unif(m.f_min[si]);
-                        const float dmin = unif(m.f_min[si + 1]);
-                        const int smini = uni(m.f_mini[si]);
-                        wave_sync_mem();
-                        if (lane == 0) {
-                            m.f_start[si + 1] = used;
-                            m.f_size[si + 1] = dsize + ssize;
-                            m.f_size[si] = 0;
-                            if (smin < dmin) {
-                                m.f_min[si + 1] = smin;
-                                m.f_mini[si + 1] = smini;
-                            }
-                        }
-                        wave_sync_mem();
-                        used += dsize + ssize;
-                        numSurfaces--;
-                    }
-                }
-            }
-            st[ST_SURFACES] += (unsigned long long)numSurfaces;
-            st[ST_MERGED] += (numSurfaces != nsurf) ? 1 : 0;
-
-            // Coverage and filtering, front to back: ReconstructionOur.cpp:165-302
-            bool found = false;
-            float cr = 0, cg = 0, cb = 0, ca = 0;
-            float nearest = kFltMax;
-            int nI0 = -1, nI1 = 1;
-            for (int si = 0; si < nsurf && !found && !overflow; si++) {
-                const int fs = uni(m.f_start[si]);
-                const int n = uni(m.f_size[si]);
-                if (n == 0) continue;
-                const float fmin = unif(m.f_min[si]);
-                const int fmini = uni(m.f_mini[si]);
-                if (fmin < nearest) {
-                    nearest = fmin;
-                    nI0 = si;
-                    nI1 = fmini;
-                }
-                if (!f2 || fmin < 0.f || si == nsurf - 1) found = true;
-                if (n >= 200) found = true;
-                if (!found) found = triangle_search(m, fs, n, o, disp);
-                if (found) {
-                    cr = cg = cb = ca = 0;
-                    // weights and colours of 64 samples load in parallel; the
-                    // sum then runs in sample order over lane-held values
-                    for (int c0 = 0; c0 < n; c0 += 64) {
-                        const int k = c0 + lane;
-                        float wgt = 0.0f;
-                        float4 c = make_float4(0, 0, 0, 0);
-                        if (k < n) {
-                            const float rx = m.sx[fs + k], ry = m.sy[fs + k];
-                            const float dx = o.x - rx, dy = o.y - ry;
-                            wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
-                            if (wgt > 0) c = Q.pts.pc[m.sidx[fs + k]];
-                        }
-                        const int cnt = min(64, n - c0);
-                        for (int j = 0; j < cnt; j++) {
-                            const float wj = __shfl(wgt, j, 64);
-                            if (wj <= 0) continue;
-                            cr += wj * __shfl(c.x, j, 64);
-                            cg += wj * __shfl(c.y, j, 64);
-                            cb += wj * __shfl(c.z, j, 64);
-                            ca += wj * 1.0f;
-                        }
-                    }
-                    const float oow = rcp(ca);
-                    cr *= oow; cg *= oow; cb *= oow; ca *= oow;
-                    nI0 = si;
-                    nI1 = fmini;
-                }
-            }
-            if (!overflow && nI0 != -1) {
-                if (ca == 0) {
-                    const int fs = uni(m.f_start[nI0]);
-                    const float4 c = Q.pts.pc[uni(m.sidx[fs + nI1])];
-                    cr = c.x;
-                    cg = c.y;
-                    cb = c.z;
-                    ca = 1.0f;
-                }
-                if (Q.lens_gauss) {
-                    // Gaussian(x, 0.5, 0): Reconstruction.hpp:378
-                    const float e = 2.718281828f;
-                    const float gu = powf(e, -((o.u - 0) * (o.u - 0)) / (2 * (0.5f * 0.5f)));
-                    const float gv = powf(e, -((o.v - 0) * (o.v - 0)) / (2 * (0.5f * 0.5f)));
-                    const float gg = gu * gv;
-                    cr *= gg; cg *= gg; cb *= gg; ca *= gg;
-                }
-                if (Q.time_gauss) {
-                    const float e = 2.718281828f;
-                    const float gg = powf(e, -((o.t - 0.5f) * (o.t - 0.5f)) / (2 * (0.25f * 0.25f)));
-                    cr *= gg; cg *= gg; cb *= gg; ca *= gg;
-                }
-                col = make_float4(cr, cg, cb, ca);
-            }
-        }
-        if (overflow) st[ST_OVERFLOW] += 1;
-        if (lane == 0) Q.qcol[qi] = col;
-        wave_sync_mem();
+        general_query(Q, m, qi, skip1r, st);
     }
-    if (lane == 0 && Q.collect_stats) {
+    flush_stats(Q.stats, st, Q.collect_stats != 0);
+}
+
+// Batched form: one list of hand-offs from every buffer of a batch, with the
+// buffer of each entry beside it.
+__global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const WaveMem m = wave_mem(B.scratch + (size_t)wave * kWaveScratchWords);
+    unsigned long long st[kNumStats];
 #pragma unroll
-        for (int k = 0; k < kNumStats; k++)
-            if (st[k]) atomicAdd(&Q.stats[k], st[k]);
+    for (int k = 0; k < kNumStats; k++) st[k] = 0;
+    const unsigned nq = *B.list_count;
+    for (;;) {
+        unsigned e = 0;
+        if (lane == 0) e = atomicAdd(B.list_next, 1u);
+        e = (unsigned)uni((int)e);
+        if (e >= nq) break;
+        unsigned qi = (unsigned)uni((int)B.list[e]);
+        const int item = uni((int)B.list_item[e]);
+        const bool skip1r = (qi & kNeeds2R) != 0;
+        qi &= ~kNeeds2R;
+        const QueryParams Q = B.items[item];
+        general_query(Q, m, qi, skip1r, st);
     }
-    if (lane == 0 && !Q.collect_stats && st[ST_OVERFLOW]) atomicAdd(&Q.stats[ST_OVERFLOW], st[ST_OVERFLOW]);
+    flush_stats(B.stats, st, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -658,15 +698,12 @@ __device__ __forceinline__ float leaf_key(const LeafRec* leaves, int id, float t
     return r.x + (t - r.y) * r.z;
 }
 
-template <bool STATS>
-__global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
-    // per lane: leaf ids sorted by (key, id); keys are recomputed on compare
-    __shared__ int lid[kFastLeafCap][kFastBlock];
+// One block of the fast path: `slot` is this lane's query index (or list
+// position).  lid: the block's LDS leaf lists, CAP entries per lane.
+template <bool STATS, int CAP, int BLOCK>
+__device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, unsigned item, int (*lid)[BLOCK]) {
     const int tid = threadIdx.x;
     const int M = Q.M;
-    // Queries in scan order (pixel-major): the whole device sweeps one band
-    // of rows, whose samples stay resident in the last-level cache.
-    const unsigned slot = blockIdx.x * kFastBlock + tid;
     const bool active = slot < (Q.fast_list ? *Q.qcount : Q.num_queries);
     const unsigned q = active ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
     const unsigned pix = q / (unsigned)M;
@@ -700,7 +737,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
                     node = child;
                     continue;
                 }
-                if (n == kFastLeafCap) {
+                if (n == CAP) {
                     fail = true;
                     break;
                 }
@@ -876,7 +913,10 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
         unsigned base = 0;
         if (lane == 0) base = atomicAdd(Q.fail_count, (unsigned)__popcll(fm));
         base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-        if (active && fail) Q.fail_list[base + lanes_below(fm)] = q | (needs2r ? kNeeds2R : 0u);
+        if (active && fail) {
+            Q.fail_list[base + lanes_below(fm)] = q | (needs2r ? kNeeds2R : 0u);
+            if (Q.fail_item) Q.fail_item[base + lanes_below(fm)] = item;
+        }
     }
     if (STATS) {
         // finished queries report everything; queries handed over for a 2R
@@ -901,6 +941,40 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
             atomicAdd(&Q.stats[ST_FAST_IN_LEAVES], v[1]);
             atomicAdd(&Q.stats[ST_FAST_KEPT], v[2]);
         }
+    }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
+    // per lane: leaf ids sorted by (key, id); keys are recomputed on compare
+    __shared__ int lid[kFastLeafCap][kFastBlock];
+    // Queries in scan order (pixel-major): the whole device sweeps one band
+    // of rows, whose samples stay resident in the last-level cache.
+    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, blockIdx.x * kFastBlock + threadIdx.x, 0u, lid);
+}
+
+// Batched fast path: one persistent launch over every buffer of a batch.
+// Blocks take logical blocks of BLOCK queries from an atomic counter (buffers
+// in order, each buffer's queries in scan order) until all are done.
+template <int CAP, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
+    __shared__ int lid[CAP][BLOCK];
+    __shared__ unsigned next;
+    for (;;) {
+        __syncthreads();  // every lane is done with the previous block
+        if (threadIdx.x == 0) next = atomicAdd(B.block_next, 1u);
+        __syncthreads();
+        const unsigned b = (unsigned)uni((int)next);  // wave-uniform: scalar loads below
+        if (b >= B.total_blocks) break;
+        // buffer: the last one whose first block is <= b
+        int lo = 0, hi = B.n_items - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (B.block_prefix[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const QueryParams Q = B.items[lo];
+        fast_query<false, CAP, BLOCK>(Q, (b - B.block_prefix[lo]) * BLOCK + threadIdx.x, (unsigned)lo, lid);
     }
 }
 
@@ -1103,6 +1177,12 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ q
     reinterpret_cast<float4*>(out)[p] = o;
 }
 
+// Graph-safe clear of small counters (no memset nodes in captured graphs).
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 int general_waves_per_cu() {
@@ -1148,6 +1228,53 @@ hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out, h
     const int n = W * H;
     if (n == 0) return hipSuccess;
     k_combine<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(qcol, W, H, M, out);
+    return hipGetLastError();
+}
+
+
+hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_fill_u32<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+// Fast-kernel variants the batch autotuner chooses from: (leaf cap, block).
+static const int kFastVariants[kNumFastVariants][2] = {{24, 128}, {16, 128}, {24, 64}, {16, 256}};
+
+int fast_variant_cap(int v) { return kFastVariants[v][0]; }
+int fast_variant_block(int v) { return kFastVariants[v][1]; }
+
+template <int CAP, int BLOCK>
+static int fast_batch_grid(int cus) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_fast_batch<CAP, BLOCK>, BLOCK, 0) !=
+            hipSuccess ||
+        blocks < 1)
+        blocks = 1;
+    return blocks * cus;
+}
+
+int fast_batch_grid(int variant, int cus) {
+    switch (variant) {
+        case 0: return fast_batch_grid<24, 128>(cus);
+        case 1: return fast_batch_grid<16, 128>(cus);
+        case 2: return fast_batch_grid<24, 64>(cus);
+        default: return fast_batch_grid<16, 256>(cus);
+    }
+}
+
+hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, int grid, hipStream_t s) {
+    switch (variant) {
+        case 0: k_query_fast_batch<24, 128><<<grid, 128, 0, s>>>(b); break;
+        case 1: k_query_fast_batch<16, 128><<<grid, 128, 0, s>>>(b); break;
+        case 2: k_query_fast_batch<24, 64><<<grid, 64, 0, s>>>(b); break;
+        default: k_query_fast_batch<16, 256><<<grid, 256, 0, s>>>(b); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_query_general_batch(const BatchQuery& b, int blocks, hipStream_t s) {
+    k_query_general_batch<<<blocks, kBlock, 0, s>>>(b);
     return hipGetLastError();
 }
 

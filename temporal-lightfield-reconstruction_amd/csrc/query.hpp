@@ -52,6 +52,8 @@ struct QueryParams {
     unsigned* fail_count;
     // fast kernel over a list: qlist/qcount above when fast_list != 0
     int fast_list;
+    // batched launches: the buffer index of each fail_list entry (else null)
+    unsigned* fail_item;
     // fast kernel, optional: 1 where the output sample has exactly one
     // surface at 1R and no 2R fetch (the gate's training label)
     float* labels;
@@ -59,11 +61,40 @@ struct QueryParams {
     float* features;
 };
 
+// A batch of built buffers queried by one launch of each kernel.
+struct BatchQuery {
+    const QueryParams* items;     // per buffer (device array)
+    int n_items;
+    // fast kernel: logical blocks of 128 queries, buffer b owns
+    // [block_prefix[b], block_prefix[b+1]); taken from an atomic counter
+    const unsigned* block_prefix;
+    unsigned total_blocks;
+    unsigned* block_next;
+    // general kernel: the shared hand-off list and its buffer indices
+    const unsigned* list;
+    const unsigned* list_item;
+    const unsigned* list_count;
+    unsigned* list_next;
+    float* scratch;               // kWaveScratchWords per resident wave
+    unsigned long long* stats;    // aggregate counters (overflow)
+};
+
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
 // more 1R leaves go to the general kernel.
 constexpr int kFastLeafCap = 24;
 static_assert(kFastLeafCap <= 32, "the fast kernel keeps a 32-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
+
+// Batched launches (tlr_reconstruct_batch).  The fast kernel comes in
+// kNumFastVariants (leaf cap, block size) variants, chosen by the autotuner.
+constexpr int kNumFastVariants = 4;
+constexpr int kFastBatchBlockMax = 256;   // largest variant block (logical block size is per variant)
+int fast_variant_cap(int v);
+int fast_variant_block(int v);
+int fast_batch_grid(int variant, int cus);  // persistent grid: resident blocks x CUs
+hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
+hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, int grid, hipStream_t s);
+hipError_t launch_query_general_batch(const BatchQuery& b, int blocks, hipStream_t s);
 
 // Resident waves of the general kernel per CU (its occupancy limit).
 int general_waves_per_cu();

@@ -34,6 +34,26 @@ int bits_for(uint64_t n) {
     return b;
 }
 
+// One buffer of a batch: its device-resident input and, after a build, the
+// products the query kernels read plus its own output buffers.
+struct BatchSlot {
+    tlr_header h{};
+    int N = 0;
+    float* d_in = nullptr;   // 12 fields x N, like tlr_ctx::d_in
+    size_t d_in_cap = 0;
+    DeviceArena prod;
+    Points pts{};
+    LeafRec* leaves = nullptr;
+    Bounds* leaf_b = nullptr;
+    TravNode* trav = nullptr;
+    float* pattern = nullptr;
+    float4* qcol = nullptr;
+    float* img = nullptr;
+    int M = 0;
+    size_t nq = 0;
+    QueryParams q{};
+};
+
 }  // namespace
 
 struct tlr_ctx {
@@ -107,6 +127,14 @@ struct tlr_ctx {
 
     hipEvent_t ev[8];
     bool events = false;
+
+    // batched reconstruction
+    std::vector<BatchSlot*> slots;
+    DeviceArena barena;                  // items, block table, hand-off list, scratch
+    hipGraphExec_t bgraph = nullptr;     // captured query phase of the last batch shape
+    std::vector<uint64_t> bkey;          // that shape
+    int tuned_variant = -1;              // autotuned fast-kernel variant
+    int tuned_general_blocks = 0;        // autotuned general-kernel grid
 
     tlr_status fail(tlr_status s, const char* fmt, ...) {
         char buf[512];
@@ -182,6 +210,13 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->train_idx) (void)hipFree(c->train_idx);
     if (c->events)
         for (auto& e : c->ev) (void)hipEventDestroy(e);
+    if (c->bgraph) (void)hipGraphExecDestroy(c->bgraph);
+    for (BatchSlot* b : c->slots) {
+        if (b->d_in) (void)hipFree(b->d_in);
+        b->prod.release();
+        delete b;
+    }
+    c->barena.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -957,6 +992,319 @@ tlr_status tlr_gate_train(tlr_ctx* c, int epochs, int batch, float lr, float pos
         HIPCHK(c, hipStreamSynchronize(s));
         if (epoch_loss) epoch_loss[e] = l / (float)n;
     }
+    return TLR_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched reconstruction.  No reference counterpart: the reference builds and
+// filters one TreeGather at a time.  Buffers are uploaded once into slots;
+// each call builds every item in turn (the build needs a few host reads of
+// device counts), keeps each item's query products in its slot, then runs the
+// query phase for all items as one captured graph: a counter clear, one
+// persistent launch of the fast kernel over every item's queries, one
+// persistent launch of the general kernel over the shared hand-off list, and
+// the per-item combines.  The graph is replayed while the batch's shape stays
+// the same.  The first call (per context) times the fast-kernel variants and
+// the general-kernel grid on the actual batch and keeps the fastest.
+// ---------------------------------------------------------------------------
+
+tlr_status tlr_batch_set_samples(tlr_ctx* c, int slot, const tlr_header* h, const float* rec, int64_t n) {
+    if (!c) return TLR_E_INVALID;
+    if (slot < 0 || slot > 4096) return c->fail(TLR_E_INVALID, "slot %d out of range", slot);
+    tlr_status st = check_header(c, h);
+    if (st != TLR_OK) return st;
+    if (!rec || n != (int64_t)h->width * h->height * h->spp)
+        return c->fail(TLR_E_INVALID, "record count does not match the header");
+    HIPCHK(c, hipSetDevice(c->device));
+    while ((int)c->slots.size() <= slot) c->slots.push_back(new BatchSlot);
+    BatchSlot* b = c->slots[slot];
+    const int N = (int)n;
+    const size_t need = (size_t)12 * N * sizeof(float);
+    if (need > b->d_in_cap) {
+        if (b->d_in) HIPCHK(c, hipFree(b->d_in));
+        b->d_in = nullptr;
+        b->d_in_cap = 0;
+        HIPCHK(c, hipMalloc(&b->d_in, need));
+        b->d_in_cap = need;
+    }
+    static const int field[12] = {0, 1, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13};  // x,y,w,u,v,t,r,g,b,mv
+    std::vector<float> tmp((size_t)N);
+    for (int f = 0; f < 12; f++) {
+        const int src = field[f];
+        for (int k = 0; k < N; k++) tmp[k] = rec[(size_t)16 * k + src];
+        HIPCHK(c, hipMemcpy(b->d_in + (size_t)f * N, tmp.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    }
+    b->h = *h;
+    b->N = N;
+    return TLR_OK;
+}
+
+// Builds slot b with the context's build pipeline and keeps its products.
+static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it, float* build_ms) {
+    // the build reads the context's input: lend it the slot's
+    std::swap(c->d_in, b->d_in);
+    std::swap(c->d_in_cap, b->d_in_cap);
+    const tlr_header h0 = c->h;
+    const int N0 = c->N;
+    const bool has0 = c->has_samples;
+    c->h = b->h;
+    c->N = b->N;
+    c->has_samples = true;
+    tlr_status st = tlr_build(c, it.aperture_adjust, it.focal_adjust);
+    std::swap(c->d_in, b->d_in);
+    std::swap(c->d_in_cap, b->d_in_cap);
+    c->h = h0;
+    c->N = N0;
+    c->has_samples = has0;
+    c->built = false;  // the context's own input was not what was built
+    if (st != TLR_OK) return st;
+    float ms = 0;
+    if (build_ms && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) *build_ms = ms;
+
+    std::vector<float> pat;
+    const int M = make_output_pattern(it.out_spp, false, nullptr, pat);
+    const int W = b->h.width, H = b->h.height;
+    const size_t nq = (size_t)W * H * M;
+    if (nq >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "too many output samples");
+    const size_t nb = (size_t)c->nb, L = (size_t)c->L, nv = (size_t)c->nv;
+    const size_t nt = nv > 0 ? 2 * nv - 1 : 0;
+    const size_t bytes = sizeof(float4) * (2 * (nb + kPointPad) + nb) + sizeof(LeafRec) * L + sizeof(Bounds) * L +
+                         sizeof(TravNode) * nt + 4 * pat.size() + sizeof(float4) * nq + 16 * (size_t)W * H +
+                         16 * 256;
+    HIPCHK(c, b->prod.reserve(bytes));
+    b->prod.reset();
+    b->pts.pab = b->prod.alloc<float4>(2 * (nb + kPointPad));
+    b->pts.pc = b->prod.alloc<float4>(std::max<size_t>(nb, 1));
+    b->leaves = b->prod.alloc<LeafRec>(std::max<size_t>(L, 1));
+    b->leaf_b = b->prod.alloc<Bounds>(std::max<size_t>(L, 1));
+    b->trav = b->prod.alloc<TravNode>(std::max<size_t>(nt, 1));
+    b->pattern = b->prod.alloc<float>(pat.size());
+    b->qcol = b->prod.alloc<float4>(nq);
+    b->img = b->prod.alloc<float>((size_t)W * H * 4);
+    if (!b->img) return c->fail(TLR_E_HIP, "batch slot arena exhausted");
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemcpyAsync(b->pts.pab, c->pts.pab, sizeof(float4) * 2 * (nb + kPointPad), hipMemcpyDeviceToDevice, s));
+    if (nb) HIPCHK(c, hipMemcpyAsync(b->pts.pc, c->pts.pc, sizeof(float4) * nb, hipMemcpyDeviceToDevice, s));
+    if (L) {
+        HIPCHK(c, hipMemcpyAsync(b->leaves, c->leaves, sizeof(LeafRec) * L, hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(b->leaf_b, c->leaf_b, sizeof(Bounds) * L, hipMemcpyDeviceToDevice, s));
+    }
+    if (nt) HIPCHK(c, hipMemcpyAsync(b->trav, c->trav, sizeof(TravNode) * nt, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(b->pattern, pat.data(), 4 * pat.size(), hipMemcpyHostToDevice, s));
+    b->M = M;
+    b->nq = nq;
+    QueryParams& Q = b->q;
+    memset(&Q, 0, sizeof(Q));
+    Q.W = W; Q.H = H; Q.M = M;
+    Q.spp_pow2 = c->spp_pow2;
+    Q.cc0 = c->cc0; Q.cc1 = c->cc1;
+    Q.n_nodes = (int)nt;
+    Q.root = nv > 1 ? (int)nv / 2 - 1 : 0;
+    Q.num_queries = (unsigned)nq;
+    Q.pattern = b->pattern;
+    Q.trav = b->trav;
+    Q.leaves = b->leaves;
+    Q.leaf_b = b->leaf_b;
+    Q.pts = b->pts;
+    Q.qcol = b->qcol;
+    // the pattern copy reads host memory that goes away with `pat`
+    HIPCHK(c, hipStreamSynchronize(s));
+    return TLR_OK;
+}
+
+// Query phase of a batch with a given fast variant and general grid.
+static tlr_status batch_queries(tlr_ctx* c, const BatchQuery& B, int variant, int fast_grid, int general_blocks,
+                                const std::vector<BatchSlot*>& its, unsigned* counters) {
+    hipStream_t s = c->stream;
+    // counters: block_next, list_count (fail_count), list_next, stats (2 * kNumStats words)
+    HIPCHK(c, launch_fill_u32(counters, 3 + 2 * kNumStats, 0u, s));
+    HIPCHK(c, launch_query_fast_batch(B, variant, fast_grid, s));
+    HIPCHK(c, launch_query_general_batch(B, general_blocks, s));
+    for (BatchSlot* b : its) HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
+    return TLR_OK;
+}
+
+tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n, tlr_batch_stats* bs) {
+    if (!c) return TLR_E_INVALID;
+    if (n <= 0 || !items) return c->fail(TLR_E_INVALID, "empty batch");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    std::vector<BatchSlot*> its((size_t)n);
+    double build_ms = 0;
+    for (int i = 0; i < n; i++) {
+        const tlr_batch_item& it = items[i];
+        if (it.slot < 0 || it.slot >= (int)c->slots.size() || c->slots[it.slot]->N == 0)
+            return c->fail(TLR_E_STATE, "batch item %d: slot %d has no samples", i, it.slot);
+        if (it.out_spp <= 0) return c->fail(TLR_E_INVALID, "batch item %d: out_spp must be positive", i);
+        for (int j = 0; j < i; j++)
+            if (items[j].slot == it.slot) return c->fail(TLR_E_INVALID, "slot %d appears twice in the batch", it.slot);
+        its[i] = c->slots[it.slot];
+        float ms = 0;
+        tlr_status st = batch_build(c, its[i], it, &ms);
+        if (st != TLR_OK) return st;
+        if (it.build_ms) *it.build_ms = ms;
+        build_ms += ms;
+    }
+    // shared buffers
+    std::vector<unsigned> prefix((size_t)n + 1, 0);
+    size_t total_q = 0;
+    if (c->tuned_variant < 0) c->tuned_variant = 0;
+    if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
+    const int cus = c->prop.multiProcessorCount;
+    const int waves = std::max(4, cus * c->general_wpc);
+    for (int i = 0; i < n; i++) total_q += its[i]->nq;
+    if (total_q >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "batch has too many output samples");
+    const size_t bbytes = sizeof(QueryParams) * n + 4 * (size_t)(n + 1) * kNumFastVariants + 8 * total_q +
+                          4 * (3 + 2 * kNumStats) + (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
+    HIPCHK(c, c->barena.reserve(bbytes));
+    c->barena.reset();
+    QueryParams* d_items = c->barena.alloc<QueryParams>(n);
+    unsigned* d_prefix = c->barena.alloc<unsigned>((size_t)(n + 1) * kNumFastVariants);
+    unsigned* d_list = c->barena.alloc<unsigned>(total_q);
+    unsigned* d_item = c->barena.alloc<unsigned>(total_q);
+    unsigned* d_cnt = c->barena.alloc<unsigned>(3 + 2 * kNumStats);
+    float* d_scratch = c->barena.alloc<float>((size_t)waves * kWaveScratchWords);
+    if (!d_scratch) return c->fail(TLR_E_HIP, "batch arena exhausted");
+    std::vector<QueryParams> qp((size_t)n);
+    for (int i = 0; i < n; i++) {
+        qp[i] = its[i]->q;
+        qp[i].fail_list = d_list;
+        qp[i].fail_count = d_cnt + 1;
+        qp[i].fail_item = d_item;
+        qp[i].stats = reinterpret_cast<unsigned long long*>(d_cnt + 3);
+    }
+    // block tables, one per variant (logical blocks depend on the block size)
+    std::vector<unsigned> tables((size_t)(n + 1) * kNumFastVariants);
+    for (int v = 0; v < kNumFastVariants; v++) {
+        const unsigned bs_v = (unsigned)fast_variant_block(v);
+        unsigned acc = 0;
+        for (int i = 0; i < n; i++) {
+            tables[(size_t)v * (n + 1) + i] = acc;
+            acc += (unsigned)((its[i]->nq + bs_v - 1) / bs_v);
+        }
+        tables[(size_t)v * (n + 1) + n] = acc;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_items, qp.data(), sizeof(QueryParams) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(d_prefix, tables.data(), 4 * tables.size(), hipMemcpyHostToDevice, s));
+    auto make_bq = [&](int v) {
+        BatchQuery B;
+        memset(&B, 0, sizeof(B));
+        B.items = d_items;
+        B.n_items = n;
+        B.block_prefix = d_prefix + (size_t)v * (n + 1);
+        B.total_blocks = tables[(size_t)v * (n + 1) + n];
+        B.block_next = d_cnt;
+        B.list = d_list;
+        B.list_item = d_item;
+        B.list_count = d_cnt + 1;
+        B.list_next = d_cnt + 2;
+        B.scratch = d_scratch;
+        B.stats = reinterpret_cast<unsigned long long*>(d_cnt + 3);
+        return B;
+    };
+    const int general_full = waves / 4;  // blocks of four waves
+    // autotune once per context: fast variants at the full general grid, then
+    // the general grid (full or three quarters) with the best variant
+    bool tuned_now = false;
+    if (c->tuned_general_blocks == 0) {
+        float best = 1e30f;
+        int best_v = 0;
+        for (int v = 0; v < kNumFastVariants; v++) {
+            HIPCHK(c, hipEventRecord(c->ev[2], s));
+            tlr_status st = batch_queries(c, make_bq(v), v, fast_batch_grid(v, cus), general_full, its, d_cnt);
+            if (st != TLR_OK) return st;
+            HIPCHK(c, hipEventRecord(c->ev[3], s));
+            HIPCHK(c, hipEventSynchronize(c->ev[3]));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+            if (ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        int best_g = general_full;
+        const int g34 = std::max(1, general_full * 3 / 4);
+        HIPCHK(c, hipEventRecord(c->ev[2], s));
+        tlr_status st = batch_queries(c, make_bq(best_v), best_v, fast_batch_grid(best_v, cus), g34, its, d_cnt);
+        if (st != TLR_OK) return st;
+        HIPCHK(c, hipEventRecord(c->ev[3], s));
+        HIPCHK(c, hipEventSynchronize(c->ev[3]));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        if (ms < best) best_g = g34;
+        c->tuned_variant = best_v;
+        c->tuned_general_blocks = best_g;
+        tuned_now = true;
+    }
+    const int v = c->tuned_variant;
+    const BatchQuery B = make_bq(v);
+    // graph of the query phase, keyed by everything its launches capture
+    std::vector<uint64_t> key = {(uint64_t)n, (uint64_t)v, (uint64_t)c->tuned_general_blocks,
+                                 (uint64_t)(uintptr_t)c->barena.base, (uint64_t)B.total_blocks};
+    for (BatchSlot* b : its) {
+        key.push_back((uint64_t)(uintptr_t)b->qcol);
+        key.push_back((uint64_t)(uintptr_t)b->img);
+        key.push_back(((uint64_t)b->h.width << 40) | ((uint64_t)b->h.height << 20) | (uint64_t)b->M);
+    }
+    bool reused = c->bgraph && key == c->bkey;
+    if (!reused) {
+        if (c->bgraph) HIPCHK(c, hipGraphExecDestroy(c->bgraph));
+        c->bgraph = nullptr;
+        hipGraph_t g = nullptr;
+        const int fgrid = fast_batch_grid(v, cus);
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        tlr_status st = batch_queries(c, B, v, fgrid, c->tuned_general_blocks, its, d_cnt);
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (st != TLR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIPCHK(c, e);
+        e = hipGraphInstantiate(&c->bgraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(c, e);
+        c->bkey = key;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    HIPCHK(c, hipGraphLaunch(c->bgraph, s));
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    for (int i = 0; i < n; i++) {
+        const tlr_batch_item& it = items[i];
+        if (!it.rgba_out) continue;
+        const size_t bytes = (size_t)its[i]->h.width * its[i]->h.height * 16;
+        HIPCHK(c, hipMemcpyAsync(it.rgba_out, its[i]->img, bytes,
+                                 it.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
+    unsigned cnt[3 + 2 * kNumStats];
+    HIPCHK(c, hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    unsigned long long st64[kNumStats];
+    memcpy(st64, cnt + 3, sizeof(st64));
+    if (bs) {
+        memset(bs, 0, sizeof(*bs));
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) bs->query_ms = ms;
+        bs->build_ms = build_ms;
+        bs->total_ms = bs->build_ms + bs->query_ms;
+        bs->general_queries = (double)cnt[1];
+        bs->output_samples = (double)total_q;
+        bs->fast_leaf_cap = fast_variant_cap(v);
+        bs->fast_block = fast_variant_block(v);
+        bs->general_blocks = c->tuned_general_blocks;
+        bs->graph_reused = reused ? 1 : 0;
+        bs->autotuned = tuned_now ? 1 : 0;
+    }
+    if (st64[ST_OVERFLOW])
+        return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",
+                       (unsigned long long)st64[ST_OVERFLOW]);
+    return TLR_OK;
+}
+
+tlr_status tlr_batch_retune(tlr_ctx* c) {
+    if (!c) return TLR_E_INVALID;
+    c->tuned_variant = -1;
+    c->tuned_general_blocks = 0;
     return TLR_OK;
 }
 

@@ -22,7 +22,7 @@ def test_tlr_exports_every_declared_symbol(tlr):
     assert len(names) >= 15
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    assert set(names) == set(tlr.EXPORTED) | (set(names) - set(tlr.EXPORTED))
+    assert set(names) == set(tlr.EXPORTED), set(names) ^ set(tlr.EXPORTED)
 
 
 def test_scene_exports(tlr):

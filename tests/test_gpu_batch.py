@@ -1,0 +1,80 @@
+"""GPU: batched reconstruction (tlr_reconstruct_batch).  Every image of a
+batch of uneven buffers equals the one the single-buffer path makes from the
+same buffer, bit for bit (same kernels, same per-query arithmetic); the
+second call of the same batch replays the captured graph; one buffer is also
+checked against the CPU oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BUFS = [
+    # (preset, W, H, spp, out_spp)
+    (1, 64, 48, 4, 8),
+    (2, 96, 72, 16, 16),
+    (3, 80, 64, 8, 4),
+    (2, 40, 30, 1, 8),
+    (2, 128, 40, 32, 2),
+]
+
+
+@pytest.fixture(scope="module")
+def batch(tlr):
+    print("batch: uploading %d buffers" % len(BUFS), flush=True)
+    sbufs = [tlr.make_scene(p, W, H, spp, seed=500 + k) for k, (p, W, H, spp, m) in enumerate(BUFS)]
+    ctx = tlr.Ctx(0)
+    for k, sb in enumerate(sbufs):
+        ctx.batch_set_samples(k, sb)
+    jobs = [(k, BUFS[k][4]) for k in range(len(BUFS))]
+    imgs1, bms1, st1 = ctx.reconstruct_batch(jobs)
+    imgs2, bms2, st2 = ctx.reconstruct_batch(jobs)
+    return sbufs, ctx, jobs, (imgs1, st1), (imgs2, st2)
+
+
+def test_batch_equals_single_buffer_path(tlr, batch):
+    sbufs, ctx, jobs, (imgs, st), _ = batch
+    print("batch stats", st, flush=True)
+    assert st["autotuned"] == 1
+    assert st["output_samples"] == sum(W * H * m for (_, W, H, _, m) in BUFS)
+    single = tlr.Ctx(0)
+    for k, sb in enumerate(sbufs):
+        single.set_samples(sb)
+        single.build()
+        ref, _ = single.reconstruct(out_spp=BUFS[k][4])
+        assert np.array_equal(imgs[k], ref), "buffer %d differs" % k
+
+
+def test_batch_graph_replay(batch):
+    sbufs, ctx, jobs, (imgs1, st1), (imgs2, st2) = batch
+    assert st2["graph_reused"] == 1 and st2["autotuned"] == 0
+    for a, b in zip(imgs1, imgs2):
+        assert np.array_equal(a, b)
+
+
+def test_batch_against_oracle(tlr, orc, batch):
+    sbufs, ctx, jobs, (imgs, st), _ = batch
+    k = 1
+    sb = sbufs[k]
+    o = orc.Oracle(sb.width, sb.height, sb.spp, sb.coc[0], sb.coc[1], sb.records)
+    o.build()
+    ref, _, _ = o.reconstruct(out_spp=BUFS[k][4])
+    err = np.abs(imgs[k] - ref)
+    assert err.max() <= 2e-3 and err.mean() <= 1e-4
+
+
+def test_batch_shape_change_recaptures(tlr, batch):
+    sbufs, ctx, jobs, _, _ = batch
+    sub = [jobs[2], jobs[0]]
+    imgs, _, st = ctx.reconstruct_batch(sub)
+    assert st["graph_reused"] == 0
+    single = tlr.Ctx(0)
+    single.set_samples(sbufs[2])
+    single.build()
+    ref, _ = single.reconstruct(out_spp=BUFS[2][4])
+    assert np.array_equal(imgs[0], ref)
+
+
+def test_batch_errors(tlr):
+    ctx = tlr.Ctx(0)
+    with pytest.raises(tlr.TlrError):
+        ctx.reconstruct_batch([(3, 8)])  # empty slot
