@@ -160,6 +160,9 @@ typedef struct tlr_batch_stats {
 tlr_status  tlr_batch_set_samples(tlr_ctx* ctx, int slot, const tlr_header* h, const float* records, int64_t n);
 /* Builds and reconstructs every item; images go to rgba_out. */
 tlr_status  tlr_reconstruct_batch(tlr_ctx* ctx, const tlr_batch_item* items, int n, tlr_batch_stats* stats);
+/* Makes slot `slot` the context's input for tlr_build / tlr_reconstruct (a
+ * device-to-device copy), e.g. to run a batch's buffers one at a time. */
+tlr_status  tlr_use_slot(tlr_ctx* ctx, int slot);
 /* Forgets the tuned launch shape; the next batch tunes again. */
 tlr_status  tlr_batch_retune(tlr_ctx* ctx);
 

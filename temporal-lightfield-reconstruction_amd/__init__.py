@@ -33,7 +33,7 @@ EXPORTED = [
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
-    "tlr_batch_set_samples", "tlr_reconstruct_batch", "tlr_batch_retune",
+    "tlr_batch_set_samples", "tlr_reconstruct_batch", "tlr_batch_retune", "tlr_use_slot",
 ]
 GATE_FEATURES = 24
 GATE_PARAMS = 5825
@@ -148,6 +148,7 @@ def lib():
         L.tlr_batch_set_samples.argtypes = [V, C.c_int, C.POINTER(Header), V, C.c_int64]
         L.tlr_reconstruct_batch.argtypes = [V, C.POINTER(BatchItem), C.c_int, C.POINTER(BatchStats)]
         L.tlr_batch_retune.argtypes = [V]
+        L.tlr_use_slot.argtypes = [V, C.c_int]
         L.tlr_copy_query_colors.argtypes = [V, V]
         L.tlr_output_pattern.restype = C.c_int
         L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
@@ -335,6 +336,17 @@ class Ctx:
         st = BatchStats()
         self._chk(lib().tlr_reconstruct_batch(self.h, items, n, C.byref(st)))
         return imgs, [bms[k] for k in range(n)], st.as_dict()
+
+    def use_slot(self, slot):
+        """The context's input becomes batch slot `slot` (for build/reconstruct)."""
+        self._chk(lib().tlr_use_slot(self.h, slot))
+        h, w = self._slot_shapes[slot]
+
+        class _Shape:
+            pass
+        sb = _Shape()
+        sb.height, sb.width = h, w
+        self.sbuf = sb
 
     def batch_retune(self):
         self._chk(lib().tlr_batch_retune(self.h))

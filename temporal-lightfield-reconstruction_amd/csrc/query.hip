@@ -953,29 +953,22 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     fast_query<STATS, kFastLeafCap, kFastBlock>(Q, blockIdx.x * kFastBlock + threadIdx.x, 0u, lid);
 }
 
-// Batched fast path: one persistent launch over every buffer of a batch.
-// Blocks take logical blocks of BLOCK queries from an atomic counter (buffers
-// in order, each buffer's queries in scan order) until all are done.
+// Batched fast path: one launch over every buffer of a batch.  Block b of the
+// grid is logical block b of the concatenated buffers (each buffer's queries
+// in scan order, padded to whole blocks); its buffer is found by binary
+// search over the buffers' first blocks.
 template <int CAP, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
     __shared__ int lid[CAP][BLOCK];
-    __shared__ unsigned next;
-    for (;;) {
-        __syncthreads();  // every lane is done with the previous block
-        if (threadIdx.x == 0) next = atomicAdd(B.block_next, 1u);
-        __syncthreads();
-        const unsigned b = (unsigned)uni((int)next);  // wave-uniform: scalar loads below
-        if (b >= B.total_blocks) break;
-        // buffer: the last one whose first block is <= b
-        int lo = 0, hi = B.n_items - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (B.block_prefix[mid] <= b) lo = mid;
-            else hi = mid - 1;
-        }
-        const QueryParams Q = B.items[lo];
-        fast_query<false, CAP, BLOCK>(Q, (b - B.block_prefix[lo]) * BLOCK + threadIdx.x, (unsigned)lo, lid);
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = B.n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.block_prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
     }
+    const QueryParams Q = B.items[lo];
+    fast_query<false, CAP, BLOCK>(Q, (b - B.block_prefix[lo]) * BLOCK + threadIdx.x, (unsigned)lo, lid);
 }
 
 // ---------------------------------------------------------------------------
@@ -1244,26 +1237,9 @@ static const int kFastVariants[kNumFastVariants][2] = {{24, 128}, {16, 128}, {24
 int fast_variant_cap(int v) { return kFastVariants[v][0]; }
 int fast_variant_block(int v) { return kFastVariants[v][1]; }
 
-template <int CAP, int BLOCK>
-static int fast_batch_grid(int cus) {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_fast_batch<CAP, BLOCK>, BLOCK, 0) !=
-            hipSuccess ||
-        blocks < 1)
-        blocks = 1;
-    return blocks * cus;
-}
-
-int fast_batch_grid(int variant, int cus) {
-    switch (variant) {
-        case 0: return fast_batch_grid<24, 128>(cus);
-        case 1: return fast_batch_grid<16, 128>(cus);
-        case 2: return fast_batch_grid<24, 64>(cus);
-        default: return fast_batch_grid<16, 256>(cus);
-    }
-}
-
-hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, int grid, hipStream_t s) {
+hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, hipStream_t s) {
+    const unsigned grid = b.total_blocks;
+    if (grid == 0) return hipSuccess;
     switch (variant) {
         case 0: k_query_fast_batch<24, 128><<<grid, 128, 0, s>>>(b); break;
         case 1: k_query_fast_batch<16, 128><<<grid, 128, 0, s>>>(b); break;

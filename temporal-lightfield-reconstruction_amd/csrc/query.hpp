@@ -65,11 +65,11 @@ struct QueryParams {
 struct BatchQuery {
     const QueryParams* items;     // per buffer (device array)
     int n_items;
-    // fast kernel: logical blocks of 128 queries, buffer b owns
-    // [block_prefix[b], block_prefix[b+1]); taken from an atomic counter
+    // fast kernel: blocks of the variant's size; buffer b owns blocks
+    // [block_prefix[b], block_prefix[b+1])
     const unsigned* block_prefix;
     unsigned total_blocks;
-    unsigned* block_next;
+    unsigned* block_next;         // unused by the current kernels (kept zeroed)
     // general kernel: the shared hand-off list and its buffer indices
     const unsigned* list;
     const unsigned* list_item;
@@ -91,9 +91,8 @@ constexpr int kNumFastVariants = 4;
 constexpr int kFastBatchBlockMax = 256;   // largest variant block (logical block size is per variant)
 int fast_variant_cap(int v);
 int fast_variant_block(int v);
-int fast_batch_grid(int variant, int cus);  // persistent grid: resident blocks x CUs
 hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
-hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, int grid, hipStream_t s);
+hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, hipStream_t s);
 hipError_t launch_query_general_batch(const BatchQuery& b, int blocks, hipStream_t s);
 
 // Resident waves of the general kernel per CU (its occupancy limit).

@@ -1114,12 +1114,12 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
 }
 
 // Query phase of a batch with a given fast variant and general grid.
-static tlr_status batch_queries(tlr_ctx* c, const BatchQuery& B, int variant, int fast_grid, int general_blocks,
+static tlr_status batch_queries(tlr_ctx* c, const BatchQuery& B, int variant, int general_blocks,
                                 const std::vector<BatchSlot*>& its, unsigned* counters) {
     hipStream_t s = c->stream;
     // counters: block_next, list_count (fail_count), list_next, stats (2 * kNumStats words)
     HIPCHK(c, launch_fill_u32(counters, 3 + 2 * kNumStats, 0u, s));
-    HIPCHK(c, launch_query_fast_batch(B, variant, fast_grid, s));
+    HIPCHK(c, launch_query_fast_batch(B, variant, s));
     HIPCHK(c, launch_query_general_batch(B, general_blocks, s));
     for (BatchSlot* b : its) HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
     return TLR_OK;
@@ -1212,7 +1212,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         int best_v = 0;
         for (int v = 0; v < kNumFastVariants; v++) {
             HIPCHK(c, hipEventRecord(c->ev[2], s));
-            tlr_status st = batch_queries(c, make_bq(v), v, fast_batch_grid(v, cus), general_full, its, d_cnt);
+            tlr_status st = batch_queries(c, make_bq(v), v, general_full, its, d_cnt);
             if (st != TLR_OK) return st;
             HIPCHK(c, hipEventRecord(c->ev[3], s));
             HIPCHK(c, hipEventSynchronize(c->ev[3]));
@@ -1226,7 +1226,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         int best_g = general_full;
         const int g34 = std::max(1, general_full * 3 / 4);
         HIPCHK(c, hipEventRecord(c->ev[2], s));
-        tlr_status st = batch_queries(c, make_bq(best_v), best_v, fast_batch_grid(best_v, cus), g34, its, d_cnt);
+        tlr_status st = batch_queries(c, make_bq(best_v), best_v, g34, its, d_cnt);
         if (st != TLR_OK) return st;
         HIPCHK(c, hipEventRecord(c->ev[3], s));
         HIPCHK(c, hipEventSynchronize(c->ev[3]));
@@ -1252,9 +1252,8 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         if (c->bgraph) HIPCHK(c, hipGraphExecDestroy(c->bgraph));
         c->bgraph = nullptr;
         hipGraph_t g = nullptr;
-        const int fgrid = fast_batch_grid(v, cus);
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        tlr_status st = batch_queries(c, B, v, fgrid, c->tuned_general_blocks, its, d_cnt);
+        tlr_status st = batch_queries(c, B, v, c->tuned_general_blocks, its, d_cnt);
         hipError_t e = hipStreamEndCapture(s, &g);
         if (st != TLR_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -1298,6 +1297,24 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     if (st64[ST_OVERFLOW])
         return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",
                        (unsigned long long)st64[ST_OVERFLOW]);
+    return TLR_OK;
+}
+
+tlr_status tlr_use_slot(tlr_ctx* c, int slot) {
+    if (!c) return TLR_E_INVALID;
+    if (slot < 0 || slot >= (int)c->slots.size() || c->slots[slot]->N == 0)
+        return c->fail(TLR_E_STATE, "slot %d has no samples", slot);
+    HIPCHK(c, hipSetDevice(c->device));
+    BatchSlot* b = c->slots[slot];
+    tlr_status st = reserve_input(c, b->N);
+    if (st != TLR_OK) return st;
+    HIPCHK(c, hipMemcpyAsync(c->d_in, b->d_in, sizeof(float) * 12 * (size_t)b->N, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h = b->h;
+    c->N = b->N;
+    c->has_samples = true;
+    c->built = false;
     return TLR_OK;
 }
 
