@@ -160,16 +160,91 @@ __device__ void forward(GateLds& L, int lane) {
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_gate_forward(const float* __restrict__ X, int n,
-                                                     const float* __restrict__ P, float* __restrict__ logits) {
-    __shared__ GateLds L;
-    const int lane = threadIdx.x;
-    load_weights(L, P, lane);
-    for (int base = blockIdx.x * kRows; base < n; base += gridDim.x * kRows) {
-        load_rows(L, X, nullptr, base, n, lane);
-        forward(L, lane);
-        if (lane < kRows && base + lane < n) logits[base + lane] = L.z[lane];
-        __syncthreads();
+// Inference: the weights live in registers as MFMA B fragments (loaded once
+// per wave), each wave takes 16 feature rows at a time, and only the layer-1
+// activations pass through LDS (to become layer 2's A operand).  Layer 3 is
+// reduced from layer 2's accumulators across the 16 lanes of each row group.
+constexpr int kFwdBlock = 256;
+constexpr int kFwdWaves = kFwdBlock / 64;
+
+__global__ __launch_bounds__(kFwdBlock) void k_gate_forward(const float* __restrict__ X, int n,
+                                                            const float* __restrict__ P, float* __restrict__ logits) {
+    __shared__ float H1s[kFwdWaves][16 * kGateHid];
+    const int lane = threadIdx.x & 63;
+    float* H1 = H1s[threadIdx.x >> 6];
+    const int col = lane & 15, kb = (lane >> 4) * 8;
+    bf16x8 w1f[4], w2f[2][4];
+    float b1c[4], b2c[4], w3c[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        const int c = nt * 16 + col;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = kb + j;
+            w1f[nt][j] = to_bf16(k < kGateIn ? P[kGateOffW1 + k * kGateHid + c] : 0.0f);
+            w2f[0][nt][j] = to_bf16(P[kGateOffW2 + k * kGateHid + c]);
+            w2f[1][nt][j] = to_bf16(P[kGateOffW2 + (32 + k) * kGateHid + c]);
+        }
+        b1c[nt] = P[kGateOffB1 + c];
+        b2c[nt] = P[kGateOffB2 + c];
+        w3c[nt] = bf16_round(P[kGateOffW3 + c]);
+    }
+    const float b3 = P[kGateOffB3];
+    const int gw = (gridDim.x * kFwdBlock) >> 6;
+    for (int base = ((blockIdx.x * kFwdBlock + threadIdx.x) >> 6) * 16; base < n; base += gw * 16) {
+        // A fragment of X: row base + col, k = kb .. kb+7 (k >= 24 is zero)
+        bf16x8 a;
+        const int row = base + col;
+        if (row < n && kb < kGateIn) {
+            const float4* src = reinterpret_cast<const float4*>(X + (size_t)row * kGateIn + kb);
+            const float4 x0 = src[0], x1 = src[1];
+            a[0] = to_bf16(x0.x); a[1] = to_bf16(x0.y); a[2] = to_bf16(x0.z); a[3] = to_bf16(x0.w);
+            a[4] = to_bf16(x1.x); a[5] = to_bf16(x1.y); a[6] = to_bf16(x1.z); a[7] = to_bf16(x1.w);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[j] = to_bf16(0.0f);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(a, w1f[nt], acc);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float v = acc[j] + b1c[nt];
+                H1[(4 * (lane >> 4) + j) * kGateHid + nt * 16 + col] = v > 0.0f ? v : 0.0f;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bf16x8 a0 = frag_a(H1, kGateHid, lane);
+        const bf16x8 a1 = frag_a(H1 + 32, kGateHid, lane);
+        float zp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(a0, w2f[0][nt], acc);
+            acc = mfma(a1, w2f[1][nt], acc);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float v = acc[j] + b2c[nt];
+                zp[j] += (v > 0.0f ? v : 0.0f) * w3c[nt];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) zp[j] += __shfl_xor(zp[j], d, 64);
+        }
+        if (col == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = base + 4 * (lane >> 4) + j;
+                if (r < n) logits[r] = zp[j] + b3;
+            }
+        }
+        // H1 is rewritten by the next tile
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -313,9 +388,9 @@ __global__ __launch_bounds__(256) void k_gate_adam(float* __restrict__ P, const 
 
 hipError_t launch_gate_forward(const float* X, int n, const float* params, float* logits, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    int blocks = (n + kRows - 1) / kRows;
-    if (blocks > 8192) blocks = 8192;
-    k_gate_forward<<<blocks, 64, 0, s>>>(X, n, params, logits);
+    int blocks = (n + 16 * kFwdWaves - 1) / (16 * kFwdWaves);
+    if (blocks > 4096) blocks = 4096;
+    k_gate_forward<<<blocks, kFwdBlock, 0, s>>>(X, n, params, logits);
     return hipGetLastError();
 }
 

@@ -773,7 +773,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             //     (ReconstructionOur.cpp:466-484);
             //  3. surface 0's samples in order: quadrant mask, nearest sample
             //     and tent-weighted sum (:486-499, :268-297).
-            uint32_t kmask = 0;
+            uint64_t kmask = 0;
             for (int li = 0; li < n; li++) {
                 const int id = lid[li][tid];
                 const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
@@ -795,7 +795,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                         reproject_point(ga[e], gb[e], o, Q.cc0, Q.cc1, x, y);
                         const float dx = x - o.x, dy = y - o.y;
                         if (!(dx * dx + dy * dy <= R2)) continue;
-                        kmask |= 1u << li;
+                        kmask |= 1ull << li;
                         s_kept++;
                         if (!STATS) {
                             done_leaf = true;
@@ -806,8 +806,8 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             }
             // 2. surfaces; surface 0 = kept leaves in [first0, end0)
             int nsurf = 0, sstart = 0, first0 = n, end0 = n;
-            for (uint32_t km = kmask; km; km &= km - 1) {
-                const int li = __builtin_ctz(km);
+            for (uint64_t km = kmask; km; km &= km - 1) {
+                const int li = __builtin_ctzll(km);
                 if (nsurf == 0) {
                     nsurf = 1;
                     sstart = li;
@@ -836,9 +836,9 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             float ar = 0, ag = 0, ab = 0, aw = 0;
             float min0 = kFltMax;
             float4 near = make_float4(0, 0, 0, 0);
-            for (uint32_t km = nsurf > 0 ? (kmask & ((end0 < 32 ? (1u << end0) : 0u) - 1u)) : 0u; km;
+            for (uint64_t km = nsurf > 0 ? (kmask & ((end0 < 64 ? (1ull << end0) : 0ull) - 1ull)) : 0ull; km;
                  km &= km - 1) {
-                const int li = __builtin_ctz(km);
+                const int li = __builtin_ctzll(km);
                 const int id = lid[li][tid];
                 const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
                 for (int kg = k0; kg < k1; kg += 4) {
@@ -906,14 +906,15 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
         }
     }
     if (Q.labels && active) Q.labels[q] = (!fail && single) ? 1.0f : 0.0f;
-    // hand failed queries to the general kernel (one atomic per wave)
-    const uint64_t fm = __ballot(active && fail);
+    // failures go to the general kernel (one atomic per wave)
+    const bool hand = active && fail;
+    const uint64_t fm = __ballot(hand);
     if (fm) {
         const int lane = tid & 63;
         unsigned base = 0;
         if (lane == 0) base = atomicAdd(Q.fail_count, (unsigned)__popcll(fm));
         base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-        if (active && fail) {
+        if (hand) {
             Q.fail_list[base + lanes_below(fm)] = q | (needs2r ? kNeeds2R : 0u);
             if (Q.fail_item) Q.fail_item[base + lanes_below(fm)] = item;
         }
@@ -976,7 +977,8 @@ __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
 // Leaves are visited in traversal order and every sample within R joins one
 // tent sum: exactly the plain result when the output sample sees a single
 // surface (ReconstructionOur.cpp:99,204,268-297), up to summation order.
-// Features (kGateIn floats, roughly unit scale):
+// Features (kGateIn floats; ratios 3,4,6-9,15 pass through sign(x)log(1+|x|),
+// all are clamped to [-16, 16]):
 //   0 log1p(kept)  1 log1p(leaves)  2 log1p(samples in leaves)
 //   3 depth range / min depth  4 depth std / mean  5 log(mean depth)
 //   6 CoC range  7 mean |CoC| / 16  8,9 screen motion range x,y / 16
@@ -990,12 +992,17 @@ __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
 constexpr int kDirectBlock = 256;
 
 __global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
-    const unsigned q = blockIdx.x * kDirectBlock + threadIdx.x;
-    if (q >= Q.num_queries) return;
+    // leaves found by the walk, in traversal order (scanned after the walk,
+    // so lanes in different phases do not serialise each other)
+    __shared__ int dl[kFastLeafCap][kDirectBlock];
+    const int tid = threadIdx.x;
+    const unsigned q = blockIdx.x * kDirectBlock + tid;
+    const bool active = q < Q.num_queries;
     const int M = Q.M;
-    const unsigned pix = q / (unsigned)M;
+    const unsigned pix = active ? q / (unsigned)M : 0u;
     const int i = (int)(q - pix * (unsigned)M);
     const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    if (!active) return;
     const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
     const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
     OSample o;
@@ -1019,83 +1026,113 @@ __global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
     // depth separation relative to the first leaf / first kept sample met
     float key_first = 0.0f, w_first = 0.0f;
     int far_leaves = 0, off_depth = 0;
+    // The walk fills the list; when it is full (or the walk ends) the listed
+    // leaves are scanned and the walk resumes where it stopped.
+    const float4* __restrict__ pab = Q.pts.pab;
     int node = Q.n_nodes > 0 ? Q.root : -1;
-    while (node >= 0) {
-        const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
-        const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
-        const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-        const int child = __float_as_int(d.x), escape = __float_as_int(d.y);
-        if (node_hit(p, o, R)) {
-            if (child >= 0) {
-                node = child;
-                continue;
+    while (true) {
+        int nl = 0;
+        while (node >= 0) {
+            const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
+            const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
+            const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+            const int child = __float_as_int(d.x), escape = __float_as_int(d.y);
+            if (node_hit(p, o, R)) {
+                if (child >= 0) {
+                    node = child;
+                    continue;
+                }
+                dl[nl][tid] = -child - 1;
+                nl++;
+                if (nl == kFastLeafCap) {
+                    node = escape;
+                    break;
+                }
             }
-            const LeafRec lr = Q.leaves[-child - 1];
-            const float key = lr.w0 + (o.t - lr.t0) * lr.mvw0;
-            if (nleaf == 0) key_first = key;
-            far_leaves += (nleaf > 0 && (key > 1.05f * key_first || key_first > 1.05f * key)) ? 1 : 0;
-            nleaf++;
-            inleaf += lr.s1 - lr.s0;
+            node = escape;
+        }
+        for (int lj = 0; lj < nl; lj++) {
+            const int id = dl[lj][tid];
+            const int li = nleaf + lj;
+            const float4 r0 = *reinterpret_cast<const float4*>(Q.leaves + id);  // w0, t0, mvw0, s0
+            const int k0 = __float_as_int(r0.w), k1 = Q.leaves[id].s1;
+            const float key = r0.x + (o.t - r0.y) * r0.z;
+            if (li == 0) key_first = key;
+            far_leaves += (li > 0 && (key > 1.05f * key_first || key_first > 1.05f * key)) ? 1 : 0;
+            inleaf += k1 - k0;
             kmin = fminf(kmin, key);
             kmax = fmaxf(kmax, key);
-            for (int k = lr.s0; k < lr.s1; k++) {
-                const float4 sa = pt_a(Q.pts, k), sb = pt_b(Q.pts, k);
-                float x, y, w;
-                if (sb.x != 0.0f || sb.y != 0.0f || sb.z != 0.0f) {
-                    const float dt = o.t - sa.w;
-                    const float Px = sa.x * sa.z + dt * sb.x;
-                    const float Py = sa.y * sa.z + dt * sb.y;
-                    const float Pw = 1.0f * sa.z + dt * sb.z;
-                    const float cr = rcp(Pw);
-                    const float coc = Q.cc0 / Pw + Q.cc1;
-                    x = Px * cr + coc * o.u;
-                    y = Py * cr + coc * o.v;
-                    w = Pw;
-                } else {
-                    const float coc = Q.cc0 / sa.z + Q.cc1;
-                    x = sa.x + coc * o.u;
-                    y = sa.y + coc * o.v;
-                    w = sa.z;
+            for (int kg = k0; kg < k1; kg += 4) {
+                const float4* g = pab + 2 * (size_t)kg;
+                float4 ga[4], gb[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    ga[e] = g[2 * e];
+                    gb[e] = g[2 * e + 1];
                 }
-                const float dx = x - o.x, dy = y - o.y;
-                const float d2 = dx * dx + dy * dy;
-                if (!(d2 <= R2)) continue;
-                const float dist = sqrtf(d2);
-                if (kept == 0) w_first = w;
-                off_depth += (w > 1.05f * w_first || w_first > 1.05f * w) ? 1 : 0;
-                kept++;
-                dsum += dist;
-                const float4 col = Q.pts.pc[k];
-                if (dist < dmin) {
-                    dmin = dist;
-                    near = col;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int k = kg + e;
+                    if (k >= k1) break;
+                    const float4 sa = ga[e], sb = gb[e];
+                    if (surely_outside(sa, sb, o, Q.cc0, Q.cc1, R)) continue;
+                    float x, y, w, coc, rw = 0.0f;
+                    if (sb.x != 0.0f || sb.y != 0.0f || sb.z != 0.0f) {
+                        const float dt = o.t - sa.w;
+                        const float Px = sa.x * sa.z + dt * sb.x;
+                        const float Py = sa.y * sa.z + dt * sb.y;
+                        const float Pw = 1.0f * sa.z + dt * sb.z;
+                        rw = rcp(Pw);
+                        coc = Q.cc0 / Pw + Q.cc1;
+                        x = Px * rw + coc * o.u;
+                        y = Py * rw + coc * o.v;
+                        w = Pw;
+                    } else {
+                        coc = Q.cc0 / sa.z + Q.cc1;
+                        x = sa.x + coc * o.u;
+                        y = sa.y + coc * o.v;
+                        w = sa.z;
+                    }
+                    const float dx = x - o.x, dy = y - o.y;
+                    const float d2 = dx * dx + dy * dy;
+                    if (!(d2 <= R2)) continue;
+                    const float dist = sqrtf(d2);
+                    if (kept == 0) w_first = w;
+                    off_depth += (w > 1.05f * w_first || w_first > 1.05f * w) ? 1 : 0;
+                    kept++;
+                    dsum += dist;
+                    const float4 col = Q.pts.pc[k];
+                    if (dist < dmin) {
+                        dmin = dist;
+                        near = col;
+                    }
+                    const float wgt = 1 - dist / disp;
+                    if (wgt > 0) {
+                        ar += wgt * col.x;
+                        ag += wgt * col.y;
+                        ab += wgt * col.z;
+                        aw += wgt * 1.0f;
+                    }
+                    wmin = fminf(wmin, w);
+                    wmax = fmaxf(wmax, w);
+                    wsum += w;
+                    wsq += w * w;
+                    cmin = fminf(cmin, coc);
+                    cmax = fmaxf(cmax, coc);
+                    cabs += fabsf(coc);
+                    // screen motion (zero for static samples)
+                    mxmin = fminf(mxmin, sb.x * rw);
+                    mxmax = fmaxf(mxmax, sb.x * rw);
+                    mymin = fminf(mymin, sb.y * rw);
+                    mymax = fmaxf(mymax, sb.y * rw);
+                    tmin = fminf(tmin, sa.w);
+                    tmax = fmaxf(tmax, sa.w);
+                    quad[(dx >= 0 ? 1 : 0) | (dy >= 0 ? 2 : 0)]++;
                 }
-                const float wgt = 1 - dist / disp;
-                if (wgt > 0) {
-                    ar += wgt * col.x;
-                    ag += wgt * col.y;
-                    ab += wgt * col.z;
-                    aw += wgt * 1.0f;
-                }
-                wmin = fminf(wmin, w);
-                wmax = fmaxf(wmax, w);
-                wsum += w;
-                wsq += w * w;
-                const float coc = Q.cc0 / w + Q.cc1;
-                cmin = fminf(cmin, coc);
-                cmax = fmaxf(cmax, coc);
-                cabs += fabsf(coc);
-                const float rw = rcp(w);
-                mxmin = fminf(mxmin, sb.x * rw);
-                mxmax = fmaxf(mxmax, sb.x * rw);
-                mymin = fminf(mymin, sb.y * rw);
-                mymax = fmaxf(mymax, sb.y * rw);
-                tmin = fminf(tmin, sa.w);
-                tmax = fmaxf(tmax, sa.w);
-                quad[(dx >= 0 ? 1 : 0) | (dy >= 0 ? 2 : 0)]++;
             }
         }
-        node = escape;
+        nleaf += nl;
+        if (node < 0) break;
     }
     // colour: tent result, or the nearest sample when all weights vanish
     const float oow = rcp(aw);
@@ -1130,6 +1167,15 @@ __global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
         f[21] = kept ? dsum / kf / R : 1.0f;
         f[22] = kept ? aw / kf : 0.0f;
         f[23] = kept ? (float)off_depth / kf : 0.0f;
+        // ratios can be unbounded (a leaf key or depth near zero): signed log,
+        // then everything clamped to [-16, 16]
+#pragma unroll
+        for (int k = 0; k < kGateFeatures; k++) {
+            const bool ratio = k == 3 || k == 4 || k == 6 || k == 7 || k == 8 || k == 9 || k == 15;
+            float v = f[k];
+            if (ratio) v = copysignf(log1pf(fabsf(v)), v);
+            f[k] = fminf(fmaxf(v, -16.0f), 16.0f);
+        }
         float* out = Q.features + (size_t)q * kGateFeatures;
 #pragma unroll
         for (int k = 0; k < kGateFeatures; k++) out[k] = f[k];
@@ -1141,8 +1187,13 @@ __global__ __launch_bounds__(256) void k_gate_route(const float* __restrict__ lo
                                                     unsigned* __restrict__ list, unsigned* __restrict__ count) {
     const unsigned q = blockIdx.x * 256 + threadIdx.x;
     const bool act = q < n;
-    // feature 0 is log1p(kept): no gathered sample -> plain path
-    const bool plain = act && (logits[q] < thr || features[(size_t)q * kGateFeatures] == 0.0f);
+    // feature 0 is log1p(kept): no gathered sample -> plain path; so does a
+    // NaN logit or feature
+    bool plain = false;
+    if (act) {
+        const float z = logits[q], f0 = features[(size_t)q * kGateFeatures];
+        plain = !(z >= thr && f0 > 0.0f);
+    }
     const uint64_t m = __ballot(plain);
     if (!m) return;
     unsigned base = 0;
@@ -1174,6 +1225,15 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ q
 __global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// dst[r] = src[r * stride], rows of `width` 32-bit words.
+__global__ void k_take_rows(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t rows, int width,
+                            int stride) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= rows * (size_t)width) return;
+    const size_t r = i / (size_t)width, c = i - r * (size_t)width;
+    dst[i] = src[r * (size_t)stride * width + c];
 }
 
 }  // namespace
@@ -1251,6 +1311,14 @@ hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, hipStream_t
 
 hipError_t launch_query_general_batch(const BatchQuery& b, int blocks, hipStream_t s) {
     k_query_general_batch<<<blocks, kBlock, 0, s>>>(b);
+    return hipGetLastError();
+}
+
+hipError_t launch_take_rows(const void* src, void* dst, size_t rows, int width_words, int stride, hipStream_t s) {
+    const size_t n = rows * (size_t)width_words;
+    if (n == 0) return hipSuccess;
+    k_take_rows<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(static_cast<const uint32_t*>(src),
+                                                          static_cast<uint32_t*>(dst), rows, width_words, stride);
     return hipGetLastError();
 }
 

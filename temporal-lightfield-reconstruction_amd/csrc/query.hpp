@@ -82,7 +82,7 @@ struct BatchQuery {
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
 // more 1R leaves go to the general kernel.
 constexpr int kFastLeafCap = 24;
-static_assert(kFastLeafCap <= 32, "the fast kernel keeps a 32-bit mask over its leaves");
+static_assert(kFastLeafCap <= 64, "the fast kernel keeps a 64-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
 
 // Batched launches (tlr_reconstruct_batch).  The fast kernel comes in
@@ -92,6 +92,8 @@ constexpr int kFastBatchBlockMax = 256;   // largest variant block (logical bloc
 int fast_variant_cap(int v);
 int fast_variant_block(int v);
 hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
+// dst row r = src row r*stride (rows of width_words 32-bit words), on the device.
+hipError_t launch_take_rows(const void* src, void* dst, size_t rows, int width_words, int stride, hipStream_t s);
 hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, hipStream_t s);
 hipError_t launch_query_general_batch(const BatchQuery& b, int blocks, hipStream_t s);
 

@@ -545,6 +545,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
     HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
     HIPCHK(c, hipMemsetAsync(c->d_route_count, 0, sizeof(unsigned), s));
+
     memset(&Q, 0, sizeof(Q));
     Q.W = W; Q.H = H; Q.M = M;
     Q.spp_pow2 = c->spp_pow2;
@@ -905,15 +906,21 @@ tlr_status tlr_gate_collect(tlr_ctx* c, const tlr_recon_params* p, int stride, f
     hipStream_t s = c->stream;
     const size_t rows = (c->qcol_n + stride - 1) / stride;
     hipError_t e = hipSuccess;
-    if (X && e == hipSuccess)
-        e = hipMemcpy2DAsync(X, kGateFeatures * 4, c->features, (size_t)stride * kGateFeatures * 4, kGateFeatures * 4,
-                             rows, hipMemcpyDeviceToHost, s);
-    if (y && e == hipSuccess)
-        e = hipMemcpy2DAsync(y, 4, c->labels, (size_t)stride * 4, 4, rows, hipMemcpyDeviceToHost, s);
-    if (direct && e == hipSuccess)
-        e = hipMemcpy2DAsync(direct, 16, c->qcol, (size_t)stride * 16, 16, rows, hipMemcpyDeviceToHost, s);
-    if (plain && e == hipSuccess)
-        e = hipMemcpy2DAsync(plain, 16, plain_dev, (size_t)stride * 16, 16, rows, hipMemcpyDeviceToHost, s);
+    // every stride-th row, compacted on the device, then one copy per array
+    // (a pitched copy of millions of short rows is slow)
+    void* tmp = nullptr;
+    e = hipMalloc(&tmp, rows * sizeof(float) * kGateFeatures);
+    auto take = [&](const void* src, void* dst, int words) {
+        if (!dst || e != hipSuccess) return;
+        e = launch_take_rows(src, tmp, rows, words, stride, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, tmp, rows * 4 * (size_t)words, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    };
+    take(c->features, X, kGateFeatures);
+    take(c->labels, y, 1);
+    take(c->qcol, direct, 4);
+    if (plain_dev) take(plain_dev, plain, 4);
+    if (tmp) (void)hipFree(tmp);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (plain_dev) (void)hipFree(plain_dev);
     if (e != hipSuccess) return c->fail(TLR_E_HIP, "gate collect: %s", hipGetErrorString(e));
@@ -944,10 +951,9 @@ tlr_status tlr_gate_append_training(tlr_ctx* c, const tlr_recon_params* p, int s
         c->train_cap = cap;
     }
     hipStream_t s = c->stream;
-    HIPCHK(c, hipMemcpy2DAsync(c->train_X + kGateFeatures * c->train_n, kGateFeatures * 4, c->features,
-                               (size_t)stride * kGateFeatures * 4, kGateFeatures * 4, rows, hipMemcpyDeviceToDevice, s));
-    HIPCHK(c, hipMemcpy2DAsync(c->train_y + c->train_n, 4, c->labels, (size_t)stride * 4, 4, rows,
-                               hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, launch_take_rows(c->features, c->train_X + kGateFeatures * c->train_n, (size_t)rows, kGateFeatures,
+                               stride, s));
+    HIPCHK(c, launch_take_rows(c->labels, c->train_y + c->train_n, (size_t)rows, 1, stride, s));
     HIPCHK(c, hipStreamSynchronize(s));
     c->train_n += rows;
     return TLR_OK;

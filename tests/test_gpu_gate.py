@@ -96,3 +96,38 @@ def test_gated_reconstruction_error_budget(tlr):
     g, st = c.reconstruct(out_spp=8, gate=True, collect_stats=True)
     assert np.array_equal(g, plain)
     assert st["gate_rejected"] == 96 * 72 * 8
+
+
+def test_trained_gate_gains_over_untrained(tlr):
+    """What training buys: on a held-out scene, the trained gate routes far
+    more output samples to the direct filter than the untrained network at
+    the same image-error budget, and the gated image stays within it."""
+    import importlib
+    gt = importlib.import_module(tlr.__name__ + ".gate_train")
+    ctx = tlr.Ctx(0)
+    train = [(2, 240, 136, 16, 700 + k) for k in range(4)]
+    params, losses = gt.train_gate(tlr, ctx, train, out_spp=8, stride=2, epochs=6, batch=2048, lr=2e-3, seed=3)
+    print("losses", losses, flush=True)
+    assert losses[-1] < 0.5 * losses[0]
+    val = tlr.make_scene(2, 240, 136, 16, seed=800)
+    budget = 1e-3
+    # calibrated at half the budget: the held-out image must stay within it
+    thr, curve = gt.calibrate_threshold(tlr, ctx, val, 0.5 * budget, out_spp=8)
+    acc_trained = max([a for t, a, e in curve if e <= 0.25 * budget] + [0.0])
+    c0 = tlr.Ctx(0)
+    c0.gate_set_params(gt.init_params(3))
+    thr0, curve0 = gt.calibrate_threshold(tlr, c0, val, 0.5 * budget, out_spp=8)
+    acc_untrained = max([a for t, a, e in curve0 if e <= 0.25 * budget] + [0.0])
+    print("accepted at %.0e: trained %.3f untrained %.3f" % (budget, acc_trained, acc_untrained), flush=True)
+    assert acc_trained > acc_untrained + 0.2
+    # held-out image through the gated pipeline
+    test = tlr.make_scene(2, 240, 136, 16, seed=900)
+    ctx.set_samples(test)
+    ctx.build()
+    plain, _ = ctx.reconstruct(out_spp=8)
+    ctx.gate_set_threshold(thr)
+    gated, st = ctx.reconstruct(out_spp=8, gate=True)
+    err = np.abs(gated - plain)[..., :3].mean()
+    print("held-out: rejected %.3f, mean-abs %.2e" % (st["gate_rejected"] / (240 * 136 * 8), err), flush=True)
+    assert st["gate_rejected"] < 0.8 * 240 * 136 * 8
+    assert err <= budget
