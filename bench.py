@@ -30,6 +30,9 @@ CONFIGS = {
             name="1920x1080 16spp in, 32 out-spp, 400 quads + 5000 occluders, strong defocus, mixed motion"),
     1: dict(preset=1, W=128, H=128, spp=4, out_spp=16,
             name="128x128 4spp in, 16 out-spp, 3 quads, defocus only"),
+    4: dict(preset=2, W=1920, H=1080, spp=16, out_spp=16,
+            name="learned gate: trained on 200 seeded 960x540 16spp scenes of the 960x540 workload type, "
+                 "evaluated on 20 held-out seeded 1920x1080 16spp scenes against the plain GPU path"),
     5: dict(preset=2, W=0, H=0, spp=0, out_spp=16,
             name="64 buffers, resolution uniform in {128^2, 256^2, 512^2, 1024^2, 960x540, 1920x1080}, "
                  "input spp uniform in {1,4,8,16,32}, scene type of the 960x540 workload, 16 out-spp"),
@@ -177,6 +180,78 @@ def bench_batch(args):
     print(json.dumps(res))
 
 
+def bench_gate(args):
+    """Trains the gate on the GPU (own kernels), calibrates its threshold for
+    a 1e-3 mean-abs budget on a validation scene, then times plain and gated
+    reconstruction on held-out 1920x1080 scenes."""
+    import importlib
+    tlr = ge.load_package()
+    gt = importlib.import_module(tlr.__name__ + ".gate_train")
+    cfg = CONFIGS[4]
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    ctx = tlr.Ctx(0)
+    t0 = time.perf_counter()
+    train = [(2, 960, 540, 16, 100000 + k) for k in range(args.gate_train)]
+    params, losses = gt.train_gate(tlr, ctx, train, out_spp=16, stride=32, epochs=6, batch=4096, lr=2e-3,
+                                   log=log if args.gate_train <= 20 else None)
+    t_train = time.perf_counter() - t0
+    log("trained on %d scenes in %.1f s, losses %s" % (args.gate_train, t_train, losses))
+    budget = 1e-3
+    val = tlr.make_scene(2, 960, 540, 16, seed=200000)
+    thr, curve = gt.calibrate_threshold(tlr, ctx, val, budget)
+    acc_val = max([a for t, a, e in curve if e <= 0.5 * budget] + [0.0])
+    ctx.gate_set_threshold(thr)
+    W, H, M = cfg["W"], cfg["H"], cfg["out_spp"]
+    rows = []
+    for k, sb in enumerate(gt._scenes(tlr, [(2, W, H, 16, 300000 + k) for k in range(args.gate_eval)], workers=4)):
+        ctx.set_samples(sb)
+        ctx.build()
+        for _ in range(max(args.warmup, 1)):
+            ctx.reconstruct(out_spp=M)
+            ctx.reconstruct(out_spp=M, gate=True)
+        tp, tg = [], []
+        for _ in range(args.steps):
+            plain, sp = ctx.reconstruct(out_spp=M)
+            gated, sg = ctx.reconstruct(out_spp=M, gate=True)
+            tp.append(sp["query_ms"] + sp["combine_ms"])
+            tg.append(sg["query_ms"] + sg["combine_ms"])
+        err = np.abs(gated - plain)[..., :3]
+        rows.append(dict(plain_ms=float(np.mean(tp)), gated_ms=float(np.mean(tg)), mean_abs=float(err.mean()),
+                         max_abs=float(err.max()), accepted=1.0 - sg["gate_rejected"] / (W * H * M),
+                         direct_ms=sg["direct_ms"], gate_ms=sg["gate_ms"], rest_ms=sg["fast_ms"] + sg["general_ms"],
+                         build_ms=sp["build_ms"]))
+        log("eval %d: %s" % (k, {a: round(b, 5) for a, b in rows[-1].items()}))
+    mean = lambda k: float(np.mean([r[k] for r in rows]))
+    gated_ms = mean("gated_ms") + mean("build_ms")
+    plain_ms = mean("plain_ms") + mean("build_ms")
+    res = {
+        "metric": "reconstructed Mpix/s @16 out-spp, 1 GPU; learned-gate vs plain at <=1e-3 err",
+        "value": W * H / (gated_ms * 1e-3) / 1e6,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": gated_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (gate MLP: bf16 operands, f32 accumulation)",
+        "data": "synthetic (seeded scenes generated on the host, uploaded before timing)",
+        "config": {"workload": cfg["name"], "train_scenes": args.gate_train, "eval_scenes": args.gate_eval,
+                   "width": W, "height": H, "input_spp": 16, "out_spp": M},
+        "plain_mpix_per_s": W * H / (plain_ms * 1e-3) / 1e6,
+        "speedup_vs_plain": plain_ms / gated_ms,
+        "device_ms_per_frame": {"build": mean("build_ms"), "plain_query": mean("plain_ms"),
+                                "gated_query": mean("gated_ms"), "gated_direct": mean("direct_ms"),
+                                "gated_mlp_and_route": mean("gate_ms"), "gated_plain_on_rejected": mean("rest_ms")},
+        "gate": {"threshold": thr, "accepted_fraction": mean("accepted"), "accepted_on_validation": acc_val,
+                 "mean_abs_vs_plain": mean("mean_abs"), "worst_scene_mean_abs": max(r["mean_abs"] for r in rows),
+                 "max_abs_vs_plain": max(r["max_abs"] for r in rows), "budget": budget,
+                 "training_s": t_train, "final_loss": float(losses[-1])},
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +262,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="scanlines in the CPU baseline sample (0: the whole frame)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gate-train", type=int, default=200, help="config 4: training scenes")
+    ap.add_argument("--gate-eval", type=int, default=20, help="config 4: held-out evaluation scenes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (profiler counter scaling)")
     args = ap.parse_args()
@@ -199,6 +276,9 @@ def main():
         return
     if args.config == 5:
         bench_batch(args)
+        return
+    if args.config == 4:
+        bench_gate(args)
         return
 
     cfg = CONFIGS[args.config]
