@@ -57,8 +57,16 @@ typedef struct tlr_recon_params {
     int   collect_stats;       /* fill the Stats counters (slower)                  */
     int   gate;                /* 0: plain method; 1: learned gate                  */
     int   path;                /* 0: fast 1R kernel + general kernel for the rest;
-                                  1: general kernel for every output sample        */
-    int   reserved[3];
+                                  1: general kernel for every output sample;
+                                  2: the reference's GPU-variant method
+                                     (ReconstructionCudaKernels.cu:242-868:
+                                     1R first surface, 2R surfaces, nearest
+                                     sample); lens/time filters do not apply */
+    int   list_cap;            /* path 2, if > 0: per-lane list capacity of the first
+                                  1R and 2R launches (output samples beyond it are
+                                  run again with lists sized for them; results do
+                                  not change)                                       */
+    int   reserved[2];
 } tlr_recon_params;
 
 /* TreeGather::Stats (Reconstruction.hpp:134-147) as sums over output samples,
@@ -183,6 +191,10 @@ tlr_status  tlr_copy_leaves(tlr_ctx* ctx, int32_t* s01, float* bounds14);
 tlr_status  tlr_copy_hierarchy(tlr_ctx* ctx, int32_t* leaf_order, int32_t* children2, float* bounds14);
 /* per output sample colour of the last reconstruct, (y,x,i) order, 4 floats */
 tlr_status  tlr_copy_query_colors(tlr_ctx* ctx, float* out4);
+/* After a path-2 reconstruction: per output sample, the stage that produced
+ * its colour (0: recKernel 1R, 1: failKernel 2R, 2: nearestKernel),
+ * ReconstructionCudaKernels.cu:242,476,634. */
+tlr_status  tlr_copy_query_stages(tlr_ctx* ctx, int32_t* out);
 /* output pattern: 64*M templates x,y,u,v,t (host computation) */
 int         tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, float* out5);
 

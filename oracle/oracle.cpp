@@ -1322,6 +1322,295 @@ void parallel_for(int n, int threads, F f) {
     for (auto& t : ts) t.join();
 }
 
+// ---------------------------------------------------------------------------
+// The reference's GPU-variant algorithm (ReconstructionCudaKernels.cu:242-868:
+// recKernel, failKernel, nearestKernel), per output sample.  Where the
+// reference's CPU replica (ReconstructionCuda.cpp:571-872) differs from the
+// kernels, the kernels are followed: reprojection through oow = 1/pw with
+// cocr = C0*oow + C1 and no static-sample branch (.cu:425-437); the 1R
+// "any point inside" test is strict (.cu:378) and the tent sum starts at
+// that point (.cu:411); failKernel's points are per surface (.cu:722) and its
+// triangle test is the sign-partition form (.cu:586-632); the nearest search
+// shrinks its radius as it walks the hierarchy (.cu:545-549).  Capacity
+// limits (MAX_LEAVES_*, MAX_POINTS, .cu:39-43) and the CPU fallback for
+// queries beyond them (ReconstructionCuda.cpp:412-434) do not exist here.
+// Arithmetic is IEEE (correctly rounded division and square root); the
+// reference builds these kernels with -use_fast_math (.cu:35).
+// Leaves are ordered by (key, leaf id) (the reference's selection sort breaks
+// key ties by traversal order); the nearest walk visits child 0 first, so of
+// two points at one distance the later in that walk wins, as in .cu:545.
+// ---------------------------------------------------------------------------
+struct RefGpu {
+    const orc_ctx* c = nullptr;
+    float disp = 0.f;
+    std::vector<std::pair<float, int>> keyed;
+    std::vector<int> leafs;
+    std::vector<uint8_t> empty;
+    std::vector<float> xy;    // current surface: dx, dy per point
+    std::vector<float> good;  // triangle test partition
+    std::vector<int> st;
+
+    // .cu:419-437
+    void point(const RS& p, const OSample& o, float& dx, float& dy, float& d2) const {
+        const float p0x = p.x * p.w, p0y = p.y * p.w, p0w = p.w;
+        const float px = p0x + (o.t - p.t) * p.mvx;
+        const float py = p0y + (o.t - p.t) * p.mvy;
+        const float pw = p0w + (o.t - p.t) * p.mvw;
+        const float oow = 1.f / pw;
+        const float cocr = c->cc[0] * oow + c->cc[1];
+        const float x = px * oow + cocr * o.u;
+        const float y = py * oow + cocr * o.v;
+        dx = x - o.x;
+        dy = y - o.y;
+        d2 = dx * dx + dy * dy;
+    }
+
+    // .cu:262-330: leaves within radius, front to back by depth at o.t
+    void sorted_leaves(const OSample& o, float R) {
+        find_leaves(c, o, R, leafs);
+        keyed.clear();
+        for (int lf : leafs) {
+            const RS& s = c->samples[c->leaves[lf].s0];
+            keyed.push_back({s.w + (o.t - s.t) * s.mvw, lf});
+        }
+        std::sort(keyed.begin(), keyed.end(), [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
+            if (a.first < b.first) return true;
+            if (b.first < a.first) return false;
+            return a.second < b.second;
+        });
+    }
+
+    // angular extent of the points around the output sample (.cu:443-445)
+    static void wedge(float* box, float dx, float dy) {
+        const float k = dx / dy;
+        if (dy >= 0.f) {
+            box[0] = fminf(box[0], k);
+            box[1] = fmaxf(box[1], k);
+        } else {
+            box[2] = fminf(box[2], k);
+            box[3] = fmaxf(box[3], k);
+        }
+    }
+
+    // recKernel (.cu:242-474).  Returns 0: colour found, 1: go to 2R,
+    // 2: go to the nearest search.
+    int first_surface(const OSample& o, float* col) {
+        const float radius = disp;
+        sorted_leaves(o, radius);
+        if (keyed.empty()) return 1;
+        const float RANGE = 1.0f / std::sqrt(128.0f);
+        const float umin[3] = {o.u - RANGE, o.v - RANGE, o.t - 0.5f * RANGE};
+        const float umax[3] = {o.u + RANGE, o.v + RANGE, o.t + 0.5f * RANGE};
+        float wbox[4] = {FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+        float color[4] = {0, 0, 0, 0};
+        bool multi = false;
+        empty.assign(keyed.size(), 0);
+        for (size_t li = 0; li < keyed.size(); li++) {
+            const Leaf& L = c->leaves[keyed[li].second];
+            int i0 = L.s0;
+            bool found = false;
+            for (; i0 < L.s1; i0++) {
+                float dx, dy, d2;
+                point(c->samples[i0], o, dx, dy, d2);
+                if (d2 < radius * radius) {
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) {
+                empty[li] = 1;
+            } else {
+                for (size_t j = 0; j < li; j++) {
+                    if (empty[j]) continue;
+                    if (!same_surface(L.b, c->leaves[keyed[j].second].b, umin, umax)) {
+                        multi = true;
+                        break;
+                    }
+                }
+            }
+            if (multi) break;
+            for (; i0 < L.s1; i0++) {
+                float dx, dy, d2;
+                const RS& p = c->samples[i0];
+                point(p, o, dx, dy, d2);
+                if (d2 <= radius * radius) {
+                    wedge(wbox, dx, dy);
+                    const float w = fmaxf(0.f, 1.f - std::sqrt(d2) / radius);
+                    color[0] += w * p.r;
+                    color[1] += w * p.g;
+                    color[2] += w * p.b;
+                    color[3] += w * 1.0f;
+                }
+            }
+        }
+        if (color[3] > 0.f && wbox[1] > wbox[2] && wbox[0] < wbox[3]) {
+            const float s = 1.f / color[3];
+            for (int k = 0; k < 4; k++) col[k] = color[k] * s;
+            return 0;
+        }
+        return multi ? 1 : 2;
+    }
+
+    // checkTriangles (.cu:586-632) over the current surface's points
+    bool triangles(int n) {
+        if (n < 3) return false;
+        const float dcomp = 4.f * disp * disp;
+        good.assign(2 * (size_t)n, 0.f);
+        for (int i = 0; i < n - 2; i++) {
+            const float p0x = xy[2 * i], p0y = xy[2 * i + 1];
+            int pg = 0, ng = n;  // positive side fills from the front, negative from the back
+            for (int j = i + 1; j < n; j++) {
+                const float p1x = xy[2 * j], p1y = xy[2 * j + 1];
+                const float ex = p0x - p1x, ey = p0y - p1y;
+                if (ex * ex + ey * ey < dcomp) {
+                    const float c0 = p0x * p1y - p1x * p0y;
+                    const int idx = c0 >= 0.f ? pg++ : --ng;
+                    good[2 * idx] = p1x;
+                    good[2 * idx + 1] = p1y;
+                }
+            }
+            if (ng != n)
+                for (int j = 0; j < pg; j++) {
+                    const float p1x = good[2 * j], p1y = good[2 * j + 1];
+                    for (int k = ng; k < n; k++) {
+                        const float p2x = good[2 * k], p2y = good[2 * k + 1];
+                        const float ex = p1x - p2x, ey = p1y - p2y;
+                        if (p1x * p2y >= p2x * p1y && ex * ex + ey * ey < dcomp) return true;
+                    }
+                }
+        }
+        return false;
+    }
+
+    // failKernel (.cu:634-868).  Returns true when a colour was found.
+    bool failed(const OSample& o, float* col) {
+        const float radius = disp * 2.f;
+        sorted_leaves(o, radius);
+        if (keyed.empty()) return false;
+        const float RANGE = 1.0f / std::sqrt(128.0f);
+        const float umin[3] = {o.u - RANGE, o.v - RANGE, o.t - 0.5f * RANGE};
+        const float umax[3] = {o.u + RANGE, o.v + RANGE, o.t + 0.5f * RANGE};
+        const int nl = (int)keyed.size();
+        float color[4] = {0, 0, 0, 0};
+        int lidx = 0;
+        do {
+            const int sfirst = lidx;
+            int np = 0;
+            xy.clear();
+            for (int k = 0; k < 4; k++) color[k] = 0.f;
+            float abox[4] = {FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+            float wbox[4] = {FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+            do {
+                const Leaf& L = c->leaves[keyed[lidx].second];
+                bool end = false;
+                if (np >= 4)
+                    for (int j = sfirst; j < lidx; j++)
+                        if (!same_surface(L.b, c->leaves[keyed[j].second].b, umin, umax)) {
+                            end = true;
+                            break;
+                        }
+                if (end) break;
+                for (int i = L.s0; i < L.s1; i++) {
+                    float dx, dy, d2;
+                    const RS& p = c->samples[i];
+                    point(p, o, dx, dy, d2);
+                    if (d2 <= radius * radius) {
+                        xy.push_back(dx);
+                        xy.push_back(dy);
+                        np++;
+                        const float w = 1.f - std::sqrt(d2) / disp;
+                        if (w <= 0.f) {
+                            wedge(abox, dx, dy);
+                        } else {
+                            wedge(wbox, dx, dy);
+                            color[0] += w * p.r;
+                            color[1] += w * p.g;
+                            color[2] += w * p.b;
+                            color[3] += w * 1.0f;
+                        }
+                    }
+                }
+                lidx++;
+            } while (lidx < nl);
+            if (lidx < nl) {
+                abox[0] = fminf(abox[0], wbox[0]);
+                abox[1] = fmaxf(abox[1], wbox[1]);
+                abox[2] = fminf(abox[2], wbox[2]);
+                abox[3] = fmaxf(abox[3], wbox[3]);
+                if (abox[1] > abox[2] && abox[0] < abox[3])
+                    if ((wbox[1] > wbox[2] && wbox[0] < wbox[3]) || triangles(np)) break;
+            }
+        } while (lidx < nl);
+        if (color[3] > 0.f) {
+            const float s = 1.f / color[3];
+            for (int k = 0; k < 4; k++) col[k] = color[k] * s;
+            return true;
+        }
+        return false;
+    }
+
+    // nearestKernel (.cu:476-579): radius 1, 2, 4, 8, 16 x dispersion; within
+    // a pass the radius shrinks to each nearer point found.  Zero colour when
+    // nothing lies within 16 x dispersion.
+    void nearest(const OSample& o, float* col) {
+        for (int k = 0; k < 4; k++) col[k] = 0.f;
+        if (c->root == -1) return;
+        for (float rmul = 1.f; rmul <= 16.f; rmul *= 2.f) {
+            float radius = rmul * disp;
+            int best = -1;
+            st.clear();
+            st.push_back(c->root);
+            while (!st.empty()) {
+                const int id = st.back();
+                st.pop_back();
+                if (id & LEAF_TAG) {
+                    const Leaf& L = c->leaves[id & ~LEAF_TAG];
+                    if (!node_hit(L.b, o.x, o.y, o.u, o.v, o.t, radius)) continue;
+                    for (int i = L.s0; i < L.s1; i++) {
+                        float dx, dy, d2;
+                        point(c->samples[i], o, dx, dy, d2);
+                        if (d2 <= radius * radius) {
+                            radius = std::sqrt(d2);
+                            best = i;
+                        }
+                    }
+                } else {
+                    const Inner& nd = c->inner[id];
+                    if (!nd.b.valid()) continue;
+                    if (node_hit(nd.b, o.x, o.y, o.u, o.v, o.t, radius)) {
+                        st.push_back(nd.c1);
+                        st.push_back(nd.c0);
+                    }
+                }
+            }
+            if (best >= 0) {
+                const RS& p = c->samples[best];
+                col[0] = p.r;
+                col[1] = p.g;
+                col[2] = p.b;
+                col[3] = 1.0f;
+                return;
+            }
+        }
+    }
+
+    // one output sample: 1R, then 2R or the nearest search.  *stage: the
+    // pass that produced the colour (0: 1R, 1: 2R, 2: nearest).
+    void query(const OSample& o, float* col, int* stage) {
+        const int r = first_surface(o, col);
+        if (r == 0) {
+            *stage = 0;
+            return;
+        }
+        if (r == 1 && failed(o, col)) {
+            *stage = 1;
+            return;
+        }
+        *stage = 2;
+        nearest(o, col);
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1616,6 +1905,52 @@ int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image, float* 
             stats->count += s.count;
         }
     }
+    return 0;
+}
+
+// CudaReconstruction::reconstructGPU (ReconstructionCuda.cpp:327-465): every
+// output sample through recKernel / failKernel / nearestKernel, per-pixel sum
+// normalised by alpha, red where the sum has no weight (.cpp:446-456).  The
+// per-pixel sum is in output-sample order (the reference adds atomically).
+int orc_reconstruct_refgpu(orc_ctx* c, const orc_recon_params* p, float* image, float* qcol, int32_t* stage) {
+    if (!c->built) return 1;
+    std::vector<Tmpl> pat;
+    const int M = make_pattern(p->out_spp, p->override_uvt, p->uvt, pat);
+    const int W = c->W, H = c->H;
+    const int stride = std::max(1, p->row_stride);
+    const int threads = hw_threads(p->threads);
+    const int rows = (H + stride - 1) / stride;
+    std::vector<RefGpu> rg(threads);
+    for (auto& r : rg) {
+        r.c = c;
+        r.disp = dispersion_for(c->spp_pow2);
+    }
+    parallel_for(rows, threads, [&](int w, int ri) {
+        const int y = ri * stride;
+        for (int x = 0; x < W; x++) {
+            const int pat_i = (int)(hash_bits((uint32_t)x, (uint32_t)y) % 64u);
+            float pc[4] = {0, 0, 0, 0};
+            for (int i = 0; i < M; i++) {
+                const Tmpl& tp = pat[(size_t)pat_i * M + i];
+                OSample o;
+                o.x = tp.x + (float)x;
+                o.y = tp.y + (float)y;
+                o.u = tp.u; o.v = tp.v; o.t = tp.t;
+                float col[4];
+                int stg = 0;
+                rg[w].query(o, col, &stg);
+                for (int k = 0; k < 4; k++) pc[k] += col[k];
+                const size_t q = ((size_t)y * W + x) * M + i;
+                if (qcol)
+                    for (int k = 0; k < 4; k++) qcol[4 * q + k] = col[k];
+                if (stage) stage[q] = stg;
+            }
+            if (pc[3] == 0) { pc[0] = 1; pc[1] = 0; pc[2] = 0; pc[3] = 1; }
+            const float rw = rcpf(pc[3]);
+            if (image)
+                for (int k = 0; k < 4; k++) image[4 * ((size_t)y * W + x) + k] = pc[k] * rw;
+        }
+    });
     return 0;
 }
 

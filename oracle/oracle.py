@@ -83,6 +83,8 @@ def lib():
         L.orc_output_pattern.argtypes = [C.c_int, C.c_int, fp, C.c_void_p]
         L.orc_reconstruct.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
                                       C.POINTER(Stats)]
+        L.orc_reconstruct_refgpu.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
+                                             C.c_void_p]
         L.orc_gather.argtypes = [P, fp, C.c_float, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                  C.c_void_p, C.c_void_p, C.c_int]
         L.orc_query.argtypes = [P, C.POINTER(ReconParams), C.c_int, C.c_int, C.c_int,
@@ -207,6 +209,20 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle reconstruct failed (%d)" % rc)
         return img, q, st.as_dict()
+
+    def reconstruct_refgpu(self, out_spp=16, want_queries=False, **kw):
+        """The reference's GPU-variant method (recKernel / failKernel /
+        nearestKernel): image, per-output-sample colours and stages (0: 1R,
+        1: 2R, 2: nearest) when want_queries."""
+        p = self.params(out_spp=out_spp, **kw)
+        M = 1 if p.override_uvt else out_spp
+        img = np.zeros((self.H, self.W, 4), np.float32)
+        q = np.zeros((self.H, self.W, M, 4), np.float32) if want_queries else None
+        stg = np.zeros((self.H, self.W, M), np.int32) if want_queries else None
+        rc = lib().orc_reconstruct_refgpu(self.h, C.byref(p), _ptr(img), _ptr(q), _ptr(stg))
+        if rc != 0:
+            raise RuntimeError("oracle reconstruct_refgpu failed (%d)" % rc)
+        return img, q, stg
 
     def gather(self, o5, R, max_leaves=4096, max_samples=1 << 16):
         o = np.asarray(o5, np.float32)

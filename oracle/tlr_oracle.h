@@ -107,6 +107,14 @@ int orc_output_pattern(int out_spp, int override_uvt, const float* uvt, float* o
 int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image,
                     float* query_colors, orc_stats* stats);
 
+/* The reference's GPU-variant algorithm (recKernel, failKernel and
+ * nearestKernel of ReconstructionCudaKernels.cu) over the same hierarchy.
+ * image and query_colors as for orc_reconstruct; stage (optional): per
+ * output sample, the pass that produced its colour (0: first surface at 1R,
+ * 1: surfaces at 2R, 2: nearest-sample search). */
+int orc_reconstruct_refgpu(orc_ctx* c, const orc_recon_params* p, float* image,
+                           float* query_colors, int32_t* stage);
+
 /* Debug gather for one output sample at radius R: leaf ids (sorted order),
  * gathered sample layout indices and their surface ids.  Returns the number
  * of gathered samples; *num_leaves receives the leaf count.  Buffers may be

@@ -29,7 +29,8 @@ EXPORTED = [
     "tlr_create", "tlr_destroy", "tlr_last_error", "tlr_load_sample_file", "tlr_save_sample_file",
     "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_sync",
     "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
-    "tlr_copy_leaves", "tlr_copy_hierarchy", "tlr_copy_query_colors", "tlr_output_pattern",
+    "tlr_copy_leaves", "tlr_copy_hierarchy", "tlr_copy_query_colors", "tlr_copy_query_stages",
+    "tlr_output_pattern",
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
@@ -59,7 +60,7 @@ class ReconParams(C.Structure):
     _fields_ = [("out_spp", C.c_int), ("override_uvt", C.c_int), ("uvt", C.c_float * 3),
                 ("lens_gaussian", C.c_int), ("time_gaussian", C.c_int),
                 ("collect_stats", C.c_int), ("gate", C.c_int), ("path", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("list_cap", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class Stats(C.Structure):
@@ -150,6 +151,7 @@ def lib():
         L.tlr_batch_retune.argtypes = [V]
         L.tlr_use_slot.argtypes = [V, C.c_int]
         L.tlr_copy_query_colors.argtypes = [V, V]
+        L.tlr_copy_query_stages.argtypes = [V, V]
         L.tlr_output_pattern.restype = C.c_int
         L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
         L.tlr_gate_set_params.argtypes = [V, V]
@@ -286,7 +288,7 @@ class Ctx:
         self._chk(lib().tlr_build(self.h, aperture_adjust, focal_adjust))
 
     def reconstruct(self, out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,
-                    collect_stats=False, gate=False, path=0, out=None):
+                    collect_stats=False, gate=False, path=0, out=None, list_cap=0):
         p = ReconParams()
         p.out_spp = out_spp
         p.override_uvt = 1 if override_uvt is not None else 0
@@ -298,6 +300,7 @@ class Ctx:
         p.collect_stats = int(collect_stats)
         p.gate = int(gate)
         p.path = int(path)
+        p.list_cap = int(list_cap)
         img = out if out is not None else np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
         st = Stats()
         self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img), 0, C.byref(st)))
@@ -463,6 +466,13 @@ class Ctx:
     def query_colors(self, out_spp):
         out = np.empty((self.sbuf.height, self.sbuf.width, out_spp, 4), np.float32)
         self._chk(lib().tlr_copy_query_colors(self.h, _p(out)))
+        return out
+
+    def query_stages(self, out_spp):
+        """After reconstruct(path=2): per output sample, the stage that gave
+        its colour (0: 1R first surface, 1: 2R surfaces, 2: nearest)."""
+        out = np.empty((self.sbuf.height, self.sbuf.width, out_spp), np.int32)
+        self._chk(lib().tlr_copy_query_stages(self.h, _p(out)))
         return out
 
 

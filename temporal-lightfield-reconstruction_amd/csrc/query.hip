@@ -14,6 +14,7 @@
 // Sums that the reference accumulates sequentially (tent filter) are
 // accumulated in the same order by every lane.
 #include "query.hpp"
+#include "query_util.hpp"
 
 namespace tlr {
 namespace {
@@ -24,9 +25,6 @@ constexpr int kWavesPerBlock = kBlock / 64;
 constexpr unsigned kNeeds2R = 0x80000000u;
 enum { XPOS = 1, YPOS = 2, XNEG = 4, YNEG = 8 };
 
-__device__ __forceinline__ unsigned lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -68,19 +66,6 @@ __device__ __forceinline__ WaveMem wave_mem(float* base) {
     m.f_mini = reinterpret_cast<int*>(take(kLeafCap));
     m.f_min = reinterpret_cast<float*>(take(kLeafCap));
     return m;
-}
-
-struct OSample {
-    float x, y, u, v, t;
-};
-
-// Node::intersect: Reconstruction.hpp:204-216
-__device__ __forceinline__ bool node_hit(const float* p, const OSample& o, float R) {
-    const float mnx = plane_eval(p + 0, o.t, o.u) - R;
-    const float mny = plane_eval(p + 6, o.t, o.v) - R;
-    const float mxx = plane_eval(p + 3, o.t, o.u) + R;
-    const float mxy = plane_eval(p + 9, o.t, o.v) + R;
-    return (o.x >= mnx && o.y >= mny && o.x <= mxx && o.y <= mxy);
 }
 
 // Reprojection of a tree sample (pa, pb records) to the output's (u,v,t):
@@ -690,13 +675,6 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
 // pass stops when the second surface starts.
 // ---------------------------------------------------------------------------
 constexpr int kFastBlock = 128;
-
-// Key of leaf id at output time t (ReconstructionOur.cpp:384-385), from the
-// first 16 bytes of its record.
-__device__ __forceinline__ float leaf_key(const LeafRec* leaves, int id, float t) {
-    const float4 r = *reinterpret_cast<const float4*>(leaves + id);
-    return r.x + (t - r.y) * r.z;
-}
 
 // One block of the fast path: `slot` is this lane's query index (or list
 // position).  lid: the block's LDS leaf lists, CAP entries per lane.

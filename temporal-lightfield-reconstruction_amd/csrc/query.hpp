@@ -115,4 +115,34 @@ hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rg
 hipError_t launch_gate_route(const float* logits, const float* features, unsigned n, float threshold,
                              unsigned* list, unsigned* count, hipStream_t s);
 
+// The reference's GPU-variant method (ReconstructionCudaKernels.cu:242-868):
+// stage launches over output samples and lists of them.
+constexpr int kRefFirstCap = 24;        // 1R leaves per lane in LDS (first launch)
+constexpr int kRefFailLeafCap = 96;     // 2R leaves per lane (global memory)
+constexpr int kRefFailPointCap = 192;   // points of one surface per lane (global memory)
+constexpr int kRefWavesPerCU = 8;       // resident waves of the persistent stage launches
+struct RefLaunch {
+    QueryParams q;             // geometry, tree, points, qcol
+    const unsigned* list;      // stage input (query ids) and its length
+    const unsigned* count;
+    unsigned* next;            // work counter (zeroed before launch)
+    unsigned* to_fail;         // output lists: 2R, nearest, beyond this launch's lists
+    unsigned* n_fail;
+    unsigned* to_near;
+    unsigned* n_near;
+    unsigned* to_over;
+    unsigned* n_over;
+    unsigned* need;            // [0] leaves, [1] points: maxima over the to_over entries
+    int* stage;                // optional: per query 0 (1R), 1 (2R), 2 (nearest)
+    int2* lists;               // per lane leaf_cap (key, id) pairs, lane-interleaved per wave
+    float2* xy;                // per lane point_cap points
+    int leaf_cap, point_cap;   // per-lane list capacities (first launch: leaf_cap <= kRefFirstCap)
+};
+int ref_stage_blocks(int cus, int waves_per_cu);
+// first stage over every output sample (1R, LDS leaf lists)
+hipError_t launch_ref_first(const RefLaunch& r, hipStream_t s);
+// stage 0: 1R over r.list; stage 1: 2R over r.list (blocks of 4 waves)
+hipError_t launch_ref_stage(const RefLaunch& r, int stage, int blocks, hipStream_t s);
+hipError_t launch_ref_nearest(const RefLaunch& r, int blocks, hipStream_t s);
+
 }  // namespace tlr

@@ -110,6 +110,10 @@ struct tlr_ctx {
     float* labels = nullptr;
     unsigned* route_list = nullptr;
     unsigned* d_route_count = nullptr;
+    // reference GPU-variant method (path 2)
+    DeviceArena rarena;             // lists, counters, stage codes, first 2R lists
+    DeviceArena rwide;              // lists of the overflow launches
+    int* ref_stage = nullptr;       // per query stage of the last path-2 call
 
     // learned gate
     float* gate_p = nullptr;        // fp32 master parameters
@@ -199,6 +203,8 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->arena.release();
     c->qarena.release();
+    c->rarena.release();
+    c->rwide.release();
     if (c->d_in) (void)hipFree(c->d_in);
     if (c->d_gi) (void)hipFree(c->d_gi);
     if (c->d_stats) (void)hipFree(c->d_stats);
@@ -597,15 +603,136 @@ static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* l
     return TLR_OK;
 }
 
+// The reference's GPU-variant method (ReconstructionCuda.cpp:327-465): 1R
+// over every output sample, 2R over the ones with several surfaces, nearest
+// sample for the rest; samples beyond a launch's per-lane lists are run again
+// with lists sized for them.  *n2r: output samples that went to 2R.
+static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, unsigned* n2r) {
+    hipStream_t s = c->stream;
+    const size_t nq = Q.num_queries;
+    const int cus = c->prop.multiProcessorCount;
+    const int blocks = ref_stage_blocks(cus, kRefWavesPerCU);
+    const size_t lanes = (size_t)blocks * 256;
+    const size_t fail_bytes = lanes * ((size_t)kRefFailLeafCap * sizeof(int2) + (size_t)kRefFailPointCap * 8);
+    enum { N_FAIL, N_NEAR, N_OVER1, N_OVER2, N_OVER3, N_OVER4, NX0, NX1, NX2, NX3, NX4, NEED1, NEED1B, NEED2,
+           NEED2B, NEED3, NEED3B, NEED4, NEED4B, NUM_COUNTERS };
+    HIPCHK(c, c->rarena.reserve(5 * nq * 4 + fail_bytes + 16 * 256));
+    c->rarena.reset();
+    unsigned* cnt = c->rarena.alloc<unsigned>(NUM_COUNTERS);
+    unsigned* fail = c->rarena.alloc<unsigned>(nq);
+    unsigned* near = c->rarena.alloc<unsigned>(nq);
+    unsigned* over1 = c->rarena.alloc<unsigned>(nq);
+    unsigned* over2 = c->rarena.alloc<unsigned>(nq);
+    c->ref_stage = c->rarena.alloc<int>(nq);
+    int2* lists = c->rarena.alloc<int2>(lanes * kRefFailLeafCap);
+    float2* xy = c->rarena.alloc<float2>(lanes * kRefFailPointCap);
+    if (!xy) return c->fail(TLR_E_HIP, "reference-method arena exhausted");
+    HIPCHK(c, hipMemsetAsync(cnt, 0, NUM_COUNTERS * sizeof(unsigned), s));
+
+    RefLaunch R;
+    memset(&R, 0, sizeof(R));
+    R.q = Q;
+    R.to_fail = fail;
+    R.n_fail = cnt + N_FAIL;
+    R.to_near = near;
+    R.n_near = cnt + N_NEAR;
+    R.stage = c->ref_stage;
+
+    // Overflow launch: lists sized by need[] for the samples in `list`.
+    auto wide = [&](int stage, unsigned* list, unsigned* count, unsigned n, const unsigned* need, unsigned* over,
+                    unsigned* n_over, unsigned* need_out, unsigned* next) -> tlr_status {
+        const size_t leaf_cap = std::max(1u, need[0]), point_cap = stage == 1 ? std::max(1u, need[1]) : 0;
+        const size_t per_wave = 64 * (leaf_cap * sizeof(int2) + point_cap * 8);
+        const size_t budget = (size_t)1 << 30;
+        size_t waves = std::min<size_t>((n + 63) / 64, (size_t)cus * kRefWavesPerCU);
+        waves = std::min(waves, std::max<size_t>(1, budget / per_wave));
+        waves = (waves + 3) / 4 * 4;
+        if (per_wave > ((size_t)4 << 30))
+            return c->fail(TLR_E_CAPACITY, "an output sample needs %zu leaves and %zu points", leaf_cap, point_cap);
+        HIPCHK(c, c->rwide.reserve(waves * per_wave + 2 * 256));
+        c->rwide.reset();
+        RefLaunch W = R;
+        W.list = list;
+        W.count = count;
+        W.next = next;
+        W.to_over = over;
+        W.n_over = n_over;
+        W.need = need_out;
+        W.leaf_cap = (int)leaf_cap;
+        W.point_cap = (int)point_cap;
+        W.lists = c->rwide.alloc<int2>(waves * 64 * leaf_cap);
+        W.xy = point_cap ? c->rwide.alloc<float2>(waves * 64 * point_cap) : nullptr;
+        if (!W.lists || (point_cap && !W.xy)) return c->fail(TLR_E_HIP, "reference-method overflow arena exhausted");
+        HIPCHK(c, launch_ref_stage(W, stage, (int)(waves / 4), s));
+        return TLR_OK;
+    };
+
+    // 1R over every output sample
+    const int cap1 = list_cap > 0 ? std::min(list_cap, kRefFirstCap) : kRefFirstCap;
+    RefLaunch F = R;
+    F.leaf_cap = cap1;
+    F.to_over = over1;
+    F.n_over = cnt + N_OVER1;
+    F.need = cnt + NEED1;
+    HIPCHK(c, launch_ref_first(F, s));
+    HIPCHK(c, hipEventRecord(c->ev[5], s));
+    unsigned h[NUM_COUNTERS];
+    HIPCHK(c, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (h[N_OVER1]) {
+        tlr_status st = wide(0, over1, cnt + N_OVER1, h[N_OVER1], h + NEED1, over2, cnt + N_OVER3, cnt + NEED3,
+                             cnt + NX1);
+        if (st != TLR_OK) return st;
+    }
+    // 2R over the samples with several surfaces at 1R
+    RefLaunch T = R;
+    T.list = fail;
+    T.count = cnt + N_FAIL;
+    T.next = cnt + NX0;
+    T.to_over = over1;
+    T.n_over = cnt + N_OVER2;
+    T.need = cnt + NEED2;
+    T.lists = lists;
+    T.xy = xy;
+    T.leaf_cap = list_cap > 0 ? std::min(list_cap, kRefFailLeafCap) : kRefFailLeafCap;
+    T.point_cap = list_cap > 0 ? std::min(list_cap, kRefFailPointCap) : kRefFailPointCap;
+    HIPCHK(c, launch_ref_stage(T, 1, blocks, s));
+    HIPCHK(c, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (h[N_OVER3])
+        return c->fail(TLR_E_CAPACITY, "%u output samples exceeded the sized 1R lists", h[N_OVER3]);
+    if (h[N_OVER2]) {
+        tlr_status st = wide(1, over1, cnt + N_OVER2, h[N_OVER2], h + NEED2, over2, cnt + N_OVER4, cnt + NEED4,
+                             cnt + NX2);
+        if (st != TLR_OK) return st;
+    }
+    // nearest sample for the rest
+    RefLaunch N = R;
+    N.list = near;
+    N.count = cnt + N_NEAR;
+    N.next = cnt + NX3;
+    HIPCHK(c, launch_ref_nearest(N, blocks, s));
+    HIPCHK(c, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (h[N_OVER4])
+        return c->fail(TLR_E_CAPACITY, "%u output samples exceeded the sized 2R lists", h[N_OVER4]);
+    *n2r = h[N_FAIL];
+    return TLR_OK;
+}
+
 tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, int out_on_device, tlr_stats* stats) {
     if (!c || !p) return TLR_E_INVALID;
     if (p->gate && !c->has_gate) return c->fail(TLR_E_STATE, "gate requested but no gate parameters set");
+    if (p->path < 0 || p->path > 2) return c->fail(TLR_E_INVALID, "unknown path %d", p->path);
+    if (p->path == 2 && p->gate) return c->fail(TLR_E_INVALID, "the gate applies to the plain method only");
+    c->ref_stage = nullptr;
     QueryParams Q;
     tlr_status st = prepare_queries(c, p, p->gate != 0, Q);
     if (st != TLR_OK) return st;
     hipStream_t s = c->stream;
     const int W = c->h.width, H = c->h.height, M = c->last_M;
     const size_t nq = c->qcol_n;
+    unsigned n2r = 0;
 
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (p->gate) {
@@ -620,6 +747,8 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
                                     c->d_route_count, s));
         HIPCHK(c, hipEventRecord(c->ev[7], s));
         st = run_plain(c, Q, c->route_list, c->d_route_count, false, 0);
+    } else if (p->path == 2) {
+        st = run_refgpu(c, Q, p->list_cap, &n2r);
     } else {
         st = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0, p->path);
     }
@@ -665,6 +794,7 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         }
         if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) stats->general_ms = ms;
         stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)nfail;
+        if (p->path == 2) stats->num_2r = stats->general_queries = (double)n2r;
         stats->total_ms = stats->build_ms + stats->query_ms + stats->combine_ms;
         stats->num_leaves = c->L;
         stats->num_samples = c->nb;
@@ -790,6 +920,13 @@ tlr_status tlr_copy_leaves(tlr_ctx* c, int32_t* s01, float* b14) {
 tlr_status tlr_copy_query_colors(tlr_ctx* c, float* out4) {
     if (!c || !c->qcol || !out4) return TLR_E_STATE;
     HIPCHK(c, hipMemcpy(out4, c->qcol, c->qcol_n * sizeof(float4), hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_query_stages(tlr_ctx* c, int32_t* out) {
+    if (!c || !out) return TLR_E_INVALID;
+    if (!c->ref_stage) return c->fail(TLR_E_STATE, "the last reconstruction did not use path 2");
+    HIPCHK(c, hipMemcpy(out, c->ref_stage, c->qcol_n * sizeof(int32_t), hipMemcpyDeviceToHost));
     return TLR_OK;
 }
 
