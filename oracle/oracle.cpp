@@ -30,6 +30,13 @@
  *   tent filter / image                 constant-colour scene reconstructs
  *       to that colour; single static plane scene
  *                                                        [test_oracle_filter]
+ *   GPU-variant method (RefGpu)         checkTriangles equals a float64
+ *       brute force over triples; reprojection equals the closed form; the
+ *       nearest search is nearest over every leaf its bounds reach; 1R first
+ *       surface and 2R surfaces equal float64 tent means on a single plane;
+ *       near plane first with two planes present         [test_oracle_refgpu]
+ *       The 2R surface split (MIN_SAMPLES_PER_SURFACE) has no separate
+ *       check beyond these.
  * The reference itself cannot be built here (Win32-only headers), so no
  * function is pinned against reference outputs.
  */
@@ -1552,9 +1559,9 @@ struct RefGpu {
     // nearestKernel (.cu:476-579): radius 1, 2, 4, 8, 16 x dispersion; within
     // a pass the radius shrinks to each nearer point found.  Zero colour when
     // nothing lies within 16 x dispersion.
-    void nearest(const OSample& o, float* col) {
+    int nearest(const OSample& o, float* col) {
         for (int k = 0; k < 4; k++) col[k] = 0.f;
-        if (c->root == -1) return;
+        if (c->root == -1) return -1;
         for (float rmul = 1.f; rmul <= 16.f; rmul *= 2.f) {
             float radius = rmul * disp;
             int best = -1;
@@ -1589,9 +1596,10 @@ struct RefGpu {
                 col[1] = p.g;
                 col[2] = p.b;
                 col[3] = 1.0f;
-                return;
+                return best;
             }
         }
+        return -1;
     }
 
     // one output sample: 1R, then 2R or the nearest search.  *stage: the
@@ -1951,6 +1959,59 @@ int orc_reconstruct_refgpu(orc_ctx* c, const orc_recon_params* p, float* image, 
                 for (int k = 0; k < 4; k++) image[4 * ((size_t)y * W + x) + k] = pc[k] * rw;
         }
     });
+    return 0;
+}
+
+// Pieces of the GPU-variant method for the oracle's own checks.
+int orc_refgpu_query(orc_ctx* c, const float* o5, float* col4, int32_t* stage) {
+    if (!c->built) return 1;
+    RefGpu r;
+    r.c = c;
+    r.disp = dispersion_for(c->spp_pow2);
+    OSample o{o5[0], o5[1], o5[2], o5[3], o5[4]};
+    int st = 0;
+    r.query(o, col4, &st);
+    if (stage) *stage = st;
+    return 0;
+}
+
+// nearestKernel alone: tree-layout index of the sample found, -1 if none.
+int orc_refgpu_nearest(orc_ctx* c, const float* o5, float* col4) {
+    if (!c->built) return -2;
+    RefGpu r;
+    r.c = c;
+    r.disp = dispersion_for(c->spp_pow2);
+    OSample o{o5[0], o5[1], o5[2], o5[3], o5[4]};
+    return r.nearest(o, col4);
+}
+
+// failKernel alone (2R surfaces): 1 when it produced a colour.
+int orc_refgpu_failed(orc_ctx* c, const float* o5, float* col4) {
+    if (!c->built) return -2;
+    RefGpu r;
+    r.c = c;
+    r.disp = dispersion_for(c->spp_pow2);
+    OSample o{o5[0], o5[1], o5[2], o5[3], o5[4]};
+    for (int k = 0; k < 4; k++) col4[k] = 0.f;
+    return r.failed(o, col4) ? 1 : 0;
+}
+
+// checkTriangles on n points (x, y pairs relative to the output sample).
+int orc_refgpu_triangles(const float* xy, int n, float disp) {
+    RefGpu r;
+    r.disp = disp;
+    r.xy.assign(xy, xy + 2 * (size_t)n);
+    return r.triangles(n) ? 1 : 0;
+}
+
+// The GPU-variant reprojection of every tree sample (layout order) to o:
+// dx, dy, dist^2 per sample.
+int orc_refgpu_reproject(orc_ctx* c, const float* o5, float* out3) {
+    if (!c->built) return 1;
+    RefGpu r;
+    r.c = c;
+    OSample o{o5[0], o5[1], o5[2], o5[3], o5[4]};
+    for (size_t i = 0; i < c->samples.size(); i++) r.point(c->samples[i], o, out3[3 * i], out3[3 * i + 1], out3[3 * i + 2]);
     return 0;
 }
 

@@ -57,6 +57,13 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def refgpu_triangles(xy, disp):
+    """checkTriangles of the GPU-variant method on points relative to the
+    output sample."""
+    p = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    return bool(lib().orc_refgpu_triangles(_ptr(p), p.shape[0], float(disp)))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -85,6 +92,11 @@ def lib():
                                       C.POINTER(Stats)]
         L.orc_reconstruct_refgpu.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
                                              C.c_void_p]
+        L.orc_refgpu_query.argtypes = [P, fp, C.c_void_p, C.POINTER(C.c_int32)]
+        L.orc_refgpu_nearest.argtypes = [P, fp, C.c_void_p]
+        L.orc_refgpu_failed.argtypes = [P, fp, C.c_void_p]
+        L.orc_refgpu_triangles.argtypes = [C.c_void_p, C.c_int, C.c_float]
+        L.orc_refgpu_reproject.argtypes = [P, fp, C.c_void_p]
         L.orc_gather.argtypes = [P, fp, C.c_float, C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                  C.c_void_p, C.c_void_p, C.c_int]
         L.orc_query.argtypes = [P, C.POINTER(ReconParams), C.c_int, C.c_int, C.c_int,
@@ -223,6 +235,35 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle reconstruct_refgpu failed (%d)" % rc)
         return img, q, stg
+
+    def refgpu_query(self, o5):
+        """One output sample through the GPU-variant method: (colour, stage)."""
+        o = np.asarray(o5, np.float32)
+        col = np.zeros(4, np.float32)
+        st = C.c_int32(0)
+        if lib().orc_refgpu_query(self.h, o, _ptr(col), C.byref(st)) != 0:
+            raise RuntimeError("not built")
+        return col, st.value
+
+    def refgpu_nearest(self, o5):
+        """nearestKernel alone: (tree-layout index or -1, colour)."""
+        o = np.asarray(o5, np.float32)
+        col = np.zeros(4, np.float32)
+        k = lib().orc_refgpu_nearest(self.h, o, _ptr(col))
+        return k, col
+
+    def refgpu_failed(self, o5):
+        """failKernel alone (2R surfaces): (found, colour)."""
+        o = np.asarray(o5, np.float32)
+        col = np.zeros(4, np.float32)
+        return lib().orc_refgpu_failed(self.h, o, _ptr(col)) == 1, col
+
+    def refgpu_reproject(self, o5):
+        """dx, dy, dist^2 of every tree sample (layout order) at o5."""
+        o = np.asarray(o5, np.float32)
+        out = np.zeros((self.info().num_bucketed, 3), np.float32)
+        lib().orc_refgpu_reproject(self.h, o, _ptr(out))
+        return out
 
     def gather(self, o5, R, max_leaves=4096, max_samples=1 << 16):
         o = np.asarray(o5, np.float32)

@@ -115,6 +115,17 @@ int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image,
 int orc_reconstruct_refgpu(orc_ctx* c, const orc_recon_params* p, float* image,
                            float* query_colors, int32_t* stage);
 
+/* Pieces of the GPU-variant method, for checks: one output sample (o5 =
+ * x,y,u,v,t) through all stages; the nearest search alone (returns the
+ * tree-layout index of the sample found or -1); checkTriangles on n points
+ * (x,y pairs relative to the output sample); the variant's reprojection of
+ * every tree sample to o5 (dx, dy, dist^2 per sample, layout order). */
+int orc_refgpu_query(orc_ctx* c, const float* o5, float* col4, int32_t* stage);
+int orc_refgpu_nearest(orc_ctx* c, const float* o5, float* col4);
+int orc_refgpu_failed(orc_ctx* c, const float* o5, float* col4);  /* 1: colour found */
+int orc_refgpu_triangles(const float* xy, int n, float disp);
+int orc_refgpu_reproject(orc_ctx* c, const float* o5, float* out3);
+
 /* Debug gather for one output sample at radius R: leaf ids (sorted order),
  * gathered sample layout indices and their surface ids.  Returns the number
  * of gathered samples; *num_leaves receives the leaf count.  Buffers may be
