@@ -62,7 +62,7 @@ def read_traffic(kernel, workload_key):
     return None
 
 
-def cpu_baseline(cfg, sbuf, sample_rows):
+def cpu_baseline(cfg, sbuf, sample_rows, method="plain"):
     orc = ge.load_oracle()
     W, H = cfg["W"], cfg["H"]
     threads = os.cpu_count() or 1
@@ -76,7 +76,10 @@ def cpu_baseline(cfg, sbuf, sample_rows):
     t_build = time.perf_counter() - t0
     stride = max(1, H // sample_rows) if sample_rows > 0 else 1
     t0 = time.perf_counter()
-    o.reconstruct(out_spp=cfg["out_spp"], threads=threads, row_stride=stride)
+    if method == "refgpu":
+        o.reconstruct_refgpu(out_spp=cfg["out_spp"], threads=threads, row_stride=stride)
+    else:
+        o.reconstruct(out_spp=cfg["out_spp"], threads=threads, row_stride=stride)
     t_filter = time.perf_counter() - t0
     rows = (H + stride - 1) // stride
     est = t_build + t_filter * (H / rows)
@@ -252,6 +255,74 @@ def bench_gate(args):
     print(json.dumps(res))
 
 
+def bench_refgpu(args):
+    """--path 2: the reference's GPU-variant method (recKernel, failKernel,
+    nearestKernel) on the same workload; step = build + reconstruct."""
+    cfg = CONFIGS[args.config]
+    tlr = ge.load_package()
+    W, H, spp, M = cfg["W"], cfg["H"], cfg["spp"], cfg["out_spp"]
+    print("generating scene %d (%dx%d, %d spp)" % (args.config, W, H, spp), file=sys.stderr, flush=True)
+    sbuf = tlr.make_scene(cfg["preset"], W, H, spp, seed=args.seed)
+    ctx = tlr.Ctx(0)
+    ctx.set_samples(sbuf)
+    img = np.empty((H, W, 4), np.float32)
+
+    def step():
+        ctx.build()
+        return ctx.reconstruct(out_spp=M, path=2, out=img)[1]
+
+    for i in range(args.warmup):
+        st = step()
+        print("warmup %d: build %.2f ms query %.2f ms" % (i, st["build_ms"], st["query_ms"]),
+              file=sys.stderr, flush=True)
+    tlr.lib().tlr_sync(ctx.h)
+    sts = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sts.append(step())
+    tlr.lib().tlr_sync(ctx.h)
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    print("timed: %.2f ms/step" % (dt * 1e3), file=sys.stderr, flush=True)
+    stages = ctx.query_stages(M)
+    mean = lambda k: float(np.mean([s[k] for s in sts]))
+    # counters from one more (untimed) call with statistics
+    ctx.build()
+    st = ctx.reconstruct(out_spp=M, path=2, collect_stats=True, out=img)[1]
+    first_ms = mean("fast_ms")
+    b_first = 64.0 * st["leaf_nodes"] + 28.0 * st["samples_in_leaf_nodes"] + 12.0 * st["samples_within_1r"]
+    achieved = b_first / (first_ms * 1e-3) / 1e9
+    res = {
+        "metric": "reconstructed Mpix/s @%d out-spp, 1 GPU (reference GPU-variant method)" % M,
+        "value": W * H / dt / 1e6,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded quad scene, generated on the host, uploaded before timing)",
+        "config": {"workload": cfg["name"], "width": W, "height": H, "input_spp": spp, "out_spp": M,
+                   "records": W * H * spp, "seed": args.seed, "method": "path 2"},
+        "device_ms": {"build": mean("build_ms"), "query": mean("query_ms"), "query_1r": mean("fast_ms"),
+                      "query_2r_nearest": mean("general_ms"), "combine": mean("combine_ms")},
+        "stage_fractions": {"1r": float(np.mean(stages == 0)), "2r": float(np.mean(stages == 1)),
+                            "nearest": float(np.mean(stages == 2))},
+        "roofline": {"bound": "hbm", "kernel": "k_ref_first", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic("k_ref_first", "cfg%d" % args.config),
+                     "algorithmic_bytes_per_launch": b_first, "avg_launch_ms": first_ms},
+    }
+    if not args.no_cpu:
+        print("cpu baseline ...", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(cfg, sbuf, args.cpu_rows, method="refgpu")
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +335,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gate-train", type=int, default=200, help="config 4: training scenes")
     ap.add_argument("--gate-eval", type=int, default=20, help="config 4: held-out evaluation scenes")
+    ap.add_argument("--path", type=int, default=0, choices=(0, 2),
+                    help="2: the reference's GPU-variant method instead of the plain one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (profiler counter scaling)")
     args = ap.parse_args()
@@ -279,6 +352,10 @@ def main():
         return
     if args.config == 4:
         bench_gate(args)
+        return
+
+    if args.path == 2:
+        bench_refgpu(args)
         return
 
     cfg = CONFIGS[args.config]

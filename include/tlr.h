@@ -61,7 +61,11 @@ typedef struct tlr_recon_params {
                                   2: the reference's GPU-variant method
                                      (ReconstructionCudaKernels.cu:242-868:
                                      1R first surface, 2R surfaces, nearest
-                                     sample); lens/time filters do not apply */
+                                     sample); lens/time filters do not apply;
+                                     stats: leaf_nodes, samples_in_leaf_nodes,
+                                     samples_within_1r count the first 1R
+                                     launch (leaves, samples tested, colours
+                                     read), num_2r the samples sent to 2R   */
     int   list_cap;            /* path 2, if > 0: per-lane list capacity of the first
                                   1R and 2R launches (output samples beyond it are
                                   run again with lists sized for them; results do

@@ -90,40 +90,6 @@ __device__ __forceinline__ void reproject_point(const float4 a, const float4 b, 
     }
 }
 
-// Early rejection for the per-sample test: the same reprojection with the
-// hardware reciprocal (v_rcp_f32, <= 1 ulp) instead of the two correctly
-// rounded divisions.  True only when the sample lies farther from the output
-// sample than R plus a bound on the difference to the exact computation
-// (1e-5 relative to the magnitudes involved, about 80 ulp), so the exact test
-// of ReconstructionOur.cpp:438-455 would reject it as well.  NaN results are
-// never rejected here and take the exact path.
-__device__ __forceinline__ bool surely_outside(const float4 a, const float4 b, const OSample& o, float cc0, float cc1,
-                                               float R) {
-    const float sx = a.x, sy = a.y, sw = a.z, st = a.w;
-    const float mvx = b.x, mvy = b.y, mvw = b.z;
-    float px, py, r;
-    if (mvx != 0.0f || mvy != 0.0f || mvw != 0.0f) {
-        const float dt = o.t - st;
-        const float Px = sx * sw + dt * mvx;
-        const float Py = sy * sw + dt * mvy;
-        const float Pw = 1.0f * sw + dt * mvw;
-        r = __builtin_amdgcn_rcpf(Pw);
-        px = Px * r;
-        py = Py * r;
-    } else {
-        r = __builtin_amdgcn_rcpf(sw);
-        px = sx;
-        py = sy;
-    }
-    const float c0r = cc0 * r;
-    const float coc = c0r + cc1;
-    const float dx = (px + coc * o.u) - o.x;
-    const float dy = (py + coc * o.v) - o.y;
-    const float err = 1e-5f * (fabsf(px) + fabsf(py) + fabsf(c0r) + fabsf(cc1) + fabsf(o.x) + fabsf(o.y)) + 1e-6f;
-    const float lim = R + err;
-    return dx * dx + dy * dy > lim * lim;
-}
-
 __device__ __forceinline__ void reproject_to(const Points& P, int k, const OSample& o, float cc0, float cc1,
                                              float& x, float& y) {
     reproject_point(pt_a(P, k), pt_b(P, k), o, cc0, cc1, x, y);

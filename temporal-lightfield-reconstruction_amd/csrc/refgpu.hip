@@ -48,10 +48,10 @@ struct PointList {
     __device__ __forceinline__ void put(int k, float2 v) const { p[(size_t)k * stride] = v; }
 };
 
-// Reprojection of tree sample k to the output sample (.cu:419-437).
-__device__ __forceinline__ void ref_point(const float4* __restrict__ pab, int k, const OSample& o, float cc0,
-                                          float cc1, float& dx, float& dy, float& d2) {
-    const float4 a = pab[2 * (size_t)k], b = pab[2 * (size_t)k + 1];
+// Reprojection of a tree sample (records a, b) to the output sample
+// (.cu:419-437).
+__device__ __forceinline__ void ref_point_ab(const float4 a, const float4 b, const OSample& o, float cc0, float cc1,
+                                             float& dx, float& dy, float& d2) {
     const float p0x = a.x * a.z, p0y = a.y * a.z, p0w = a.z;
     const float px = p0x + (o.t - a.w) * b.x;
     const float py = p0y + (o.t - a.w) * b.y;
@@ -129,8 +129,9 @@ __device__ __forceinline__ void sig_of(const QueryParams& Q, int id, const OSamp
 enum { R_COLOUR = 0, R_FAIL = 1, R_NEAREST = 2, R_OVERFLOW = 3 };
 
 // recKernel after the gather (.cu:332-473) over the n sorted 1R leaves.
+// tested / fetched: samples reprojected, colours read.
 __device__ __forceinline__ int ref_first_surface(const QueryParams& Q, const OSample& o, const LeafList& L, int n,
-                                                 float4& out) {
+                                                 float4& out, unsigned& tested, unsigned& fetched) {
     const float R = dispersion_for_spp(Q.spp_pow2);
     const float R2 = R * R;
     const float4* __restrict__ pab = Q.pts.pab;
@@ -140,17 +141,33 @@ __device__ __forceinline__ int ref_first_surface(const QueryParams& Q, const OSa
     for (int li = 0; li < n; li++) {
         const int2 e = L.get(li);
         const int id = e.y;
-        const int k1 = Q.leaves[id].s1;
-        int i0 = Q.leaves[id].s0;
-        bool found = false;
-        for (; i0 < k1; i0++) {
-            float dx, dy, d2;
-            ref_point(pab, i0, o, Q.cc0, Q.cc1, dx, dy, d2);
-            if (d2 < R2) {
-                found = true;
-                break;
+        const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
+        // first sample strictly within R; four records per load, samples
+        // that surely lie outside R skip the exact test (pab is padded past
+        // its last record for the group loads)
+        int i0 = k1;
+        for (int kg = k0; kg < k1 && i0 == k1; kg += 4) {
+            const float4* g = pab + 2 * (size_t)kg;
+            float4 ga[4], gb[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                ga[e] = g[2 * e];
+                gb[e] = g[2 * e + 1];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if (kg + e >= k1) break;
+                tested++;
+                if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
+                float dx, dy, d2;
+                ref_point_ab(ga[e], gb[e], o, Q.cc0, Q.cc1, dx, dy, d2);
+                if (d2 < R2) {
+                    i0 = kg + e;
+                    break;
+                }
             }
         }
+        const bool found = i0 < k1;
         if (!found) {
             L.put(li, make_int2(e.x, id | kEmpty));
         } else {
@@ -167,17 +184,33 @@ __device__ __forceinline__ int ref_first_surface(const QueryParams& Q, const OSa
             }
         }
         if (multi) break;
-        for (; i0 < k1; i0++) {
-            float dx, dy, d2;
-            ref_point(pab, i0, o, Q.cc0, Q.cc1, dx, dy, d2);
-            if (d2 <= R2) {
-                wedge(wb, dx, dy);
-                const float w = fmaxf(0.f, 1.f - sqrtf(d2) / R);
-                const float4 c = Q.pts.pc[i0];
-                cr += w * c.x;
-                cg += w * c.y;
-                cb += w * c.z;
-                ca += w * 1.0f;
+        // the rest of the leaf from that sample on, in order
+        for (int kg = i0; kg < k1; kg += 4) {
+            const float4* g = pab + 2 * (size_t)kg;
+            float4 ga[4], gb[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                ga[e] = g[2 * e];
+                gb[e] = g[2 * e + 1];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int k = kg + e;
+                if (k >= k1) break;
+                tested++;
+                if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
+                float dx, dy, d2;
+                ref_point_ab(ga[e], gb[e], o, Q.cc0, Q.cc1, dx, dy, d2);
+                if (d2 <= R2) {
+                    wedge(wb, dx, dy);
+                    const float w = fmaxf(0.f, 1.f - sqrtf(d2) / R);
+                    const float4 c = Q.pts.pc[k];
+                    fetched++;
+                    cr += w * c.x;
+                    cg += w * c.y;
+                    cb += w * c.z;
+                    ca += w * 1.0f;
+                }
             }
         }
     }
@@ -250,22 +283,35 @@ __device__ __forceinline__ int ref_fail(const QueryParams& Q, const OSample& o, 
                 if (end) break;
             }
             const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
-            for (int i = k0; i < k1; i++) {
-                float dx, dy, d2;
-                ref_point(pab, i, o, Q.cc0, Q.cc1, dx, dy, d2);
-                if (d2 <= R2) {
-                    if (np < xy_cap) xy.put(np, make_float2(dx, dy));
-                    np++;
-                    const float w = 1.f - sqrtf(d2) / disp;
-                    if (w <= 0.f) {
-                        wedge(ab, dx, dy);
-                    } else {
-                        wedge(wb, dx, dy);
-                        const float4 c = Q.pts.pc[i];
-                        cr += w * c.x;
-                        cg += w * c.y;
-                        cb += w * c.z;
-                        ca += w * 1.0f;
+            for (int kg = k0; kg < k1; kg += 4) {
+                const float4* g = pab + 2 * (size_t)kg;
+                float4 ga[4], gb[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    ga[e] = g[2 * e];
+                    gb[e] = g[2 * e + 1];
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int i = kg + e;
+                    if (i >= k1) break;
+                    if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
+                    float dx, dy, d2;
+                    ref_point_ab(ga[e], gb[e], o, Q.cc0, Q.cc1, dx, dy, d2);
+                    if (d2 <= R2) {
+                        if (np < xy_cap) xy.put(np, make_float2(dx, dy));
+                        np++;
+                        const float w = 1.f - sqrtf(d2) / disp;
+                        if (w <= 0.f) {
+                            wedge(ab, dx, dy);
+                        } else {
+                            wedge(wb, dx, dy);
+                            const float4 c = Q.pts.pc[i];
+                            cr += w * c.x;
+                            cg += w * c.y;
+                            cb += w * c.z;
+                            ca += w * 1.0f;
+                        }
                     }
                 }
             }
@@ -312,12 +358,25 @@ __device__ __forceinline__ float4 ref_nearest(const QueryParams& Q, const OSampl
                 }
                 const int id = -child - 1;
                 const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
-                for (int i = k0; i < k1; i++) {
-                    float dx, dy, d2;
-                    ref_point(pab, i, o, Q.cc0, Q.cc1, dx, dy, d2);
-                    if (d2 <= radius * radius) {
-                        radius = sqrtf(d2);
-                        best = i;
+                for (int kg = k0; kg < k1; kg += 4) {
+                    const float4* g = pab + 2 * (size_t)kg;
+                    float4 ga[4], gb[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        ga[e] = g[2 * e];
+                        gb[e] = g[2 * e + 1];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int i = kg + e;
+                        if (i >= k1) break;
+                        if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, radius)) continue;
+                        float dx, dy, d2;
+                        ref_point_ab(ga[e], gb[e], o, Q.cc0, Q.cc1, dx, dy, d2);
+                        if (d2 <= radius * radius) {
+                            radius = sqrtf(d2);
+                            best = i;
+                        }
                     }
                 }
             }
@@ -357,11 +416,16 @@ __device__ __forceinline__ void finish(const RefLaunch& R, bool active, unsigned
     }
 }
 
+// With q.collect_stats: leaves, samples tested and colours read by this
+// launch, summed into q.stats (ST_LEAF_NODES, ST_SAMPLES_IN_LEAVES,
+// ST_WITHIN_1R).
+template <bool STATS>
 __global__ __launch_bounds__(kRefFirstBlock) void k_ref_first(RefLaunch R) {
     __shared__ int2 lst[kRefFirstCap][kRefFirstBlock];
     const unsigned q = blockIdx.x * kRefFirstBlock + threadIdx.x;
     const bool active = q < R.q.num_queries;
     int r = R_COLOUR, n = 0;
+    unsigned tested = 0, fetched = 0;
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
     if (active) {
         const OSample o = make_osample(R.q, q);
@@ -373,9 +437,22 @@ __global__ __launch_bounds__(kRefFirstBlock) void k_ref_first(RefLaunch R) {
         else if (n > R.leaf_cap)
             r = R_OVERFLOW;
         else
-            r = ref_first_surface(R.q, o, L, n, col);
+            r = ref_first_surface(R.q, o, L, n, col, tested, fetched);
     }
     finish(R, active, q, r, col, 0, n, 0);
+    if (STATS) {
+        unsigned long long v[3] = {r == R_OVERFLOW ? 0ull : (unsigned long long)n, tested, fetched};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&R.q.stats[ST_LEAF_NODES], v[0]);
+            atomicAdd(&R.q.stats[ST_SAMPLES_IN_LEAVES], v[1]);
+            atomicAdd(&R.q.stats[ST_WITHIN_1R], v[2]);
+        }
+    }
 }
 
 // Persistent waves over list[0 .. *count).  STAGE 0: 1R; STAGE 1: 2R.
@@ -405,7 +482,8 @@ __global__ __launch_bounds__(kRefStageBlock) void k_ref_stage(RefLaunch R) {
             } else if (n > R.leaf_cap) {
                 r = R_OVERFLOW;
             } else if (STAGE == 0) {
-                r = ref_first_surface(R.q, o, L, n, col);
+                unsigned tested = 0, fetched = 0;
+                r = ref_first_surface(R.q, o, L, n, col, tested, fetched);
             } else {
                 r = ref_fail(R.q, o, L, n, XY, R.point_cap, col);
             }
@@ -438,7 +516,10 @@ int ref_stage_blocks(int cus, int waves_per_cu) { return std::max(1, cus * waves
 hipError_t launch_ref_first(const RefLaunch& r, hipStream_t s) {
     if (r.q.num_queries == 0) return hipSuccess;
     const unsigned blocks = (r.q.num_queries + kRefFirstBlock - 1) / kRefFirstBlock;
-    k_ref_first<<<blocks, kRefFirstBlock, 0, s>>>(r);
+    if (r.q.collect_stats)
+        k_ref_first<true><<<blocks, kRefFirstBlock, 0, s>>>(r);
+    else
+        k_ref_first<false><<<blocks, kRefFirstBlock, 0, s>>>(r);
     return hipGetLastError();
 }
 

@@ -794,7 +794,10 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         }
         if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) stats->general_ms = ms;
         stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)nfail;
-        if (p->path == 2) stats->num_2r = stats->general_queries = (double)n2r;
+        if (p->path == 2) {
+            stats->num_2r = stats->general_queries = (double)n2r;
+            stats->count = (double)nq;
+        }
         stats->total_ms = stats->build_ms + stats->query_ms + stats->combine_ms;
         stats->num_leaves = c->L;
         stats->num_samples = c->nb;
