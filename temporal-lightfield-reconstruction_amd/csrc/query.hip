@@ -111,7 +111,34 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     // --- traversal: wave-wide frontier stack ---
     if (Q.n_nodes > 0) {
         int sp = 1;
-        if (lane == 0) m.stack[0] = Q.root;
+        const int nv = (Q.n_nodes + 1) / 2;
+        if (nv >= 128) {
+            // Start at depth 6: lane l takes the node whose path from the
+            // root follows the bits of l (internal node of leaf range
+            // [lo, hi) is (lo + hi) / 2 - 1), and keeps it when it and its
+            // six ancestors are hit -- the nodes a walk from the root would
+            // reach at that depth, tested in one step instead of six.
+            int lo = 0, hi = nv;
+            bool ok = true;
+#pragma unroll
+            for (int d = 0; d < 6; d++) {
+                const int mid = (lo + hi) / 2;
+                const TravNode& an = Q.trav[mid - 1];
+                float p[12];
+#pragma unroll
+                for (int q = 0; q < 12; q++) p[q] = an.p[q];
+                ok = ok && node_hit(p, o, R);
+                if ((lane >> (5 - d)) & 1)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint64_t km = __ballot(ok);
+            if (ok) m.stack[lanes_below(km)] = (lo + hi) / 2 - 1;
+            sp = __popcll(km);
+        } else if (lane == 0) {
+            m.stack[0] = Q.root;
+        }
         wave_sync_mem();
         while (sp > 0) {
             const int take = sp < 64 ? sp : 64;
@@ -895,7 +922,17 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     __shared__ int lid[kFastLeafCap][kFastBlock];
     // Queries in scan order (pixel-major): the whole device sweeps one band
     // of rows, whose samples stay resident in the last-level cache.
-    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, blockIdx.x * kFastBlock + threadIdx.x, 0u, lid);
+    unsigned slot = blockIdx.x * kFastBlock + threadIdx.x;
+    if (Q.tile_order && !Q.fast_list) {
+        // each wave takes one output-sample index over an 8x8 pixel tile:
+        // its lanes' (u,v,t) come from one time stratum of the patterns
+        const unsigned lane = slot & 63u, grp = slot >> 6;
+        const unsigned i = grp % (unsigned)Q.M, tile = grp / (unsigned)Q.M;
+        const unsigned tiles_x = ((unsigned)Q.W + 7u) / 8u;
+        const unsigned x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
+        slot = (x < (unsigned)Q.W && y < (unsigned)Q.H) ? (y * (unsigned)Q.W + x) * (unsigned)Q.M + i : 0xFFFFFFFFu;
+    }
+    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, slot, 0u, lid);
 }
 
 // Batched fast path: one launch over every buffer of a batch.  Block b of the
@@ -1200,7 +1237,11 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
 hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     // list mode launches for the worst case; surplus lanes exit at once
-    const unsigned blocks = (p.num_queries + kFastBlock - 1) / kFastBlock;
+    unsigned blocks = (p.num_queries + kFastBlock - 1) / kFastBlock;
+    if (p.tile_order && !p.fast_list) {
+        const size_t tiles = (size_t)((p.W + 7) / 8) * ((p.H + 7) / 8);
+        blocks = (unsigned)((tiles * p.M * 64 + kFastBlock - 1) / kFastBlock);
+    }
     if (stats)
         k_query_fast<true><<<blocks, kFastBlock, 0, s>>>(p);
     else

@@ -59,6 +59,9 @@ struct QueryParams {
     float* labels;
     // direct kernel: per query features (kGateIn floats) and colours
     float* features;
+    // fast kernel over all queries: 0 = lanes in query order (pixel-major);
+    // 1 = each wave takes one output-sample index over an 8x8 pixel tile
+    int tile_order;
 };
 
 // A batch of built buffers queried by one launch of each kernel.

@@ -574,6 +574,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     Q.num_waves = waves;
     Q.fail_list = c->fail_list;
     Q.fail_count = c->d_fail_count;
+    Q.tile_order = 1;
     return TLR_OK;
 }
 

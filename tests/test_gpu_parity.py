@@ -137,6 +137,7 @@ def test_fast_path_equals_general(tlr, orc, case):
         st_n["count"] if st_n["count"] else qg.size // 4))
     assert np.array_equal(_bits(qg), _bits(qf))
     assert np.array_equal(_bits(qg), _bits(qn))
+    assert st_n["general_queries"] == st_f["general_queries"]
     for k in ("leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
               "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count"):
         assert st_g[k] == st_f[k], k
