@@ -121,9 +121,13 @@ hipError_t launch_gate_route(const float* logits, const float* features, unsigne
 // The reference's GPU-variant method (ReconstructionCudaKernels.cu:242-868):
 // stage launches over output samples and lists of them.
 constexpr int kRefFirstCap = 24;        // 1R leaves per lane in LDS (first launch)
-constexpr int kRefFailLeafCap = 96;     // 2R leaves per lane (global memory)
-constexpr int kRefFailPointCap = 192;   // points of one surface per lane (global memory)
 constexpr int kRefWavesPerCU = 8;       // resident waves of the persistent stage launches
+// 2R stage with one wave per output sample: per-wave lists in global scratch
+constexpr int kRefWaveStack = 4096;
+constexpr int kRefWaveLeaves = 2048;
+constexpr int kRefWavePoints = 4096;
+constexpr size_t kRefWaveScratchWords = (size_t)kRefWaveStack + 3 * (size_t)kRefWaveLeaves +
+                                        4 * (size_t)kRefWavePoints;
 struct RefLaunch {
     QueryParams q;             // geometry, tree, points, qcol
     const unsigned* list;      // stage input (query ids) and its length
@@ -137,15 +141,20 @@ struct RefLaunch {
     unsigned* n_over;
     unsigned* need;            // [0] leaves, [1] points: maxima over the to_over entries
     int* stage;                // optional: per query 0 (1R), 1 (2R), 2 (nearest)
-    int2* lists;               // per lane leaf_cap (key, id) pairs, lane-interleaved per wave
+    int* lists;                // per lane leaf_cap leaf ids, lane-interleaved per wave
     float2* xy;                // per lane point_cap points
     int leaf_cap, point_cap;   // per-lane list capacities (first launch: leaf_cap <= kRefFirstCap)
+    float* wscratch;           // 2R wave launch: kRefWaveScratchWords per resident wave
 };
 int ref_stage_blocks(int cus, int waves_per_cu);
 // first stage over every output sample (1R, LDS leaf lists)
 hipError_t launch_ref_first(const RefLaunch& r, hipStream_t s);
 // stage 0: 1R over r.list; stage 1: 2R over r.list (blocks of 4 waves)
 hipError_t launch_ref_stage(const RefLaunch& r, int stage, int blocks, hipStream_t s);
+// 2R stage, one wave per output sample of r.list (blocks of 4 waves); samples
+// beyond r.leaf_cap leaves or r.point_cap points (<= the kRefWave* caps) go
+// to r.to_over with their list sizes in r.need
+hipError_t launch_ref_fail_wave(const RefLaunch& r, int blocks, hipStream_t s);
 hipError_t launch_ref_nearest(const RefLaunch& r, int blocks, hipStream_t s);
 
 }  // namespace tlr

@@ -2,17 +2,21 @@
 // (ReconstructionCudaKernels.cu:242-868), driven as in
 // CudaReconstruction::reconstructGPU (ReconstructionCuda.cpp:327-465).
 //
-// One lane per output sample in every stage:
-//   k_ref_first   1R, leaf list in LDS (kRefFirstCap entries per lane);
-//                 the first surface's tent sum when its points surround the
-//                 output sample, else the sample goes to the 2R or the
-//                 nearest list;
-//   k_ref_stage   persistent waves over a list: 1R for the samples whose
-//                 leaves did not fit the LDS list (STAGE 0), or 2R surface by
-//                 surface with the triangle test (STAGE 1); per-lane leaf
-//                 and point lists in global memory, lane-interleaved per wave;
-//   k_ref_nearest the nearest reprojected sample within 1, 2, 4, 8, 16 x
-//                 the dispersion, radius shrinking during the walk.
+// Stages:
+//   k_ref_first     one lane per output sample, 1R, leaf ids in LDS
+//                   (kRefFirstCap per lane): the first surface's tent sum
+//                   when its points surround the output sample, else the
+//                   sample goes to the 2R or the nearest list;
+//   k_ref_fail_wave one wave per output sample of the 2R list: frontier
+//                   walk, rank sort, surfaces front to back with the
+//                   sameSurface checks and sample tests across the lanes,
+//                   lane-parallel triangle test;
+//   k_ref_nearest   one lane per output sample: the nearest reprojected
+//                   sample within 1, 2, 4, 8, 16 x the dispersion, radius
+//                   shrinking during the walk;
+//   k_ref_stage     one lane per output sample with per-lane lists in
+//                   global memory, sized for the samples the first two
+//                   stages could not hold (STAGE 0: 1R, STAGE 1: 2R).
 // A sample whose leaves or points exceed a launch's lists goes to an
 // overflow list and is run again with lists sized for it.  The reference
 // instead caps lists at MAX_LEAVES_R1/R2 and MAX_POINTS (.cu:39-43) and
@@ -34,12 +38,13 @@ constexpr int kRefFirstBlock = 128;
 constexpr int kRefStageBlock = 256;
 constexpr int kEmpty = (int)0x80000000;  // leaf-list flag: no point within 1R
 
-// Per-lane list of (key bits, leaf id) pairs: element k at p[k * stride].
+// Per-lane list of leaf ids (keys are recomputed from the leaf records):
+// element k at p[k * stride].
 struct LeafList {
-    int2* p;
+    int* p;
     int stride;
-    __device__ __forceinline__ int2 get(int k) const { return p[(size_t)k * stride]; }
-    __device__ __forceinline__ void put(int k, int2 v) const { p[(size_t)k * stride] = v; }
+    __device__ __forceinline__ int get(int k) const { return p[(size_t)k * stride]; }
+    __device__ __forceinline__ void put(int k, int v) const { p[(size_t)k * stride] = v; }
 };
 struct PointList {
     float2* p;
@@ -101,13 +106,13 @@ __device__ __forceinline__ int ref_collect(const QueryParams& Q, const OSample& 
                 const float key = leaf_key(Q.leaves, id, o.t);
                 int j = n;
                 while (j > 0) {
-                    const int2 e = L.get(j - 1);
-                    const float kk = __int_as_float(e.x);
-                    if (!(kk > key || (!(key > kk) && e.y > id))) break;
+                    const int e = L.get(j - 1);
+                    const float kk = leaf_key(Q.leaves, e, o.t);
+                    if (!(kk > key || (!(key > kk) && e > id))) break;
                     L.put(j, e);
                     j--;
                 }
-                L.put(j, make_int2(__float_as_int(key), id));
+                L.put(j, id);
             }
             if (SIZES) {
                 const int id = -child - 1;
@@ -139,8 +144,7 @@ __device__ __forceinline__ int ref_first_surface(const QueryParams& Q, const OSa
     float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
     bool multi = false;
     for (int li = 0; li < n; li++) {
-        const int2 e = L.get(li);
-        const int id = e.y;
+        const int id = L.get(li);
         const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
         // first sample strictly within R; four records per load, samples
         // that surely lie outside R skip the exact test (pab is padded past
@@ -169,12 +173,12 @@ __device__ __forceinline__ int ref_first_surface(const QueryParams& Q, const OSa
         }
         const bool found = i0 < k1;
         if (!found) {
-            L.put(li, make_int2(e.x, id | kEmpty));
+            L.put(li, id | kEmpty);
         } else {
             SurfSig cur, prev;
             sig_of(Q, id, o, cur);
             for (int j = 0; j < li; j++) {
-                const int pj = L.get(j).y;
+                const int pj = L.get(j);
                 if (pj & kEmpty) continue;
                 sig_of(Q, pj, o, prev);
                 if (!same_surface_sig(cur, prev)) {
@@ -268,13 +272,13 @@ __device__ __forceinline__ int ref_fail(const QueryParams& Q, const OSample& o, 
         float ab[4] = {kFltMax, -kFltMax, kFltMax, -kFltMax};
         float wb[4] = {kFltMax, -kFltMax, kFltMax, -kFltMax};
         do {
-            const int id = L.get(lidx).y;
+            const int id = L.get(lidx);
             if (np >= 4) {
                 SurfSig cur, prev;
                 sig_of(Q, id, o, cur);
                 bool end = false;
                 for (int j = sfirst; j < lidx; j++) {
-                    sig_of(Q, L.get(j).y, o, prev);
+                    sig_of(Q, L.get(j), o, prev);
                     if (!same_surface_sig(cur, prev)) {
                         end = true;
                         break;
@@ -421,7 +425,7 @@ __device__ __forceinline__ void finish(const RefLaunch& R, bool active, unsigned
 // ST_WITHIN_1R).
 template <bool STATS>
 __global__ __launch_bounds__(kRefFirstBlock) void k_ref_first(RefLaunch R) {
-    __shared__ int2 lst[kRefFirstCap][kRefFirstBlock];
+    __shared__ int lst[kRefFirstCap][kRefFirstBlock];
     const unsigned q = blockIdx.x * kRefFirstBlock + threadIdx.x;
     const bool active = q < R.q.num_queries;
     int r = R_COLOUR, n = 0;
@@ -509,6 +513,302 @@ __global__ __launch_bounds__(kRefStageBlock) void k_ref_nearest(RefLaunch R) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// failKernel with one wave per output sample (the 2R stage): the wave walks
+// the hierarchy as a frontier, sorts the leaves by (key, leaf id) by rank,
+// then follows the surfaces front to back with one leaf's samples across the
+// lanes: the sameSurface check against the surface's earlier leaves and the
+// per-sample tests run lane-parallel, the wedge boxes are lane reductions
+// (min and max do not depend on order) and the tent sum is accumulated in
+// sample order.  Per-wave lists live in global scratch (RefWaveMem).
+// ---------------------------------------------------------------------------
+struct RefWaveMem {
+    int* stack;
+    float* lkey;
+    int* lid;
+    int* sid;
+    float* px;
+    float* py;
+    int* pos;
+    int* neg;
+};
+__device__ __forceinline__ RefWaveMem ref_wave_mem(float* base) {
+    RefWaveMem m;
+    m.stack = reinterpret_cast<int*>(base);
+    m.lkey = base + kRefWaveStack;
+    m.lid = reinterpret_cast<int*>(m.lkey + kRefWaveLeaves);
+    m.sid = m.lid + kRefWaveLeaves;
+    m.px = reinterpret_cast<float*>(m.sid + kRefWaveLeaves);
+    m.py = m.px + kRefWavePoints;
+    m.pos = reinterpret_cast<int*>(m.py + kRefWavePoints);
+    m.neg = m.pos + kRefWavePoints;
+    return m;
+}
+
+__device__ __forceinline__ float wave_fmin(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_fmax(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int wuni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float wunif(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// Orders the wave's global scratch writes before its later reads.
+__device__ __forceinline__ void wave_mem_sync() { __threadfence_block(); }
+
+// checkTriangles (.cu:586-632) over the wave's np points: for each p0, the
+// later points near it on its non-negative and on its negative side are
+// listed, then every (positive, negative) pair is tested, positives across
+// the lanes.
+__device__ bool ref_triangles_wave(const RefWaveMem& m, int np, float disp) {
+    const int lane = threadIdx.x & 63;
+    if (np < 3) return false;
+    const float dcomp = 4.f * disp * disp;
+    for (int i = 0; i < np - 2; i++) {
+        const float p0x = wunif(m.px[i]), p0y = wunif(m.py[i]);
+        int npos = 0, nneg = 0;
+        for (int j0 = i + 1; j0 < np; j0 += 64) {
+            const int j = j0 + lane;
+            bool near = false, posv = false;
+            if (j < np) {
+                const float p1x = m.px[j], p1y = m.py[j];
+                const float ex = p0x - p1x, ey = p0y - p1y;
+                near = ex * ex + ey * ey < dcomp;
+                posv = near && (p0x * p1y - p1x * p0y >= 0.f);
+            }
+            const bool negv = near && !posv;
+            const uint64_t pm = __ballot(posv), nm = __ballot(negv);
+            if (posv) m.pos[npos + (int)lanes_below(pm)] = j;
+            if (negv) m.neg[nneg + (int)lanes_below(nm)] = j;
+            npos += __popcll(pm);
+            nneg += __popcll(nm);
+        }
+        wave_mem_sync();
+        if (npos == 0 || nneg == 0) continue;
+        for (int a0 = 0; a0 < npos; a0 += 64) {
+            const int a = a0 + lane;
+            float p1x = 0.f, p1y = 0.f;
+            if (a < npos) {
+                const int ia = m.pos[a];
+                p1x = m.px[ia];
+                p1y = m.py[ia];
+            }
+            bool any = false;
+            for (int b = 0; b < nneg; b++) {
+                const int kb = wuni(m.neg[b]);
+                const float p2x = wunif(m.px[kb]), p2y = wunif(m.py[kb]);
+                const float fx = p1x - p2x, fy = p1y - p2y;
+                if (a < npos && p1x * p2y >= p2x * p1y && fx * fx + fy * fy < dcomp) any = true;
+            }
+            if (__ballot(any)) return true;
+        }
+    }
+    return false;
+}
+
+// One output sample through failKernel's body; R_OVERFLOW when its leaves or
+// (where the triangle test needs them) its points exceed the wave's lists.
+__device__ int ref_fail_wave(const QueryParams& Q, const RefWaveMem& m, const OSample& o, int leaf_cap,
+                             int point_cap, float4& out) {
+    const int lane = threadIdx.x & 63;
+    const float disp = dispersion_for_spp(Q.spp_pow2);
+    const float R = disp * 2.f;
+    const float R2 = R * R;
+    // --- leaves within 2R (.cu:654-689): frontier walk, any visiting order ---
+    int nl = 0;
+    if (Q.n_nodes > 0) {
+        int sp = 1;
+        if (lane == 0) m.stack[0] = Q.root;
+        wave_mem_sync();
+        while (sp > 0) {
+            const int take = sp < 64 ? sp : 64;
+            const int base = sp - take;
+            int node = -1;
+            if (lane < take) node = m.stack[base + lane];
+            bool hit = false, internal = false;
+            int c0 = 0, c1 = 0, lidv = 0;
+            float key = 0.0f;
+            if (node >= 0) {
+                const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
+                const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
+                const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+                hit = node_hit(p, o, R);
+                if (hit) {
+                    const int child = __float_as_int(d.x);
+                    if (child >= 0) {
+                        internal = true;
+                        c0 = child;
+                        c1 = __float_as_int(d.z);
+                    } else {
+                        lidv = -child - 1;
+                        key = leaf_key(Q.leaves, lidv, o.t);
+                    }
+                }
+            }
+            const uint64_t im = __ballot(hit && internal);
+            const uint64_t lm = __ballot(hit && !internal);
+            const int ni = __popcll(im), nlh = __popcll(lm);
+            if (base + 2 * ni > kRefWaveStack || nl + nlh > leaf_cap) return R_OVERFLOW;
+            if (hit && internal) {
+                const int r = (int)lanes_below(im);
+                m.stack[base + 2 * r] = c0;
+                m.stack[base + 2 * r + 1] = c1;
+            }
+            if (hit && !internal) {
+                const int r = nl + (int)lanes_below(lm);
+                m.lkey[r] = key;
+                m.lid[r] = lidv;
+            }
+            sp = base + 2 * ni;
+            nl += nlh;
+            wave_mem_sync();
+        }
+    }
+    if (nl == 0) return R_NEAREST;
+    // --- front to back by (key, leaf id): rank of each entry ---
+    for (int j = lane; j < nl; j += 64) {
+        const float kj = m.lkey[j];
+        const int ij = m.lid[j];
+        int rank = 0;
+        for (int i = 0; i < nl; i++) {
+            const float ki = m.lkey[i];
+            const int ii = m.lid[i];
+            rank += (ki < kj || (!(kj < ki) && ii < ij)) ? 1 : 0;
+        }
+        m.sid[rank] = ij;
+    }
+    wave_mem_sync();
+    // --- surfaces (.cu:706-867) ---
+    const float RANGE = 1.0f / sqrtf(128.0f);
+    const float u0 = o.u - RANGE, v0 = o.v - RANGE, t0 = o.t - 0.5f * RANGE;
+    const float u1 = o.u + RANGE, v1 = o.v + RANGE, t1 = o.t + 0.5f * RANGE;
+    const float inf = __builtin_inff();
+    const float4* __restrict__ pab = Q.pts.pab;
+    float cr = 0.f, cg = 0.f, cb = 0.f, ca = 0.f;
+    int lidx = 0;
+    do {
+        const int sfirst = lidx;
+        int np = 0;
+        cr = cg = cb = ca = 0.f;
+        float ab0 = kFltMax, ab1 = -kFltMax, ab2 = kFltMax, ab3 = -kFltMax;
+        float wb0 = kFltMax, wb1 = -kFltMax, wb2 = kFltMax, wb3 = -kFltMax;
+        do {
+            const int id = wuni(m.sid[lidx]);
+            if (np >= 4) {
+                bool end = false;
+                const float* nb = Q.leaf_b[id].p;
+                for (int j0 = sfirst; j0 < lidx && !end; j0 += 64) {
+                    const int j = j0 + lane;
+                    bool ok = true;
+                    if (j < lidx) ok = same_surface(nb, Q.leaf_b[m.sid[j]].p, u0, v0, t0, u1, v1, t1);
+                    end = __ballot(!ok) != 0;
+                }
+                if (end) break;
+            }
+            const int s0 = wuni(Q.leaves[id].s0), s1 = wuni(Q.leaves[id].s1);
+            for (int c = s0; c < s1; c += 64) {
+                const int k = c + lane;
+                bool kept = false;
+                float dx = 0.f, dy = 0.f, d2 = 0.f;
+                if (k < s1) {
+                    ref_point_ab(pab[2 * (size_t)k], pab[2 * (size_t)k + 1], o, Q.cc0, Q.cc1, dx, dy, d2);
+                    kept = d2 <= R2;
+                }
+                const uint64_t km = __ballot(kept);
+                if (!km) continue;
+                const int pos = np + (int)lanes_below(km);
+                if (kept && pos < point_cap) {
+                    m.px[pos] = dx;
+                    m.py[pos] = dy;
+                }
+                np += __popcll(km);
+                const float w = kept ? 1.f - sqrtf(d2) / disp : 0.f;
+                const float kk = kept ? dx / dy : 0.f;
+                const bool inA = kept && w <= 0.f, inW = kept && !(w <= 0.f);
+                const bool up = dy >= 0.f;
+                ab0 = fminf(ab0, wave_fmin(inA && up ? kk : inf));
+                ab1 = fmaxf(ab1, wave_fmax(inA && up ? kk : -inf));
+                ab2 = fminf(ab2, wave_fmin(inA && !up ? kk : inf));
+                ab3 = fmaxf(ab3, wave_fmax(inA && !up ? kk : -inf));
+                wb0 = fminf(wb0, wave_fmin(inW && up ? kk : inf));
+                wb1 = fmaxf(wb1, wave_fmax(inW && up ? kk : -inf));
+                wb2 = fminf(wb2, wave_fmin(inW && !up ? kk : inf));
+                wb3 = fmaxf(wb3, wave_fmax(inW && !up ? kk : -inf));
+                // tent sum in sample order over lane-held weights and colours
+                const float4 col = inW ? Q.pts.pc[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (uint64_t cm = __ballot(inW); cm; cm &= cm - 1) {
+                    const int j = __builtin_ctzll(cm);
+                    const float wj = __shfl(w, j, 64);
+                    cr += wj * __shfl(col.x, j, 64);
+                    cg += wj * __shfl(col.y, j, 64);
+                    cb += wj * __shfl(col.z, j, 64);
+                    ca += wj * 1.0f;
+                }
+            }
+            lidx++;
+        } while (lidx < nl);
+        if (lidx < nl) {
+            ab0 = fminf(ab0, wb0);
+            ab1 = fmaxf(ab1, wb1);
+            ab2 = fminf(ab2, wb2);
+            ab3 = fmaxf(ab3, wb3);
+            if (ab1 > ab2 && ab0 < ab3) {
+                if (wb1 > wb2 && wb0 < wb3) break;
+                if (np > point_cap) return R_OVERFLOW;
+                wave_mem_sync();
+                if (ref_triangles_wave(m, np, disp)) break;
+            }
+        }
+    } while (lidx < nl);
+    if (ca > 0.f) {
+        const float s = 1.f / ca;
+        out = make_float4(cr * s, cg * s, cb * s, ca * s);
+        return R_COLOUR;
+    }
+    return R_NEAREST;
+}
+
+// Persistent waves over list[0 .. *count): one output sample per wave.
+__global__ __launch_bounds__(kRefStageBlock) void k_ref_fail_wave(RefLaunch R) {
+    const int lane = threadIdx.x & 63;
+    const size_t wave = ((size_t)blockIdx.x * kRefStageBlock + threadIdx.x) >> 6;
+    const RefWaveMem m = ref_wave_mem(R.wscratch + wave * kRefWaveScratchWords);
+    const unsigned total = *R.count;
+    for (;;) {
+        unsigned e = 0;
+        if (lane == 0) e = atomicAdd(R.next, 1u);
+        e = (unsigned)wuni((int)e);
+        if (e >= total) break;
+        const unsigned q = (unsigned)wuni((int)R.list[e]);
+        const OSample o = make_osample(R.q, q);
+        float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int r = ref_fail_wave(R.q, m, o, R.leaf_cap, R.point_cap, col);
+        if (lane == 0) {
+            if (r == R_COLOUR) {
+                R.q.qcol[q] = col;
+                if (R.stage) R.stage[q] = 1;
+            } else if (r == R_NEAREST) {
+                R.to_near[atomicAdd(R.n_near, 1u)] = q;
+            } else {
+                // lists for a rerun: this sample's leaves and their samples
+                int npts = 0;
+                const LeafList none{nullptr, 0};
+                const int n = ref_collect<true>(R.q, o, dispersion_for_spp(R.q.spp_pow2) * 2.f, none, 0, npts);
+                atomicMax(&R.need[0], (unsigned)n);
+                atomicMax(&R.need[1], (unsigned)npts);
+                R.to_over[atomicAdd(R.n_over, 1u)] = q;
+            }
+        }
+        wave_mem_sync();
+    }
+}
+
 }  // namespace
 
 int ref_stage_blocks(int cus, int waves_per_cu) { return std::max(1, cus * waves_per_cu / (kRefStageBlock / 64)); }
@@ -528,6 +828,11 @@ hipError_t launch_ref_stage(const RefLaunch& r, int stage, int blocks, hipStream
         k_ref_stage<0><<<blocks, kRefStageBlock, 0, s>>>(r);
     else
         k_ref_stage<1><<<blocks, kRefStageBlock, 0, s>>>(r);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_fail_wave(const RefLaunch& r, int blocks, hipStream_t s) {
+    k_ref_fail_wave<<<blocks, kRefStageBlock, 0, s>>>(r);
     return hipGetLastError();
 }
 

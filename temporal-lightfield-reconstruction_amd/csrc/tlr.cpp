@@ -613,8 +613,8 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
     const size_t nq = Q.num_queries;
     const int cus = c->prop.multiProcessorCount;
     const int blocks = ref_stage_blocks(cus, kRefWavesPerCU);
-    const size_t lanes = (size_t)blocks * 256;
-    const size_t fail_bytes = lanes * ((size_t)kRefFailLeafCap * sizeof(int2) + (size_t)kRefFailPointCap * 8);
+    const size_t waves = (size_t)blocks * 4;
+    const size_t fail_bytes = waves * kRefWaveScratchWords * 4;
     enum { N_FAIL, N_NEAR, N_OVER1, N_OVER2, N_OVER3, N_OVER4, NX0, NX1, NX2, NX3, NX4, NEED1, NEED1B, NEED2,
            NEED2B, NEED3, NEED3B, NEED4, NEED4B, NUM_COUNTERS };
     HIPCHK(c, c->rarena.reserve(5 * nq * 4 + fail_bytes + 16 * 256));
@@ -625,9 +625,8 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
     unsigned* over1 = c->rarena.alloc<unsigned>(nq);
     unsigned* over2 = c->rarena.alloc<unsigned>(nq);
     c->ref_stage = c->rarena.alloc<int>(nq);
-    int2* lists = c->rarena.alloc<int2>(lanes * kRefFailLeafCap);
-    float2* xy = c->rarena.alloc<float2>(lanes * kRefFailPointCap);
-    if (!xy) return c->fail(TLR_E_HIP, "reference-method arena exhausted");
+    float* wscratch = c->rarena.alloc<float>(waves * kRefWaveScratchWords);
+    if (!wscratch) return c->fail(TLR_E_HIP, "reference-method arena exhausted");
     HIPCHK(c, hipMemsetAsync(cnt, 0, NUM_COUNTERS * sizeof(unsigned), s));
 
     RefLaunch R;
@@ -643,7 +642,7 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
     auto wide = [&](int stage, unsigned* list, unsigned* count, unsigned n, const unsigned* need, unsigned* over,
                     unsigned* n_over, unsigned* need_out, unsigned* next) -> tlr_status {
         const size_t leaf_cap = std::max(1u, need[0]), point_cap = stage == 1 ? std::max(1u, need[1]) : 0;
-        const size_t per_wave = 64 * (leaf_cap * sizeof(int2) + point_cap * 8);
+        const size_t per_wave = 64 * (leaf_cap * sizeof(int) + point_cap * 8);
         const size_t budget = (size_t)1 << 30;
         size_t waves = std::min<size_t>((n + 63) / 64, (size_t)cus * kRefWavesPerCU);
         waves = std::min(waves, std::max<size_t>(1, budget / per_wave));
@@ -661,7 +660,7 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
         W.need = need_out;
         W.leaf_cap = (int)leaf_cap;
         W.point_cap = (int)point_cap;
-        W.lists = c->rwide.alloc<int2>(waves * 64 * leaf_cap);
+        W.lists = c->rwide.alloc<int>(waves * 64 * leaf_cap);
         W.xy = point_cap ? c->rwide.alloc<float2>(waves * 64 * point_cap) : nullptr;
         if (!W.lists || (point_cap && !W.xy)) return c->fail(TLR_E_HIP, "reference-method overflow arena exhausted");
         HIPCHK(c, launch_ref_stage(W, stage, (int)(waves / 4), s));
@@ -693,11 +692,10 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
     T.to_over = over1;
     T.n_over = cnt + N_OVER2;
     T.need = cnt + NEED2;
-    T.lists = lists;
-    T.xy = xy;
-    T.leaf_cap = list_cap > 0 ? std::min(list_cap, kRefFailLeafCap) : kRefFailLeafCap;
-    T.point_cap = list_cap > 0 ? std::min(list_cap, kRefFailPointCap) : kRefFailPointCap;
-    HIPCHK(c, launch_ref_stage(T, 1, blocks, s));
+    T.wscratch = wscratch;
+    T.leaf_cap = list_cap > 0 ? std::min(list_cap, kRefWaveLeaves) : kRefWaveLeaves;
+    T.point_cap = list_cap > 0 ? std::min(list_cap, kRefWavePoints) : kRefWavePoints;
+    HIPCHK(c, launch_ref_fail_wave(T, blocks, s));
     HIPCHK(c, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (h[N_OVER3])
