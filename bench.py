@@ -84,7 +84,17 @@ def cpu_baseline(cfg, sbuf, sample_rows, method="plain"):
     rows = (H + stride - 1) // stride
     est = t_build + t_filter * (H / rows)
     o.close()
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": W * H / est / 1e6, "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "build_s": t_build, "filter_s_per_frame": t_filter * (H / rows),
             "sample": "full build (%.2f s) + filter of every %d-th scanline (%d of %d rows, %.2f s); "
                       "frame time %.2f s" % (t_build, stride, rows, H, t_filter, est)}
 
@@ -269,7 +279,7 @@ def bench_refgpu(args):
 
     def step():
         ctx.build()
-        return ctx.reconstruct(out_spp=M, path=2, out=img)[1]
+        return ctx.reconstruct(out_spp=M, path=2, out=img, device_only=True)[1]
 
     for i in range(args.warmup):
         st = step()
@@ -368,9 +378,10 @@ def main():
     ctx.set_samples(sbuf)  # inputs resident in HBM before timing
     img = np.empty((H, W, 4), np.float32)
 
-    def step(stats=False):
+    # timed steps leave the image in HBM; its copy to the host is timed apart
+    def step(stats=False, to_host=False):
         ctx.build()
-        return ctx.reconstruct(out_spp=M, collect_stats=stats, out=img)[1]
+        return ctx.reconstruct(out_spp=M, collect_stats=stats, out=img, device_only=not to_host)[1]
 
     for i in range(args.warmup):
         st = step()
@@ -384,6 +395,12 @@ def main():
     tlr.lib().tlr_sync(ctx.h)
     dt = (time.perf_counter() - t0) / max(args.steps, 1)
     print("timed: %.2f ms/step" % (dt * 1e3), file=sys.stderr, flush=True)
+    # the same step with the image copied to host memory (PCIe included)
+    t0 = time.perf_counter()
+    for i in range(3):
+        step(to_host=True)
+    tlr.lib().tlr_sync(ctx.h)
+    dt_host = (time.perf_counter() - t0) / 3
     # counters come from one more (untimed) pass with statistics enabled;
     # they do not depend on timing
     st = step(stats=True)
@@ -396,6 +413,7 @@ def main():
     b_gen = b_query - b_fast
     wk = "cfg%d" % args.config
     achieved = b_fast / (fast_ms * 1e-3) / 1e9
+    traffic = read_traffic("k_query_fast", wk)
     res = {
         "metric": "reconstructed Mpix/s @%d out-spp, 1 GPU (plain path)" % M,
         "value": W * H / dt / 1e6,
@@ -412,6 +430,7 @@ def main():
         "config": {"workload": cfg["name"], "width": W, "height": H, "input_spp": spp, "out_spp": M,
                    "records": n_in, "seed": args.seed},
         "output_samples_per_s": W * H * M / dt,
+        "with_image_to_host": {"ms_per_step": dt_host * 1e3, "mpix_s": W * H / dt_host / 1e6},
         "device_ms": {"build": b_ms, "query": q_ms, "query_fast": fast_ms, "query_general": gen_ms,
                       "combine": mean("combine_ms")},
         "general_queries": st["general_queries"],
@@ -420,7 +439,8 @@ def main():
             "num_2r", "surfaces_merged", "at_least_one")},
         "roofline": {"bound": "hbm", "kernel": "k_query_fast", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic("k_query_fast", wk),
+                     "traffic": traffic,
+                     "traffic_gbs": traffic / (fast_ms * 1e-3) / 1e9 if traffic else None,
                      "algorithmic_bytes_per_launch": b_fast, "avg_launch_ms": fast_ms},
         "roofline_general": {"kernel": "k_query_general", "algorithmic_bytes_per_launch": b_gen,
                              "avg_launch_ms": gen_ms,

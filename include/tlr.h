@@ -129,7 +129,8 @@ tlr_status  tlr_build(tlr_ctx* ctx, float aperture_adjust, float focal_adjust);
 
 /* reconstructDofMotion (Reconstruction.cpp:221-295): rgba_out receives
  * width*height*4 floats (row-major, the buffer's y order); on_device selects
- * a device or host destination.  stats may be NULL. */
+ * a device or host destination.  rgba_out NULL (with out_on_device 0): the
+ * image stays in the context's device buffer.  stats may be NULL. */
 tlr_status  tlr_reconstruct(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba_out, int out_on_device,
                             tlr_stats* stats);
 

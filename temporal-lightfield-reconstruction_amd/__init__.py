@@ -288,7 +288,10 @@ class Ctx:
         self._chk(lib().tlr_build(self.h, aperture_adjust, focal_adjust))
 
     def reconstruct(self, out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,
-                    collect_stats=False, gate=False, path=0, out=None, list_cap=0):
+                    collect_stats=False, gate=False, path=0, out=None, list_cap=0, device_only=False):
+        """Reconstructs into `out` (host, H x W x 4 float32; allocated when
+        None).  device_only: the image stays in the context's device buffer
+        and (None, stats) is returned."""
         p = ReconParams()
         p.out_spp = out_spp
         p.override_uvt = 1 if override_uvt is not None else 0
@@ -301,9 +304,12 @@ class Ctx:
         p.gate = int(gate)
         p.path = int(path)
         p.list_cap = int(list_cap)
-        img = out if out is not None else np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
+        if device_only:
+            img = None
+        else:
+            img = out if out is not None else np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
         st = Stats()
-        self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img), 0, C.byref(st)))
+        self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img) if img is not None else None, 0, C.byref(st)))
         return img, st.as_dict()
 
     def batch_set_samples(self, slot, sbuf):
