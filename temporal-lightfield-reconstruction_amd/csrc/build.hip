@@ -832,6 +832,55 @@ __global__ __launch_bounds__(kBlock) void k_kd_keys(const float* __restrict__ fe
 }
 
 // Leaf slot -> leaf id after the last level.
+// Stable sort of one level's keys within chunks of whole nodes, for levels
+// whose nodes hold at most kSegSort leaves: chunk c = nodes [c*K, c*K + K)
+// (contiguous positions, as the previous level left them grouped by node),
+// sorted in LDS as (key, position in chunk) by a bitonic network.  Equals
+// the stable radix sort of the whole array: keys start with the node index.
+// Small chunks keep the network short and every CU busy.
+constexpr int kSegSort = 512;
+constexpr int kSegSortBlock = 256;
+__global__ __launch_bounds__(kSegSortBlock) void k_kd_seg_sort(const uint32_t* __restrict__ keys_in,
+                                                               const uint32_t* __restrict__ vals_in,
+                                                               uint32_t* __restrict__ keys_out,
+                                                               uint32_t* __restrict__ vals_out, int nv, int level,
+                                                               int nodes, int K) {
+    __shared__ unsigned long long sk[kSegSort];
+    const int n0 = blockIdx.x * K;
+    const int n1 = min(nodes, n0 + K) - 1;
+    int lo0, hi0, lo1, hi1;
+    kd_range(nv, level, n0, lo0, hi0);
+    kd_range(nv, level, n1, lo1, hi1);
+    const int base = lo0;
+    const int len = hi1 - lo0;
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += kSegSortBlock)
+        sk[i] = i < len ? (((unsigned long long)keys_in[base + i] << 12) | (unsigned long long)i) : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < P; i += kSegSortBlock) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const unsigned long long a = sk[i], b = sk[j];
+                    const bool up = (i & size) == 0;
+                    if ((a > b) == up) {
+                        sk[i] = b;
+                        sk[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < len; i += kSegSortBlock) {
+        const unsigned long long e = sk[i];
+        keys_out[base + i] = (uint32_t)(e >> 12);
+        vals_out[base + i] = vals_in[base + (int)(e & 0xFFFull)];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_kd_order(const uint32_t* __restrict__ vals,
                                                      const uint32_t* __restrict__ vid_leaf, int nv,
                                                      uint32_t* __restrict__ ids) {
@@ -1008,6 +1057,15 @@ hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, 
             k_kd_extent_thread<<<TLR_GRID(nodes), 0, s>>>(feat, nv, vals, level, nodes, info);
         }
         k_kd_keys<<<TLR_GRID(nv), 0, s>>>(feat, nv, vals, level, info, keys);
+        const int smax = (nv + nodes - 1) / nodes;   // largest node at this level
+        if (smax <= kSegSort) {
+            const int K = kSegSort / smax;
+            const int chunks = (nodes + K - 1) / K;
+            k_kd_seg_sort<<<chunks, kSegSortBlock, 0, s>>>(keys, vals, ko, vo, nv, level, nodes, K);
+            std::swap(keys, ko);
+            std::swap(vals, vo);
+            continue;
+        }
         const int bits = level + (level < 16 ? 16 : 32 - level);
         bool first = true;
         hipError_t e = radix_sort_pairs(keys, vals, ko, vo, (size_t)nv, bits, hist, tmp, &first, s);
