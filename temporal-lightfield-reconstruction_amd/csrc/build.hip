@@ -837,8 +837,10 @@ __global__ __launch_bounds__(kBlock) void k_kd_keys(const float* __restrict__ fe
 // (contiguous positions, as the previous level left them grouped by node),
 // sorted in LDS as (key, position in chunk) by a bitonic network.  Equals
 // the stable radix sort of the whole array: keys start with the node index.
-// Small chunks keep the network short and every CU busy.
-constexpr int kSegSort = 512;
+// Chunks of about kSegSortTarget keep the network short and every CU busy;
+// nodes of up to kSegSort leaves take a chunk of their own.
+constexpr int kSegSort = 2048;
+constexpr int kSegSortTarget = 512;
 constexpr int kSegSortBlock = 256;
 __global__ __launch_bounds__(kSegSortBlock) void k_kd_seg_sort(const uint32_t* __restrict__ keys_in,
                                                                const uint32_t* __restrict__ vals_in,
@@ -1059,7 +1061,7 @@ hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, 
         k_kd_keys<<<TLR_GRID(nv), 0, s>>>(feat, nv, vals, level, info, keys);
         const int smax = (nv + nodes - 1) / nodes;   // largest node at this level
         if (smax <= kSegSort) {
-            const int K = kSegSort / smax;
+            const int K = smax <= kSegSortTarget ? kSegSortTarget / smax : 1;
             const int chunks = (nodes + K - 1) / K;
             k_kd_seg_sort<<<chunks, kSegSortBlock, 0, s>>>(keys, vals, ko, vo, nv, level, nodes, K);
             std::swap(keys, ko);
