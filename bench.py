@@ -187,7 +187,7 @@ def bench_batch(args):
         "batch_latency_ms_p50": bdev, "batch_latency_ms_p99": bdev,
         "speedup_vs_sequential_wall": seq_wall / dt,
         "speedup_vs_sequential_device": float(sum(seq_dev)) / bdev,
-        "tuned": {k: sts[-1][k] for k in ("fast_leaf_cap", "fast_block", "general_blocks")},
+        "tuned": {k: sts[-1][k] for k in ("chained", "fast_leaf_cap", "fast_block", "general_blocks")},
         "graph_reused": sts[-1]["graph_reused"],
     }
     print(json.dumps(res))

@@ -140,13 +140,15 @@ tlr_status  tlr_sync(tlr_ctx* ctx);
 /* ---- Batched reconstruction (no reference counterpart: the reference
  * builds and filters one TreeGather at a time) ------------------------------
  * Many sample buffers of uneven size go through one build each and then one
- * query phase for all of them: a single persistent launch of each query
- * kernel over every buffer, captured in a hipGraph that is replayed while the
- * batch's shape (items, sizes, output spp) stays the same.  The first batch
- * of a context times the query kernels' launch variants on that batch and
- * keeps the fastest (tlr_batch_retune starts over).  Buffers are uploaded
- * once into numbered slots; an item names its slot.  One context runs one
- * batch at a time. */
+ * query phase for all of them, captured in a hipGraph that is replayed while
+ * the batch's shape (items, sizes, output spp) stays the same.  The query
+ * phase takes one of two forms: merged (a single persistent launch of each
+ * query kernel over every buffer) or chained (each buffer's own launches,
+ * back to back).  The first batch of a context times the merged form's
+ * launch variants and the chained form on that batch and keeps the fastest
+ * (tlr_batch_retune starts over; tlr_batch_set_mode fixes the form).
+ * Buffers are uploaded once into numbered slots; an item names its slot.  One
+ * context runs one batch at a time. */
 typedef struct tlr_batch_item {
     int    slot;                 /* filled by tlr_batch_set_samples           */
     float  aperture_adjust;      /* TreeGather's apertureAdjust (1 = none)    */
@@ -166,6 +168,7 @@ typedef struct tlr_batch_stats {
     int    fast_leaf_cap, fast_block, general_blocks;  /* tuned launch shape  */
     int    graph_reused;         /* 1: the previous call's graph was replayed */
     int    autotuned;            /* 1: this call ran the autotuner            */
+    int    chained;              /* 1: chained form, 0: merged form           */
 } tlr_batch_stats;
 
 /* Uploads buffer `slot` (AoS records as for tlr_set_samples) to device
@@ -178,6 +181,9 @@ tlr_status  tlr_reconstruct_batch(tlr_ctx* ctx, const tlr_batch_item* items, int
 tlr_status  tlr_use_slot(tlr_ctx* ctx, int slot);
 /* Forgets the tuned launch shape; the next batch tunes again. */
 tlr_status  tlr_batch_retune(tlr_ctx* ctx);
+/* Query-phase form of later batches: -1 the autotuner chooses (default),
+ * 0 merged, 1 chained.  Retunes.  TLR_E_INVALID for other values. */
+tlr_status  tlr_batch_set_mode(tlr_ctx* ctx, int mode);
 
 /* ---- Introspection for parity checks (no reference counterpart) -------- */
 tlr_status  tlr_get_build_info(tlr_ctx* ctx, tlr_build_info* out);

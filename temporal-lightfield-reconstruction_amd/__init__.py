@@ -34,7 +34,7 @@ EXPORTED = [
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
-    "tlr_batch_set_samples", "tlr_reconstruct_batch", "tlr_batch_retune", "tlr_use_slot",
+    "tlr_batch_set_samples", "tlr_reconstruct_batch", "tlr_batch_retune", "tlr_batch_set_mode", "tlr_use_slot",
 ]
 GATE_FEATURES = 24
 GATE_PARAMS = 5825
@@ -85,7 +85,7 @@ class BatchStats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("query_ms", C.c_double), ("total_ms", C.c_double),
                 ("output_samples", C.c_double), ("general_queries", C.c_double),
                 ("fast_leaf_cap", C.c_int), ("fast_block", C.c_int), ("general_blocks", C.c_int),
-                ("graph_reused", C.c_int), ("autotuned", C.c_int)]
+                ("graph_reused", C.c_int), ("autotuned", C.c_int), ("chained", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -149,6 +149,7 @@ def lib():
         L.tlr_batch_set_samples.argtypes = [V, C.c_int, C.POINTER(Header), V, C.c_int64]
         L.tlr_reconstruct_batch.argtypes = [V, C.POINTER(BatchItem), C.c_int, C.POINTER(BatchStats)]
         L.tlr_batch_retune.argtypes = [V]
+        L.tlr_batch_set_mode.argtypes = [V, C.c_int]
         L.tlr_use_slot.argtypes = [V, C.c_int]
         L.tlr_copy_query_colors.argtypes = [V, V]
         L.tlr_copy_query_stages.argtypes = [V, V]
@@ -359,6 +360,10 @@ class Ctx:
 
     def batch_retune(self):
         self._chk(lib().tlr_batch_retune(self.h))
+
+    def batch_set_mode(self, mode):
+        """Query-phase form of later batches: -1 autotuned, 0 merged, 1 chained."""
+        self._chk(lib().tlr_batch_set_mode(self.h, mode))
 
     @staticmethod
     def _params(out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,

@@ -626,7 +626,12 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
 }
 
 // Batched form: one list of hand-offs from every buffer of a batch, with the
-// buffer of each entry beside it.
+// buffer of each entry beside it.  A wave claims kBatchChunk entries at a
+// time and reads a buffer's parameters once per run of entries from it (the
+// fast kernel's waves append their hand-offs contiguously): inside a run they
+// stay in scalar registers as kernel arguments do.
+constexpr unsigned kBatchChunk = 8;
+
 __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -637,15 +642,30 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
     const unsigned nq = *B.list_count;
     for (;;) {
         unsigned e = 0;
-        if (lane == 0) e = atomicAdd(B.list_next, 1u);
+        if (lane == 0) e = atomicAdd(B.list_next, kBatchChunk);
         e = (unsigned)uni((int)e);
         if (e >= nq) break;
-        unsigned qi = (unsigned)uni((int)B.list[e]);
-        const int item = uni((int)B.list_item[e]);
-        const bool skip1r = (qi & kNeeds2R) != 0;
-        qi &= ~kNeeds2R;
-        const QueryParams Q = B.items[item];
-        general_query(Q, m, qi, skip1r, st);
+        const unsigned end = min(nq, e + kBatchChunk);
+        while (e < end) {
+            const int item = uni((int)B.list_item[e]);
+            // word by word into scalar registers (a plain struct copy lands
+            // in scratch memory)
+            QueryParams Q;
+            {
+                const int* src = reinterpret_cast<const int*>(B.items + item);
+                int* dst = reinterpret_cast<int*>(&Q);
+#pragma unroll
+                for (int k = 0; k < (int)(sizeof(QueryParams) / 4); k++)
+                    dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+            }
+            do {
+                unsigned qi = (unsigned)uni((int)B.list[e]);
+                const bool skip1r = (qi & kNeeds2R) != 0;
+                qi &= ~kNeeds2R;
+                general_query(Q, m, qi, skip1r, st);
+                e++;
+            } while (e < end && uni((int)B.list_item[e]) == item);
+        }
     }
     flush_stats(B.stats, st, false);
 }
@@ -667,8 +687,6 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
 // otherwise a leaf's scan stops at its first sample within R and the surface
 // pass stops when the second surface starts.
 // ---------------------------------------------------------------------------
-constexpr int kFastBlock = 128;
-
 // One block of the fast path: `slot` is this lane's query index (or list
 // position).  lid: the block's LDS leaf lists, CAP entries per lane.
 template <bool STATS, int CAP, int BLOCK>
@@ -916,29 +934,33 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
     }
 }
 
+// Query index of fast-kernel lane `slot` (0xFFFFFFFF: no query).  With tile
+// order each wave takes one output-sample index over an 8x8 pixel tile, so
+// its lanes' (u,v,t) come from one time stratum of the patterns; lanes of
+// tiles that overhang the image get no query.  Lists keep their own order.
+__device__ __forceinline__ unsigned tile_slot(const QueryParams& Q, unsigned slot) {
+    if (!Q.tile_order || Q.fast_list) return slot;
+    const unsigned lane = slot & 63u, grp = slot >> 6;
+    const unsigned i = grp % (unsigned)Q.M, tile = grp / (unsigned)Q.M;
+    const unsigned tiles_x = ((unsigned)Q.W + 7u) / 8u;
+    const unsigned x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
+    return (x < (unsigned)Q.W && y < (unsigned)Q.H) ? (y * (unsigned)Q.W + x) * (unsigned)Q.M + i : 0xFFFFFFFFu;
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     // per lane: leaf ids sorted by (key, id); keys are recomputed on compare
     __shared__ int lid[kFastLeafCap][kFastBlock];
     // Queries in scan order (pixel-major): the whole device sweeps one band
     // of rows, whose samples stay resident in the last-level cache.
-    unsigned slot = blockIdx.x * kFastBlock + threadIdx.x;
-    if (Q.tile_order && !Q.fast_list) {
-        // each wave takes one output-sample index over an 8x8 pixel tile:
-        // its lanes' (u,v,t) come from one time stratum of the patterns
-        const unsigned lane = slot & 63u, grp = slot >> 6;
-        const unsigned i = grp % (unsigned)Q.M, tile = grp / (unsigned)Q.M;
-        const unsigned tiles_x = ((unsigned)Q.W + 7u) / 8u;
-        const unsigned x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
-        slot = (x < (unsigned)Q.W && y < (unsigned)Q.H) ? (y * (unsigned)Q.W + x) * (unsigned)Q.M + i : 0xFFFFFFFFu;
-    }
-    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, slot, 0u, lid);
+    const unsigned slot = blockIdx.x * kFastBlock + threadIdx.x;
+    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, tile_slot(Q, slot), 0u, lid);
 }
 
 // Batched fast path: one launch over every buffer of a batch.  Block b of the
-// grid is logical block b of the concatenated buffers (each buffer's queries
-// in scan order, padded to whole blocks); its buffer is found by binary
-// search over the buffers' first blocks.
+// grid is logical block b of the concatenated buffers (each buffer's lanes in
+// tile order, padded to whole blocks); its buffer is found by binary search
+// over the buffers' first blocks.
 template <int CAP, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
     __shared__ int lid[CAP][BLOCK];
@@ -950,7 +972,8 @@ __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
         else hi = mid - 1;
     }
     const QueryParams Q = B.items[lo];
-    fast_query<false, CAP, BLOCK>(Q, (b - B.block_prefix[lo]) * BLOCK + threadIdx.x, (unsigned)lo, lid);
+    fast_query<false, CAP, BLOCK>(Q, tile_slot(Q, (b - B.block_prefix[lo]) * BLOCK + threadIdx.x), (unsigned)lo,
+                                  lid);
 }
 
 // ---------------------------------------------------------------------------
@@ -1237,11 +1260,8 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
 hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     // list mode launches for the worst case; surplus lanes exit at once
-    unsigned blocks = (p.num_queries + kFastBlock - 1) / kFastBlock;
-    if (p.tile_order && !p.fast_list) {
-        const size_t tiles = (size_t)((p.W + 7) / 8) * ((p.H + 7) / 8);
-        blocks = (unsigned)((tiles * p.M * 64 + kFastBlock - 1) / kFastBlock);
-    }
+    const size_t slots = p.fast_list ? p.num_queries : fast_slots(p);
+    const unsigned blocks = (unsigned)((slots + kFastBlock - 1) / kFastBlock);
     if (stats)
         k_query_fast<true><<<blocks, kFastBlock, 0, s>>>(p);
     else

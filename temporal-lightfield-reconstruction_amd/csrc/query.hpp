@@ -64,6 +64,15 @@ struct QueryParams {
     int tile_order;
 };
 
+// Lanes the fast kernel is launched with over all queries of Q (not a list):
+// with tile order every 8x8 pixel tile takes M whole waves, edge tiles
+// included, so the count rounds up past num_queries.
+inline size_t fast_slots(const QueryParams& Q) {
+    if (!Q.tile_order) return Q.num_queries;
+    const size_t tiles = (size_t)((Q.W + 7) / 8) * (size_t)((Q.H + 7) / 8);
+    return tiles * (size_t)Q.M * 64;
+}
+
 // A batch of built buffers queried by one launch of each kernel.
 struct BatchQuery {
     const QueryParams* items;     // per buffer (device array)
@@ -85,6 +94,7 @@ struct BatchQuery {
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
 // more 1R leaves go to the general kernel.
 constexpr int kFastLeafCap = 24;
+constexpr int kFastBlock = 128;   // block size of the single-buffer fast kernel
 static_assert(kFastLeafCap <= 64, "the fast kernel keeps a 64-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
 

@@ -139,6 +139,8 @@ struct tlr_ctx {
     std::vector<uint64_t> bkey;          // that shape
     int tuned_variant = -1;              // autotuned fast-kernel variant
     int tuned_general_blocks = 0;        // autotuned general-kernel grid
+    int tuned_chain = 0;                 // 1: per-buffer launches chained in the graph
+    int batch_mode = -1;                 // tlr_batch_set_mode: -1 autotune, 0 merged, 1 chained
 
     tlr_status fail(tlr_status s, const char* fmt, ...) {
         char buf[512];
@@ -1253,20 +1255,58 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
     Q.leaf_b = b->leaf_b;
     Q.pts = b->pts;
     Q.qcol = b->qcol;
+    Q.tile_order = 1;
     // the pattern copy reads host memory that goes away with `pat`
     HIPCHK(c, hipStreamSynchronize(s));
     return TLR_OK;
 }
 
-// Query phase of a batch with a given fast variant and general grid.
+// Batch counters, in 32-bit words: block_next, list_count (fail_count),
+// list_next, one word of padding so that the 64-bit stats start 8-byte
+// aligned, the stats (2 * kNumStats words), then a (next, fail_count) pair per
+// item for the chained form.
+constexpr int kBatchStatsWord = 4;
+constexpr int kBatchChainWord = kBatchStatsWord + 2 * kNumStats;
+
+// Query phase of a batch, merged form: one launch of each query kernel over
+// every buffer, with a given fast variant and general grid.
 static tlr_status batch_queries(tlr_ctx* c, const BatchQuery& B, int variant, int general_blocks,
                                 const std::vector<BatchSlot*>& its, unsigned* counters) {
     hipStream_t s = c->stream;
-    // counters: block_next, list_count (fail_count), list_next, stats (2 * kNumStats words)
-    HIPCHK(c, launch_fill_u32(counters, 3 + 2 * kNumStats, 0u, s));
+    HIPCHK(c, launch_fill_u32(counters, kBatchChainWord, 0u, s));
     HIPCHK(c, launch_query_fast_batch(B, variant, s));
     HIPCHK(c, launch_query_general_batch(B, general_blocks, s));
     for (BatchSlot* b : its) HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
+    return TLR_OK;
+}
+
+// Chained form: the single-buffer launches (fast kernel in tile order,
+// general kernel over its hand-offs, combine) buffer after buffer, sharing
+// the hand-off list and the general kernel's scratch.  No host work between
+// buffers, so in a graph it runs back to back.
+static tlr_status batch_chain(tlr_ctx* c, const std::vector<BatchSlot*>& its, unsigned* list, unsigned* counters,
+                              float* scratch, int waves) {
+    hipStream_t s = c->stream;
+    const int n = (int)its.size();
+    HIPCHK(c, launch_fill_u32(counters, kBatchChainWord + 2 * n, 0u, s));
+    for (int i = 0; i < n; i++) {
+        BatchSlot* b = its[i];
+        QueryParams Q = b->q;
+        unsigned* pair = counters + kBatchChainWord + 2 * i;
+        Q.next_query = pair;
+        Q.fail_list = list;
+        Q.fail_count = pair + 1;
+        Q.fail_item = nullptr;
+        Q.stats = reinterpret_cast<unsigned long long*>(counters + kBatchStatsWord);
+        Q.scratch = scratch;
+        Q.num_waves = waves;
+        Q.tile_order = 1;
+        HIPCHK(c, launch_query_fast(Q, false, s));
+        Q.qlist = list;
+        Q.qcount = pair + 1;
+        HIPCHK(c, launch_query_general(Q, s));
+        HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
+    }
     return TLR_OK;
 }
 
@@ -1301,14 +1341,15 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     for (int i = 0; i < n; i++) total_q += its[i]->nq;
     if (total_q >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "batch has too many output samples");
     const size_t bbytes = sizeof(QueryParams) * n + 4 * (size_t)(n + 1) * kNumFastVariants + 8 * total_q +
-                          4 * (3 + 2 * kNumStats) + (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
+                          4 * ((size_t)kBatchChainWord + 2 * (size_t)n) + (size_t)waves * kWaveScratchWords * 4 +
+                          16 * 256;
     HIPCHK(c, c->barena.reserve(bbytes));
     c->barena.reset();
     QueryParams* d_items = c->barena.alloc<QueryParams>(n);
     unsigned* d_prefix = c->barena.alloc<unsigned>((size_t)(n + 1) * kNumFastVariants);
     unsigned* d_list = c->barena.alloc<unsigned>(total_q);
     unsigned* d_item = c->barena.alloc<unsigned>(total_q);
-    unsigned* d_cnt = c->barena.alloc<unsigned>(3 + 2 * kNumStats);
+    unsigned* d_cnt = c->barena.alloc<unsigned>((size_t)kBatchChainWord + 2 * (size_t)n);
     float* d_scratch = c->barena.alloc<float>((size_t)waves * kWaveScratchWords);
     if (!d_scratch) return c->fail(TLR_E_HIP, "batch arena exhausted");
     std::vector<QueryParams> qp((size_t)n);
@@ -1317,7 +1358,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         qp[i].fail_list = d_list;
         qp[i].fail_count = d_cnt + 1;
         qp[i].fail_item = d_item;
-        qp[i].stats = reinterpret_cast<unsigned long long*>(d_cnt + 3);
+        qp[i].stats = reinterpret_cast<unsigned long long*>(d_cnt + kBatchStatsWord);
     }
     // block tables, one per variant (logical blocks depend on the block size)
     std::vector<unsigned> tables((size_t)(n + 1) * kNumFastVariants);
@@ -1326,7 +1367,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         unsigned acc = 0;
         for (int i = 0; i < n; i++) {
             tables[(size_t)v * (n + 1) + i] = acc;
-            acc += (unsigned)((its[i]->nq + bs_v - 1) / bs_v);
+            acc += (unsigned)((fast_slots(its[i]->q) + bs_v - 1) / bs_v);
         }
         tables[(size_t)v * (n + 1) + n] = acc;
     }
@@ -1345,48 +1386,60 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         B.list_count = d_cnt + 1;
         B.list_next = d_cnt + 2;
         B.scratch = d_scratch;
-        B.stats = reinterpret_cast<unsigned long long*>(d_cnt + 3);
+        B.stats = reinterpret_cast<unsigned long long*>(d_cnt + kBatchStatsWord);
         return B;
     };
     const int general_full = waves / 4;  // blocks of four waves
-    // autotune once per context: fast variants at the full general grid, then
-    // the general grid (full or three quarters) with the best variant
-    bool tuned_now = false;
-    if (c->tuned_general_blocks == 0) {
-        float best = 1e30f;
-        int best_v = 0;
-        for (int v = 0; v < kNumFastVariants; v++) {
-            HIPCHK(c, hipEventRecord(c->ev[2], s));
-            tlr_status st = batch_queries(c, make_bq(v), v, general_full, its, d_cnt);
-            if (st != TLR_OK) return st;
-            HIPCHK(c, hipEventRecord(c->ev[3], s));
-            HIPCHK(c, hipEventSynchronize(c->ev[3]));
-            float ms = 0;
-            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-            if (ms < best) {
-                best = ms;
-                best_v = v;
-            }
-        }
-        int best_g = general_full;
-        const int g34 = std::max(1, general_full * 3 / 4);
+    // times one query phase on the stream
+    auto timed = [&](auto&& run, float* ms) -> tlr_status {
         HIPCHK(c, hipEventRecord(c->ev[2], s));
-        tlr_status st = batch_queries(c, make_bq(best_v), best_v, g34, its, d_cnt);
+        tlr_status st = run();
         if (st != TLR_OK) return st;
         HIPCHK(c, hipEventRecord(c->ev[3], s));
         HIPCHK(c, hipEventSynchronize(c->ev[3]));
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-        if (ms < best) best_g = g34;
+        HIPCHK(c, hipEventElapsedTime(ms, c->ev[2], c->ev[3]));
+        return TLR_OK;
+    };
+    // autotune once per context: fast variants at the full general grid,
+    // then the general grid (full or three quarters) with the best variant,
+    // then the chained form against the best merged one
+    bool tuned_now = false;
+    if (c->tuned_general_blocks == 0) {
+        float best = 1e30f, ms = 0;
+        int best_v = 0, best_g = general_full, chain = c->batch_mode == 1 ? 1 : 0;
+        if (c->batch_mode != 1) {
+            for (int v = 0; v < kNumFastVariants; v++) {
+                tlr_status st = timed([&] { return batch_queries(c, make_bq(v), v, general_full, its, d_cnt); }, &ms);
+                if (st != TLR_OK) return st;
+                if (ms < best) {
+                    best = ms;
+                    best_v = v;
+                }
+            }
+            const int g34 = std::max(1, general_full * 3 / 4);
+            tlr_status st = timed([&] { return batch_queries(c, make_bq(best_v), best_v, g34, its, d_cnt); }, &ms);
+            if (st != TLR_OK) return st;
+            if (ms < best) {
+                best = ms;
+                best_g = g34;
+            }
+        }
+        if (c->batch_mode == -1) {
+            tlr_status st = timed([&] { return batch_chain(c, its, d_list, d_cnt, d_scratch, waves); }, &ms);
+            if (st != TLR_OK) return st;
+            if (ms < best) chain = 1;
+        }
         c->tuned_variant = best_v;
         c->tuned_general_blocks = best_g;
+        c->tuned_chain = chain;
         tuned_now = true;
     }
     const int v = c->tuned_variant;
     const BatchQuery B = make_bq(v);
     // graph of the query phase, keyed by everything its launches capture
     std::vector<uint64_t> key = {(uint64_t)n, (uint64_t)v, (uint64_t)c->tuned_general_blocks,
-                                 (uint64_t)(uintptr_t)c->barena.base, (uint64_t)B.total_blocks};
+                                 (uint64_t)c->tuned_chain, (uint64_t)(uintptr_t)c->barena.base,
+                                 (uint64_t)B.total_blocks};
     for (BatchSlot* b : its) {
         key.push_back((uint64_t)(uintptr_t)b->qcol);
         key.push_back((uint64_t)(uintptr_t)b->img);
@@ -1398,7 +1451,8 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         c->bgraph = nullptr;
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        tlr_status st = batch_queries(c, B, v, c->tuned_general_blocks, its, d_cnt);
+        tlr_status st = c->tuned_chain ? batch_chain(c, its, d_list, d_cnt, d_scratch, waves)
+                                       : batch_queries(c, B, v, c->tuned_general_blocks, its, d_cnt);
         hipError_t e = hipStreamEndCapture(s, &g);
         if (st != TLR_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -1420,24 +1474,28 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         HIPCHK(c, hipMemcpyAsync(it.rgba_out, its[i]->img, bytes,
                                  it.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
-    unsigned cnt[3 + 2 * kNumStats];
-    HIPCHK(c, hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    std::vector<unsigned> cnt((size_t)kBatchChainWord + 2 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), d_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     unsigned long long st64[kNumStats];
-    memcpy(st64, cnt + 3, sizeof(st64));
+    memcpy(st64, cnt.data() + kBatchStatsWord, sizeof(st64));
+    double handed = cnt[1];
+    if (c->tuned_chain)
+        for (int i = 0; i < n; i++) handed += cnt[(size_t)kBatchChainWord + 2 * (size_t)i + 1];
     if (bs) {
         memset(bs, 0, sizeof(*bs));
         float ms = 0;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) bs->query_ms = ms;
         bs->build_ms = build_ms;
         bs->total_ms = bs->build_ms + bs->query_ms;
-        bs->general_queries = (double)cnt[1];
+        bs->general_queries = handed;
         bs->output_samples = (double)total_q;
-        bs->fast_leaf_cap = fast_variant_cap(v);
-        bs->fast_block = fast_variant_block(v);
-        bs->general_blocks = c->tuned_general_blocks;
+        bs->fast_leaf_cap = c->tuned_chain ? kFastLeafCap : fast_variant_cap(v);
+        bs->fast_block = c->tuned_chain ? kFastBlock : fast_variant_block(v);
+        bs->general_blocks = c->tuned_chain ? general_full : c->tuned_general_blocks;
         bs->graph_reused = reused ? 1 : 0;
         bs->autotuned = tuned_now ? 1 : 0;
+        bs->chained = c->tuned_chain;
     }
     if (st64[ST_OVERFLOW])
         return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",
@@ -1467,7 +1525,15 @@ tlr_status tlr_batch_retune(tlr_ctx* c) {
     if (!c) return TLR_E_INVALID;
     c->tuned_variant = -1;
     c->tuned_general_blocks = 0;
+    c->tuned_chain = 0;
     return TLR_OK;
+}
+
+tlr_status tlr_batch_set_mode(tlr_ctx* c, int mode) {
+    if (!c) return TLR_E_INVALID;
+    if (mode < -1 || mode > 1) return c->fail(TLR_E_INVALID, "batch mode %d: expected -1, 0 or 1", mode);
+    c->batch_mode = mode;
+    return tlr_batch_retune(c);
 }
 
 }  // extern "C"

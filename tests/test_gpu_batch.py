@@ -1,8 +1,9 @@
 """GPU: batched reconstruction (tlr_reconstruct_batch).  Every image of a
 batch of uneven buffers equals the one the single-buffer path makes from the
-same buffer, bit for bit (same kernels, same per-query arithmetic); the
-second call of the same batch replays the captured graph; one buffer is also
-checked against the CPU oracle."""
+same buffer, bit for bit (same per-query arithmetic), in both forms of the
+query phase (merged launches, chained per-buffer launches); the second call
+of the same batch replays the captured graph; one buffer is also checked
+against the CPU oracle."""
 import numpy as np
 import pytest
 
@@ -44,6 +45,33 @@ def test_batch_equals_single_buffer_path(tlr, batch):
         assert np.array_equal(imgs[k], ref), "buffer %d differs" % k
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_forms_equal_single_buffer_path(tlr, batch, mode):
+    sbufs, _, jobs, _, _ = batch
+    ctx = tlr.Ctx(0)
+    ctx.batch_set_mode(mode)
+    for k, sb in enumerate(sbufs):
+        ctx.batch_set_samples(k, sb)
+    print("batch form %d" % mode, flush=True)
+    imgs, _, st = ctx.reconstruct_batch(jobs)
+    assert st["chained"] == mode and st["autotuned"] == 1
+    imgs2, _, st2 = ctx.reconstruct_batch(jobs)
+    assert st2["graph_reused"] == 1 and st2["chained"] == mode
+    single = tlr.Ctx(0)
+    handed = 0
+    for k, sb in enumerate(sbufs):
+        single.set_samples(sb)
+        single.build()
+        ref, sst = single.reconstruct(out_spp=BUFS[k][4])
+        handed += sst["general_queries"]
+        assert np.array_equal(imgs[k], ref), "form %d: buffer %d differs" % (mode, k)
+        assert np.array_equal(imgs2[k], ref), "form %d replay: buffer %d differs" % (mode, k)
+    if mode == 1:
+        # same fast kernel as the single-buffer path, so the same hand-offs
+        # (the merged form's tuned variant may keep fewer leaves per lane)
+        assert st["general_queries"] == handed
+
+
 def test_batch_graph_replay(batch):
     sbufs, ctx, jobs, (imgs1, st1), (imgs2, st2) = batch
     assert st2["graph_reused"] == 1 and st2["autotuned"] == 0
@@ -78,3 +106,5 @@ def test_batch_errors(tlr):
     ctx = tlr.Ctx(0)
     with pytest.raises(tlr.TlrError):
         ctx.reconstruct_batch([(3, 8)])  # empty slot
+    with pytest.raises(tlr.TlrError):
+        ctx.batch_set_mode(2)
