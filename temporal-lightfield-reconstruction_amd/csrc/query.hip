@@ -702,21 +702,52 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
     bool single = false;
     bool needs2r = false;  // failed because the reference fetches 2R
     unsigned long long s_leaves = 0, s_in_leaves = 0, s_kept = 0, s_surf = 0;
+    OSample o{0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
         const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
-        OSample o;
         o.x = tp[0] + (float)px;
         o.y = tp[1] + (float)py;
         o.u = tp[2];
         o.v = tp[3];
         o.t = tp[4];
-        const float disp = dispersion_for_spp(Q.spp_pow2);
-        const float R = disp;
+    }
+    const float disp = dispersion_for_spp(Q.spp_pow2);
+    const float R = disp;
+    // --- wave-wide descent: the deepest node whose subtree holds every leaf
+    // any lane's walk can reach.  From a node, the walk enters a child only
+    // if the lane hits the node and the child; when no lane would enter one
+    // of the two children, the wave moves to the other, keeping the lanes
+    // that hit the node (`alive`: hit every ancestor of the current node).
+    // Node loads are uniform.  Each lane then walks the subtree of `start`
+    // exactly as from the root, stopping at the subtree's escape link.
+    bool alive = active;
+    int start = -1, stop = -1;
+    if (Q.n_nodes > 0) {
+        int node = Q.root;
+        for (;;) {
+            const TravNode tn = Q.trav[node];
+            if (tn.child < 0) break;
+            const TravNode ta = Q.trav[tn.child], tb = Q.trav[tn.child1];
+            const bool hn = alive && node_hit(tn.p, o, R);
+            const uint64_t ba = __ballot(hn && node_hit(ta.p, o, R));
+            const uint64_t bb = __ballot(hn && node_hit(tb.p, o, R));
+            if (ba && bb) break;
+            if (!ba && !bb) {
+                alive = false;
+                break;
+            }
+            node = uni(ba ? tn.child : tn.child1);
+            alive = hn;
+        }
+        start = node;
+        stop = Q.trav[node].escape;
+    }
+    if (active) {
         // --- stackless traversal ---
         int n = 0;
-        int node = Q.n_nodes > 0 ? Q.root : -1;
-        while (node >= 0) {
+        int node = alive ? start : -1;
+        while (node >= 0 && node != stop) {
             const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
             const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
             const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
