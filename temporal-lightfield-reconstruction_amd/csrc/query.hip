@@ -95,6 +95,18 @@ __device__ __forceinline__ void reproject_to(const Points& P, int k, const OSamp
     reproject_point(pt_a(P, k), pt_b(P, k), o, cc0, cc1, x, y);
 }
 
+// Per-wave LDS area, shared by the gather (leaf keys, ids and sample
+// ranges of the first kStage leaves) and the triangle search that runs after
+// it (samples of one surface).
+constexpr int kStage = 128;
+constexpr int kTriCap = 200;
+constexpr int kWaveLdsWords = 3 * kTriCap + kTriCap / 2;   // tx, ty, tr (words), tl (shorts)
+static_assert(5 * kStage <= kWaveLdsWords, "gather staging must fit the wave's LDS area");
+__device__ __forceinline__ float* wave_lds() {
+    __shared__ float area[kWavesPerBlock][kWaveLdsWords];
+    return area[threadIdx.x >> 6];
+}
+
 struct Gather {
     int total;     // gathered samples
     int nsurf;     // surfaces
@@ -108,6 +120,13 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     const int lane = threadIdx.x & 63;
     Gather g{0, 0, 0, 0};
     int nl = 0;
+    // leaves found (traversal order) and sorted, the first kStage in LDS,
+    // the rest in the wave's scratch: key, id; sorted id, s0, s1
+    float* const Lkey = wave_lds();
+    int* const Lid = reinterpret_cast<int*>(Lkey + kStage);
+    int* const Sid = Lid + kStage;
+    int* const S0 = Sid + kStage;
+    int* const S1 = S0 + kStage;
     // --- traversal: wave-wide frontier stack ---
     if (Q.n_nodes > 0) {
         int sp = 1;
@@ -180,8 +199,13 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             }
             if (hit && !internal) {
                 const int r = nl + (int)lanes_below(lm);
-                m.lkey[r] = key;
-                m.lid[r] = lidv;
+                if (r < kStage) {
+                    Lkey[r] = key;
+                    Lid[r] = lidv;
+                } else {
+                    m.lkey[r] = key;
+                    m.lid[r] = lidv;
+                }
             }
             sp = base + 2 * ni;
             nl += nlh;
@@ -192,15 +216,23 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     st[ST_LEAF_NODES] += (unsigned long long)nl;
     // --- front-to-back order: rank by (key, leaf id) ---
     for (int j = lane; j < nl; j += 64) {
-        const float kj = m.lkey[j];
-        const int ij = m.lid[j];
+        const float kj = j < kStage ? Lkey[j] : m.lkey[j];
+        const int ij = j < kStage ? Lid[j] : m.lid[j];
+        const LeafRec& lr = Q.leaves[ij];
+        const int s0 = lr.s0, s1 = lr.s1;
         int rank = 0;
         for (int i = 0; i < nl; i++) {
-            const float ki = m.lkey[i];
-            const int ii = m.lid[i];
+            const float ki = i < kStage ? Lkey[i] : m.lkey[i];
+            const int ii = i < kStage ? Lid[i] : m.lid[i];
             rank += (ki < kj || (!(kj < ki) && ii < ij)) ? 1 : 0;
         }
-        m.sid[rank] = ij;
+        if (rank < kStage) {
+            Sid[rank] = ij;
+            S0[rank] = s0;
+            S1[rank] = s1;
+        } else {
+            m.sid[rank] = ij;
+        }
     }
     wave_sync_mem();
     // --- per-leaf sample tests and surface assignment ---
@@ -213,9 +245,17 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     float c_min = kFltMax;
     unsigned long long in_leaves = 0;
     for (int li = 0; li < nl; li++) {
-        const int lid = uni(m.sid[li]);
-        const LeafRec lr = Q.leaves[lid];
-        const int s0 = uni(lr.s0), s1 = uni(lr.s1);
+        int lid, s0, s1;
+        if (li < kStage) {
+            lid = uni(Sid[li]);
+            s0 = uni(S0[li]);
+            s1 = uni(S1[li]);
+        } else {
+            lid = uni(m.sid[li]);
+            const LeafRec lr = Q.leaves[lid];
+            s0 = uni(lr.s0);
+            s1 = uni(lr.s1);
+        }
         in_leaves += (unsigned long long)(s1 - s0);
         bool first = true;
         for (int c = s0; c < s1; c += 64) {
@@ -239,7 +279,7 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                 for (int j0 = sstart; nsurf > 0 && j0 < li && same; j0 += 64) {
                     const int j = j0 + lane;
                     bool ok = true;
-                    if (j < li) ok = same_surface(np, Q.leaf_b[m.sid[j]].p, u0, v0, t0, u1, v1, t1);
+                    if (j < li) ok = same_surface(np, Q.leaf_b[j < kStage ? Sid[j] : m.sid[j]].p, u0, v0, t0, u1, v1, t1);
                     same = __ballot(!ok) == 0;
                 }
                 if (!same || nsurf == 0) {
@@ -333,13 +373,12 @@ __device__ __forceinline__ void pair_of(int p, int& a, int& b) {
 //     with XNEG|YPOS);
 //   - for each m, k and l are taken from the samples j > m whose edge to m
 //     is at most 2 * dispersion (the e01 / e02 tests of :236-238).
-constexpr int kTriCap = 200;
 __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSample& o, float disp) {
-    __shared__ float tx[kWavesPerBlock][kTriCap], ty[kWavesPerBlock][kTriCap];
-    __shared__ int tr[kWavesPerBlock][kTriCap];
-    __shared__ short tl[kWavesPerBlock][kTriCap];
+    float* const tx = wave_lds();
+    float* const ty = tx + kTriCap;
+    int* const tr = reinterpret_cast<int*>(ty + kTriCap);
+    short* const tl = reinterpret_cast<short*>(tr + kTriCap);
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
     constexpr int kNear = 16;
     constexpr int kAll = XPOS | XNEG | YPOS | YNEG;
     int cls = 0;  // bit q: some sample has quadrant q (q = rpos & (XPOS|YPOS))
@@ -347,9 +386,9 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
         const float x = m.sx[start + j], y = m.sy[start + j];
         const float d = sqrtf((x - o.x) * (x - o.x) + (y - o.y) * (y - o.y));
         const int rp = m.srpos[start + j];
-        tx[w][j] = x;
-        ty[w][j] = y;
-        tr[w][j] = rp | (!(d > disp) ? kNear : 0);
+        tx[j] = x;
+        ty[j] = y;
+        tr[j] = rp | (!(d > disp) ? kNear : 0);
         cls |= 1 << (rp & (XPOS | YPOS));
     }
 #pragma unroll
@@ -362,20 +401,20 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
     const bool opposite = ((cls & (1 << 3)) && (cls & (1 << 0))) || ((cls & (1 << 1)) && (cls & (1 << 2)));
     bool any = false;
     for (int mm = 0; opposite && mm < n - 2 && !any; mm++) {
-        const float r0x = tx[w][mm], r0y = ty[w][mm];
-        const int p0 = tr[w][mm];
+        const float r0x = tx[mm], r0y = ty[mm];
+        const int p0 = tr[mm];
         // candidates j > mm with |r0 - rj| <= 2 disp, in index order
         int s = 0;
         for (int jb = mm + 1; jb < n; jb += 64) {
             const int j = jb + lane;
             bool c = false;
             if (j < n) {
-                const float rx = tx[w][j], ry = ty[w][j];
+                const float rx = tx[j], ry = ty[j];
                 const float e = sqrtf((r0x - rx) * (r0x - rx) + (r0y - ry) * (r0y - ry));
                 c = !(e > 2 * disp);
             }
             const uint64_t cm = __ballot(c);
-            if (c) tl[w][s + (int)lanes_below(cm)] = (short)j;
+            if (c) tl[s + (int)lanes_below(cm)] = (short)j;
             s += __popcll(cm);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -388,11 +427,11 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
             if (p < np) {
                 int a, b;
                 pair_of(p, a, b);
-                const int k = tl[w][a], l = tl[w][b];
-                const int p1 = tr[w][k], p2 = tr[w][l];
+                const int k = tl[a], l = tl[b];
+                const int p1 = tr[k], p2 = tr[l];
                 if (((p0 | p1 | p2) & kAll) == kAll && ((p0 | p1 | p2) & kNear)) {
-                    const float r1x = tx[w][k], r1y = ty[w][k];
-                    const float r2x = tx[w][l], r2y = ty[w][l];
+                    const float r1x = tx[k], r1y = ty[k];
+                    const float r2x = tx[l], r2y = ty[l];
                     const float e12 = sqrtf((r1x - r2x) * (r1x - r2x) + (r1y - r2y) * (r1y - r2y));
                     if (!(e12 > 2 * disp)) cov = tri_covers(r0x, r0y, r1x, r1y, r2x, r2y, o.x, o.y);
                 }
