@@ -42,14 +42,22 @@ def build_scene():
     return out
 
 
-def build_tlr():
+def build_tlr(name="libtlr.so", defines=()):
     os.makedirs(LIB, exist_ok=True)
-    out = os.path.join(LIB, "libtlr.so")
+    out = os.path.join(LIB, name)
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run(["hipcc"] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs)
+    _run(["hipcc"] + HIP_FLAGS + ["-D" + d for d in defines] + ["-I", os.path.join(ROOT, "include"), "-o", tmp] +
+         srcs)
     os.replace(tmp, out)
     return out
+
+
+def build_tlr_small_lds():
+    """Test build: the general kernel's LDS leaf staging and frontier stack at
+    their smallest, so that the paths past them run on every query
+    (tests/test_gpu_small_lds.py)."""
+    return build_tlr("libtlr_small_lds.so", ("TLR_GATHER_STAGE=4", "TLR_GATHER_LDS_STACK=64"))
 
 
 def build_facade_example():
@@ -62,7 +70,7 @@ def build_facade_example():
 
 
 def build_all():
-    return [build_scene(), build_tlr(), build_facade_example()]
+    return [build_scene(), build_tlr(), build_tlr_small_lds(), build_facade_example()]
 
 
 if __name__ == "__main__":

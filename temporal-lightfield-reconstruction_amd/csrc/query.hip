@@ -95,13 +95,24 @@ __device__ __forceinline__ void reproject_to(const Points& P, int k, const OSamp
     reproject_point(pt_a(P, k), pt_b(P, k), o, cc0, cc1, x, y);
 }
 
-// Per-wave LDS area, shared by the gather (leaf keys, ids and sample
-// ranges of the first kStage leaves) and the triangle search that runs after
-// it (samples of one surface).
-constexpr int kStage = 128;
+// Per-wave LDS area, shared by the gather (traversal frontier, leaf keys,
+// ids and sample ranges of the first kStage leaves) and the triangle search
+// that runs after it (samples of one surface).
+// (Both capacities can be set at compile time; a test build sets them tiny
+// so that every query takes the paths past them.)
+#ifndef TLR_GATHER_STAGE
+#define TLR_GATHER_STAGE 128
+#endif
+#ifndef TLR_GATHER_LDS_STACK
+#define TLR_GATHER_LDS_STACK 256
+#endif
+constexpr int kStage = TLR_GATHER_STAGE;
+constexpr int kLdsStack = TLR_GATHER_LDS_STACK;   // traversal frontier entries in LDS
+static_assert(kStage >= 1 && kLdsStack >= 64, "the depth-6 start puts up to 64 entries on the stack");
 constexpr int kTriCap = 200;
-constexpr int kWaveLdsWords = 3 * kTriCap + kTriCap / 2;   // tx, ty, tr (words), tl (shorts)
-static_assert(5 * kStage <= kWaveLdsWords, "gather staging must fit the wave's LDS area");
+constexpr int kGatherLdsWords = 5 * kStage + kLdsStack;
+constexpr int kTriLdsWords = 3 * kTriCap + kTriCap / 2;   // tx, ty, tr (words), tl (shorts)
+constexpr int kWaveLdsWords = kGatherLdsWords > kTriLdsWords ? kGatherLdsWords : kTriLdsWords;
 __device__ __forceinline__ float* wave_lds() {
     __shared__ float area[kWavesPerBlock][kWaveLdsWords];
     return area[threadIdx.x >> 6];
@@ -127,7 +138,10 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     int* const Sid = Lid + kStage;
     int* const S0 = Sid + kStage;
     int* const S1 = S0 + kStage;
-    // --- traversal: wave-wide frontier stack ---
+    // --- traversal: wave-wide frontier stack, in LDS until it outgrows
+    // kLdsStack entries, then in the wave's scratch (a generic pointer) ---
+    int* stk = S1 + kStage;
+    bool lds_stack = true;
     if (Q.n_nodes > 0) {
         int sp = 1;
         const int nv = (Q.n_nodes + 1) / 2;
@@ -153,17 +167,17 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                     hi = mid;
             }
             const uint64_t km = __ballot(ok);
-            if (ok) m.stack[lanes_below(km)] = (lo + hi) / 2 - 1;
+            if (ok) stk[lanes_below(km)] = (lo + hi) / 2 - 1;
             sp = __popcll(km);
         } else if (lane == 0) {
-            m.stack[0] = Q.root;
+            stk[0] = Q.root;
         }
         wave_sync_mem();
         while (sp > 0) {
             const int take = sp < 64 ? sp : 64;
             const int base = sp - take;
             int node = -1;
-            if (lane < take) node = m.stack[base + lane];
+            if (lane < take) node = stk[base + lane];
             bool hit = false, internal = false;
             int c0 = 0, c1 = 0, lidv = 0;
             float key = 0.0f;
@@ -192,10 +206,16 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                 g.overflow = 1;
                 break;
             }
+            if (lds_stack && base + 2 * ni > kLdsStack) {
+                // the entries still pending move to scratch
+                for (int e = lane; e < base; e += 64) m.stack[e] = stk[e];
+                stk = m.stack;
+                lds_stack = false;
+            }
             if (hit && internal) {
                 const int r = (int)lanes_below(im);
-                m.stack[base + 2 * r] = c0;
-                m.stack[base + 2 * r + 1] = c1;
+                stk[base + 2 * r] = c0;
+                stk[base + 2 * r + 1] = c1;
             }
             if (hit && !internal) {
                 const int r = nl + (int)lanes_below(lm);

@@ -1,0 +1,68 @@
+"""GPU: the general kernel past its LDS capacities.  The general kernel keeps
+the first leaves of a gather (keys, ids, sample ranges) and the traversal
+frontier in LDS and continues in the wave's scratch memory past them.  The
+test build lib/libtlr_small_lds.so sets both capacities to their minimum (4
+staged leaves, a 64-entry frontier), so those continuations run on almost
+every output sample.  Its per-output-sample colours and Stats must equal the
+oracle's bit for bit, with every output sample sent through the general
+kernel (path 1) and through the plain split (path 0).
+
+The library is loaded through TLR_LIBRARY in a child process (one process,
+both cases), since a process holds one libtlr."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SMALL_LIB = os.path.join(ROOT, "temporal-lightfield-reconstruction_amd", "lib", "libtlr_small_lds.so")
+
+CASES = [
+    # (name, preset, W, H, spp, out_spp)
+    ("defocus", 1, 96, 72, 4, 8),
+    ("strong", 3, 120, 90, 16, 8),
+]
+STATS = ("leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched", "surfaces", "num_2r",
+         "surfaces_merged", "at_least_one", "count")
+
+
+def _child():
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    tlr, orc = ge.load_package(), ge.load_oracle()
+    assert os.path.samefile(tlr.TLR_LIB, SMALL_LIB)
+    bad = 0
+    for name, preset, W, H, spp, M in CASES:
+        print("case %s" % name, flush=True)
+        sb = tlr.make_scene(preset, W, H, spp, seed=700 + preset)
+        ctx = tlr.Ctx(0)
+        ctx.set_samples(sb)
+        ctx.build()
+        o = orc.Oracle(W, H, spp, sb.coc[0], sb.coc[1], sb.records)
+        o.build()
+        _, qref, rst = o.reconstruct(out_spp=M, want_queries=True)
+        for path in (1, 0):
+            _, st = ctx.reconstruct(out_spp=M, path=path, collect_stats=True)
+            q = ctx.query_colors(M)
+            nq = int(np.count_nonzero(np.any(q.view(np.uint32) != qref.astype(np.float32).view(np.uint32), axis=-1)))
+            ns = [k for k in STATS if st[k] != rst[k]]
+            print("%s path %d: output samples differing %d / %d, stats differing %s, overflow %d" % (
+                name, path, nq, q.size // 4, ns, st["overflow"]), flush=True)
+            bad += nq + len(ns) + int(st["overflow"])
+    sys.exit(1 if bad else 0)
+
+
+@pytest.mark.gpu
+def test_general_kernel_past_lds_capacity():
+    assert os.path.exists(SMALL_LIB), "libtlr_small_lds.so is not built"
+    env = dict(os.environ, TLR_LIBRARY=SMALL_LIB)
+    r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--child"], env=env, timeout=240,
+                       capture_output=True, text=True)
+    print(r.stdout + r.stderr)
+    assert r.returncode == 0
+
+
+if __name__ == "__main__" and "--child" in sys.argv:
+    _child()
