@@ -1387,7 +1387,8 @@ hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s) {
 }
 
 // Fast-kernel variants the batch autotuner chooses from: (leaf cap, block).
-static const int kFastVariants[kNumFastVariants][2] = {{24, 128}, {16, 128}, {24, 64}, {16, 256}};
+// A cap of 22 lets the lists of 7 waves per SIMD fit the CU's 160 KB of LDS.
+static const int kFastVariants[kNumFastVariants][2] = {{22, 128}, {16, 128}, {22, 64}, {16, 256}};
 
 int fast_variant_cap(int v) { return kFastVariants[v][0]; }
 int fast_variant_block(int v) { return kFastVariants[v][1]; }
@@ -1396,9 +1397,9 @@ hipError_t launch_query_fast_batch(const BatchQuery& b, int variant, hipStream_t
     const unsigned grid = b.total_blocks;
     if (grid == 0) return hipSuccess;
     switch (variant) {
-        case 0: k_query_fast_batch<24, 128><<<grid, 128, 0, s>>>(b); break;
+        case 0: k_query_fast_batch<22, 128><<<grid, 128, 0, s>>>(b); break;
         case 1: k_query_fast_batch<16, 128><<<grid, 128, 0, s>>>(b); break;
-        case 2: k_query_fast_batch<24, 64><<<grid, 64, 0, s>>>(b); break;
+        case 2: k_query_fast_batch<22, 64><<<grid, 64, 0, s>>>(b); break;
         default: k_query_fast_batch<16, 256><<<grid, 256, 0, s>>>(b); break;
     }
     return hipGetLastError();

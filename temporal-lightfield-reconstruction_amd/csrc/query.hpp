@@ -92,8 +92,10 @@ struct BatchQuery {
 };
 
 // Capacity of the per-lane sorted leaf list of the fast kernel; queries with
-// more 1R leaves go to the general kernel.
-constexpr int kFastLeafCap = 24;
+// more 1R leaves go to the general kernel.  22 ids x 128 lanes = 11 KB of LDS
+// per block, so the 7 waves per SIMD its registers allow (14 blocks) fit the
+// CU's 160 KB (24 would hold 13 blocks).
+constexpr int kFastLeafCap = 22;
 constexpr int kFastBlock = 128;   // block size of the single-buffer fast kernel
 static_assert(kFastLeafCap <= 64, "the fast kernel keeps a 64-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
@@ -130,7 +132,7 @@ hipError_t launch_gate_route(const float* logits, const float* features, unsigne
 
 // The reference's GPU-variant method (ReconstructionCudaKernels.cu:242-868):
 // stage launches over output samples and lists of them.
-constexpr int kRefFirstCap = 24;        // 1R leaves per lane in LDS (first launch)
+constexpr int kRefFirstCap = 22;        // 1R leaves per lane in LDS (first launch): 7 waves per SIMD fit
 constexpr int kRefWavesPerCU = 8;       // resident waves of the persistent stage launches
 // 2R stage with one wave per output sample: per-wave lists in global scratch
 constexpr int kRefWaveStack = 4096;
