@@ -57,31 +57,29 @@ __device__ __forceinline__ float leaf_key(const LeafRec* leaves, int id, float t
 // GPU variant's (ReconstructionCudaKernels.cu:419-437, one division by w
 // instead of two; both are within a few ulp of the real value).  NaN results
 // are never rejected here and take the exact path.
+//
+// It runs for every sample a lane meets, so it is written for few
+// instructions: P and w as in the exact computation; one formula for static
+// samples too (with zero motion P / w is x within a few ulp, well inside the
+// bound); the later steps fused, whose rounding the bound also covers; the
+// bound's terms that depend only on the output sample are loop-invariant.
 __device__ __forceinline__ bool surely_outside(const float4 a, const float4 b, const OSample& o, float cc0, float cc1,
                                                float R) {
     const float sx = a.x, sy = a.y, sw = a.z, st = a.w;
-    const float mvx = b.x, mvy = b.y, mvw = b.z;
-    float px, py, r;
-    if (mvx != 0.0f || mvy != 0.0f || mvw != 0.0f) {
-        const float dt = o.t - st;
-        const float Px = sx * sw + dt * mvx;
-        const float Py = sy * sw + dt * mvy;
-        const float Pw = 1.0f * sw + dt * mvw;
-        r = __builtin_amdgcn_rcpf(Pw);
-        px = Px * r;
-        py = Py * r;
-    } else {
-        r = __builtin_amdgcn_rcpf(sw);
-        px = sx;
-        py = sy;
-    }
+    const float dt = o.t - st;
+    const float Px = sx * sw + dt * b.x;
+    const float Py = sy * sw + dt * b.y;
+    const float Pw = 1.0f * sw + dt * b.z;
+    const float r = __builtin_amdgcn_rcpf(Pw);
+    const float px = Px * r, py = Py * r;
     const float c0r = cc0 * r;
     const float coc = c0r + cc1;
-    const float dx = (px + coc * o.u) - o.x;
-    const float dy = (py + coc * o.v) - o.y;
-    const float err = 1e-5f * (fabsf(px) + fabsf(py) + fabsf(c0r) + fabsf(cc1) + fabsf(o.x) + fabsf(o.y)) + 1e-6f;
-    const float lim = R + err;
-    return dx * dx + dy * dy > lim * lim;
+    const float dx = __builtin_fmaf(coc, o.u, px) - o.x;
+    const float dy = __builtin_fmaf(coc, o.v, py) - o.y;
+    // R + 1e-5 (|px| + |py| + |c0r| + |cc1| + |o.x| + |o.y|) + 1e-6
+    const float slack = __builtin_fmaf(1e-5f, fabsf(cc1) + fabsf(o.x) + fabsf(o.y), R + 1e-6f);
+    const float lim = __builtin_fmaf(1e-5f, fabsf(px) + fabsf(py) + fabsf(c0r), slack);
+    return __builtin_fmaf(dx, dx, dy * dy) > lim * lim;
 }
 
 }  // namespace tlr
