@@ -803,8 +803,11 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
         stop = Q.trav[node].escape;
     }
     if (active) {
-        // --- stackless traversal ---
+        // --- stackless traversal; the LDS column holds (id, key) pairs while
+        // it has room for them (CAP / 2 leaves), ids only after that (keys
+        // then recomputed from the leaf records on compare) ---
         int n = 0;
+        bool pairs = true;
         int node = alive ? start : -1;
         while (node >= 0 && node != stop) {
             const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
@@ -822,19 +825,39 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                 }
                 const int id = -child - 1;
                 const float key = leaf_key(Q.leaves, id, o.t);
-                int j = n;
-                while (j > 0) {
-                    const int ii = lid[j - 1][tid];
-                    const float kk = leaf_key(Q.leaves, ii, o.t);
-                    if (!(kk > key || (!(key > kk) && ii > id))) break;
-                    lid[j][tid] = ii;
-                    j--;
+                if (pairs && n == CAP / 2) {
+                    // the pairs fill the column: keep the ids only
+                    for (int j = 1; j < n; j++) lid[j][tid] = lid[2 * j][tid];
+                    pairs = false;
                 }
-                lid[j][tid] = id;
+                int j = n;
+                if (pairs) {
+                    while (j > 0) {
+                        const int ii = lid[2 * j - 2][tid];
+                        const float kk = __int_as_float(lid[2 * j - 1][tid]);
+                        if (!(kk > key || (!(key > kk) && ii > id))) break;
+                        lid[2 * j][tid] = ii;
+                        lid[2 * j + 1][tid] = __float_as_int(kk);
+                        j--;
+                    }
+                    lid[2 * j][tid] = id;
+                    lid[2 * j + 1][tid] = __float_as_int(key);
+                } else {
+                    while (j > 0) {
+                        const int ii = lid[j - 1][tid];
+                        const float kk = leaf_key(Q.leaves, ii, o.t);
+                        if (!(kk > key || (!(key > kk) && ii > id))) break;
+                        lid[j][tid] = ii;
+                        j--;
+                    }
+                    lid[j][tid] = id;
+                }
                 n++;
             }
             node = escape;
         }
+        if (pairs)
+            for (int j = 1; j < n; j++) lid[j][tid] = lid[2 * j][tid];
         if (!fail) {
             s_leaves = (unsigned long long)n;
             const float RANGE = 1.0f / sqrtf(128.0f);
