@@ -38,8 +38,9 @@ constexpr int kRefFirstBlock = 128;
 constexpr int kRefStageBlock = 256;
 constexpr int kEmpty = (int)0x80000000;  // leaf-list flag: no point within 1R
 
-// Per-lane list of leaf ids (keys are recomputed from the leaf records):
-// element k at p[k * stride].
+// Per-lane list of leaf ids (ref_collect keeps (id, key) pairs while they
+// fit and compacts them to ids; keys are otherwise recomputed from the leaf
+// records): element k at p[k * stride].
 struct LeafList {
     int* p;
     int stride;
@@ -90,6 +91,9 @@ __device__ __forceinline__ int ref_collect(const QueryParams& Q, const OSample& 
                                            int cap, int& npts) {
     int n = 0;
     npts = 0;
+    // the list holds (id, key) pairs while they fit (cap / 2 leaves), ids
+    // only after that, keys then recomputed from the leaf records on compare
+    bool pairs = true;
     int node = Q.n_nodes > 0 ? Q.root : -1;
     while (node >= 0) {
         const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
@@ -104,15 +108,32 @@ __device__ __forceinline__ int ref_collect(const QueryParams& Q, const OSample& 
             if (n < cap) {
                 const int id = -child - 1;
                 const float key = leaf_key(Q.leaves, id, o.t);
-                int j = n;
-                while (j > 0) {
-                    const int e = L.get(j - 1);
-                    const float kk = leaf_key(Q.leaves, e, o.t);
-                    if (!(kk > key || (!(key > kk) && e > id))) break;
-                    L.put(j, e);
-                    j--;
+                if (pairs && n == cap / 2) {
+                    for (int j = 1; j < n; j++) L.put(j, L.get(2 * j));
+                    pairs = false;
                 }
-                L.put(j, id);
+                int j = n;
+                if (pairs) {
+                    while (j > 0) {
+                        const int e = L.get(2 * j - 2);
+                        const float kk = __int_as_float(L.get(2 * j - 1));
+                        if (!(kk > key || (!(key > kk) && e > id))) break;
+                        L.put(2 * j, e);
+                        L.put(2 * j + 1, __float_as_int(kk));
+                        j--;
+                    }
+                    L.put(2 * j, id);
+                    L.put(2 * j + 1, __float_as_int(key));
+                } else {
+                    while (j > 0) {
+                        const int e = L.get(j - 1);
+                        const float kk = leaf_key(Q.leaves, e, o.t);
+                        if (!(kk > key || (!(key > kk) && e > id))) break;
+                        L.put(j, e);
+                        j--;
+                    }
+                    L.put(j, id);
+                }
             }
             if (SIZES) {
                 const int id = -child - 1;
@@ -122,6 +143,8 @@ __device__ __forceinline__ int ref_collect(const QueryParams& Q, const OSample& 
         }
         node = escape;
     }
+    if (pairs)
+        for (int j = 1; j < min(n, cap); j++) L.put(j, L.get(2 * j));
     return n;
 }
 
@@ -424,7 +447,7 @@ __device__ __forceinline__ void finish(const RefLaunch& R, bool active, unsigned
 // launch, summed into q.stats (ST_LEAF_NODES, ST_SAMPLES_IN_LEAVES,
 // ST_WITHIN_1R).
 template <bool STATS>
-__global__ __launch_bounds__(kRefFirstBlock) void k_ref_first(RefLaunch R) {
+__global__ __launch_bounds__(kRefFirstBlock) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_ref_first(RefLaunch R) {
     __shared__ int lst[kRefFirstCap][kRefFirstBlock];
     const unsigned q = blockIdx.x * kRefFirstBlock + threadIdx.x;
     const bool active = q < R.q.num_queries;
