@@ -38,7 +38,10 @@ __device__ __forceinline__ bool node_hit(const float* p, const OSample& o, float
     const float mny = plane_eval(p + 6, o.t, o.v) - R;
     const float mxx = plane_eval(p + 3, o.t, o.u) + R;
     const float mxy = plane_eval(p + 9, o.t, o.v) + R;
-    return (o.x >= mnx && o.y >= mny && o.x <= mxx && o.y <= mxy);
+    // all four tests, not short-circuited: a node's 64 bytes then load as one
+    // request, where the first plane alone and the rest after it passes cost
+    // walks two dependent loads per node they enter
+    return (o.x >= mnx) & (o.y >= mny) & (o.x <= mxx) & (o.y <= mxy);
 }
 
 // Key of leaf id at output time t (ReconstructionOur.cpp:384-385), from the
