@@ -182,21 +182,20 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             int c0 = 0, c1 = 0, lidv = 0;
             float key = 0.0f;
             if (node >= 0) {
-                const TravNode& tn = Q.trav[node];
-                float p[12];
-#pragma unroll
-                for (int q = 0; q < 12; q++) p[q] = tn.p[q];
+                // the whole 64-byte node at once: links are taken without a
+                // branch, so they do not load after the bounds test
+                const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
+                const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
+                const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+                const int child = __float_as_int(d.x);
                 hit = node_hit(p, o, R);
-                if (hit) {
-                    if (tn.child >= 0) {
-                        internal = true;
-                        c0 = tn.child;
-                        c1 = tn.child1;
-                    } else {
-                        lidv = -tn.child - 1;
-                        const LeafRec& lr = Q.leaves[lidv];
-                        key = lr.w0 + (o.t - lr.t0) * lr.mvw0;   // ReconstructionOur.cpp:384-385
-                    }
+                internal = child >= 0;
+                c0 = child;
+                c1 = __float_as_int(d.z);
+                lidv = -child - 1;
+                if (hit && !internal) {
+                    const LeafRec& lr = Q.leaves[lidv];
+                    key = lr.w0 + (o.t - lr.t0) * lr.mvw0;   // ReconstructionOur.cpp:384-385
                 }
             }
             const uint64_t im = __ballot(hit && internal);

@@ -661,18 +661,14 @@ __device__ int ref_fail_wave(const QueryParams& Q, const RefWaveMem& m, const OS
                 const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
                 const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
                 const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+                // links taken without a branch: the node loads as one request
+                const int child = __float_as_int(d.x);
                 hit = node_hit(p, o, R);
-                if (hit) {
-                    const int child = __float_as_int(d.x);
-                    if (child >= 0) {
-                        internal = true;
-                        c0 = child;
-                        c1 = __float_as_int(d.z);
-                    } else {
-                        lidv = -child - 1;
-                        key = leaf_key(Q.leaves, lidv, o.t);
-                    }
-                }
+                internal = child >= 0;
+                c0 = child;
+                c1 = __float_as_int(d.z);
+                lidv = -child - 1;
+                if (hit && !internal) key = leaf_key(Q.leaves, lidv, o.t);
             }
             const uint64_t im = __ballot(hit && internal);
             const uint64_t lm = __ballot(hit && !internal);
