@@ -725,7 +725,7 @@ __device__ int ref_fail_wave(const QueryParams& Q, const RefWaveMem& m, const OS
                 for (int j0 = sfirst; j0 < lidx && !end; j0 += 64) {
                     const int j = j0 + lane;
                     bool ok = true;
-                    if (j < lidx) ok = same_surface(nb, Q.leaf_b[m.sid[j]].p, u0, v0, t0, u1, v1, t1);
+                    if (j < lidx) ok = same_surface_flat(nb, Q.leaf_b[m.sid[j]].p, u0, v0, t0, u1, v1, t1);
                     end = __ballot(!ok) != 0;
                 }
                 if (end) break;

@@ -270,6 +270,29 @@ TLR_HD bool same_surface_sig(const SurfSig& a, const SurfSig& b) {
     return r;
 }
 
+// same_surface evaluated without branches over both leaves' signatures (the
+// same 8 pair tests, all required): the 24 coefficients load together, where
+// testing plane by plane lets the compiler load each plane only after the one
+// before it passed.  Faster where registers are to spare (path 2's 2R stage);
+// the general kernel keeps same_surface, whose registers are not.
+TLR_HD bool same_surface_flat(const float* n0, const float* n1, float u0, float v0, float t0, float u1, float v1,
+                              float t1) {
+    float a[12], b[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+        a[q] = n0[q];
+        b[q] = n1[q];
+    }
+    SurfSig sa, sb;
+    surf_sig(a, u0, v0, t0, u1, v1, t1, sa);
+    surf_sig(b, u0, v0, t0, u1, v1, t1, sb);
+    const float T = 1 * 0.1f;
+    bool r = true;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) r = r & tern_eq2(tern_cmp(sa.v[k], sb.v[k], T), tern_cmp(sa.v[k + 1], sb.v[k + 1], T));
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Point-in-triangle with FP64 edge functions: common/EdgeFunction.cpp:39-45,
 // EdgeFunction.hpp:45-47; test at ReconstructionOur.cpp:249-258.
