@@ -293,12 +293,21 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             if (first) {
                 first = false;
                 bool same = true;
-                const float* np = Q.leaf_b[lid].p;
+                // both leaves' 12 coefficients as three 16-byte loads each,
+                // issued together ahead of the plane-by-plane test
+                const float4* np4 = reinterpret_cast<const float4*>(Q.leaf_b[lid].p);
+                const float4 na = np4[0], nb = np4[1], nc = np4[2];
+                const float np[12] = {na.x, na.y, na.z, na.w, nb.x, nb.y, nb.z, nb.w, nc.x, nc.y, nc.z, nc.w};
                 // with no surface yet a new one opens whatever the test says
                 for (int j0 = sstart; nsurf > 0 && j0 < li && same; j0 += 64) {
                     const int j = j0 + lane;
                     bool ok = true;
-                    if (j < li) ok = same_surface(np, Q.leaf_b[j < kStage ? Sid[j] : m.sid[j]].p, u0, v0, t0, u1, v1, t1);
+                    if (j < li) {
+                        const float4* b4 = reinterpret_cast<const float4*>(Q.leaf_b[j < kStage ? Sid[j] : m.sid[j]].p);
+                        const float4 ba = b4[0], bb = b4[1], bc = b4[2];
+                        const float bp[12] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w, bc.x, bc.y, bc.z, bc.w};
+                        ok = same_surface(np, bp, u0, v0, t0, u1, v1, t1);
+                    }
                     same = __ballot(!ok) == 0;
                 }
                 if (!same || nsurf == 0) {
