@@ -4,10 +4,14 @@ One step = tlr_build (reprojection, bucketing, leaves, hierarchy) + tlr_reconstr
 (all output samples, per-pixel combine) over a synthetic sample buffer that is
 already resident in HBM.  Prints one JSON line.
 
-Workload (default): 960x540, 16 input spp (8,294,400 records), 16 output
-samples per pixel, scene type 2 (50 quads, defocus C0=-15.24 C1=19.05,
-perspective motion).  --config 3 selects 1920x1080 / 400 quads + 5000
-occluders / 32 output spp.
+Workload (default, --config 4): 1920x1080, 16 input spp (33,177,600 records),
+16 output samples per pixel, one held-out seeded scene of the 960x540 scene
+type (50 quads, defocus C0=-15.24 C1=19.05, perspective motion): the plain
+path is `value`; the learned gate (parameters from gate_params.f32, trained by
+`python3 bench.py --gate-train`) is timed on the same scene beside it, with its
+error against the plain output.  --config 2 selects 960x540 (16 out-spp),
+--config 3 1920x1080 with 400 quads + 5000 occluders at 32 out-spp, --config 1
+the 128x128 4-spp case, --config 5 the 64-buffer batch.
 """
 import argparse
 import json
@@ -30,9 +34,9 @@ CONFIGS = {
             name="1920x1080 16spp in, 32 out-spp, 400 quads + 5000 occluders, strong defocus, mixed motion"),
     1: dict(preset=1, W=128, H=128, spp=4, out_spp=16,
             name="128x128 4spp in, 16 out-spp, 3 quads, defocus only"),
-    4: dict(preset=2, W=1920, H=1080, spp=16, out_spp=16,
-            name="learned gate: trained on 200 seeded 960x540 16spp scenes of the 960x540 workload type, "
-                 "evaluated on 20 held-out seeded 1920x1080 16spp scenes against the plain GPU path"),
+    4: dict(preset=2, W=1920, H=1080, spp=16, out_spp=16, seed=300000,
+            name="1920x1080 16spp in, 16 out-spp, one held-out seeded scene of the 960x540 scene type "
+                 "(50 quads, defocus C0=-15.24 C1=19.05, perspective motion); plain path, learned gate beside it"),
     5: dict(preset=2, W=0, H=0, spp=0, out_spp=16,
             name="64 buffers, resolution uniform in {128^2, 256^2, 512^2, 1024^2, 960x540, 1920x1080}, "
                  "input spp uniform in {1,4,8,16,32}, scene type of the 960x540 workload, 16 out-spp"),
@@ -100,16 +104,32 @@ def cpu_baseline(cfg, sbuf, sample_rows, method="plain"):
 
 
 def copy_calibration(mib):
-    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
-    WRITE_SIZE readings to bytes (the copy reads and writes `mib` MiB)."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
+    """Library kernels that move a known number of bytes, for scaling
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes: a streaming copy and
+    gathers of 32-byte and 64-byte records (the query kernels' sample record
+    and hierarchy node) in a scattered order (tlr_calibrate_traffic)."""
+    tlr = ge.load_package()
+    ctx = tlr.Ctx(0)
+    for kind, name in ((0, "k_calib_stream"), (1, "k_calib_gather<2>"), (2, "k_calib_gather<4>")):
+        r, w = ctx.calibrate_traffic(kind, mib << 20)
+        print(json.dumps({"kernel": name, "bytes_read": r, "bytes_written": w}), flush=True)
+
+
+GATE_PARAMS = os.path.join(ROOT, "temporal-lightfield-reconstruction_amd", "gate_params.f32")
+GATE_META = os.path.join(ROOT, "temporal-lightfield-reconstruction_amd", "gate_params.json")
+
+
+def load_gate():
+    """Committed gate parameters (5,825 little-endian float32) and their
+    threshold, or None."""
+    if not (os.path.exists(GATE_PARAMS) and os.path.exists(GATE_META)):
+        return None
+    params = np.fromfile(GATE_PARAMS, dtype="<f4")
+    with open(GATE_META) as f:
+        meta = json.load(f)
+    if params.size != 5825:
+        return None
+    return params, meta
 
 
 def bench_batch(args):
@@ -333,68 +353,42 @@ def bench_refgpu(args):
     print(json.dumps(res))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
-    ap.add_argument("--seed", type=int, default=2011)
-    ap.add_argument("--cpu-rows", type=int, default=0,
-                    help="scanlines in the CPU baseline sample (0: the whole frame)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--gate-train", type=int, default=200, help="config 4: training scenes")
-    ap.add_argument("--gate-eval", type=int, default=20, help="config 4: held-out evaluation scenes")
-    ap.add_argument("--path", type=int, default=0, choices=(0, 2),
-                    help="2: the reference's GPU-variant method instead of the plain one")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (profiler counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    if args.config == 5:
-        bench_batch(args)
-        return
-    if args.config == 4:
-        bench_gate(args)
-        return
-
-    if args.path == 2:
-        bench_refgpu(args)
-        return
-
-    cfg = CONFIGS[args.config]
+def bench_single(args, cfg):
+    """One buffer: step = build + reconstruct, image left in HBM."""
     tlr = ge.load_package()
     W, H, spp, M = cfg["W"], cfg["H"], cfg["spp"], cfg["out_spp"]
-    print("generating scene %d (%dx%d, %d spp)" % (args.config, W, H, spp), file=sys.stderr, flush=True)
-    sbuf = tlr.make_scene(cfg["preset"], W, H, spp, seed=args.seed)
+    seed = args.seed
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    log("generating %dx%d, %d spp (preset %d, seed %d)" % (W, H, spp, cfg["preset"], seed))
+    sbuf = tlr.make_scene(cfg["preset"], W, H, spp, seed=seed)
     n_in = W * H * spp
     ctx = tlr.Ctx(0)
-    ctx.set_samples(sbuf)  # inputs resident in HBM before timing
+    # host -> device upload (AoS -> SoA transpose on the host + copies), timed
+    # apart: the timed steps start with the inputs resident in HBM
+    t0 = time.perf_counter()
+    ctx.set_samples(sbuf)
+    upload_s = time.perf_counter() - t0
     img = np.empty((H, W, 4), np.float32)
 
-    # timed steps leave the image in HBM; its copy to the host is timed apart
-    def step(stats=False, to_host=False):
+    def step(stats=False, to_host=False, gate=False):
         ctx.build()
-        return ctx.reconstruct(out_spp=M, collect_stats=stats, out=img, device_only=not to_host)[1]
+        return ctx.reconstruct(out_spp=M, collect_stats=stats, gate=gate, out=img, device_only=not to_host)[1]
 
-    for i in range(args.warmup):
-        st = step()
-        print("warmup %d: build %.2f ms query %.2f ms" % (i, st["build_ms"], st["query_ms"]),
-              file=sys.stderr, flush=True)
-    tlr.lib().tlr_sync(ctx.h)
-    sts = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        sts.append(step())
-    tlr.lib().tlr_sync(ctx.h)
-    dt = (time.perf_counter() - t0) / max(args.steps, 1)
-    print("timed: %.2f ms/step" % (dt * 1e3), file=sys.stderr, flush=True)
+    def timed(gate=False):
+        for i in range(args.warmup):
+            st = step(gate=gate)
+            log("warmup %d%s: build %.2f ms query %.2f ms" % (i, " (gate)" if gate else "", st["build_ms"],
+                                                             st["query_ms"]))
+        tlr.lib().tlr_sync(ctx.h)
+        sts = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            sts.append(step(gate=gate))
+        tlr.lib().tlr_sync(ctx.h)
+        return (time.perf_counter() - t0) / max(args.steps, 1), sts
+
+    dt, sts = timed()
+    log("timed: %.2f ms/step" % (dt * 1e3))
     # the same step with the image copied to host memory (PCIe included)
     t0 = time.perf_counter()
     for i in range(3):
@@ -404,7 +398,7 @@ def main():
     # counters come from one more (untimed) pass with statistics enabled;
     # they do not depend on timing
     st = step(stats=True)
-    mean = lambda k: float(np.mean([s[k] for s in sts]))
+    mean = lambda k, ss=None: float(np.mean([x[k] for x in (ss or sts)]))
     q_ms, b_ms = mean("query_ms"), mean("build_ms")
     fast_ms, gen_ms = mean("fast_ms"), mean("general_ms")
     b_in, b_build, b_query, b_out = algorithmic_bytes(st, n_in, W, H)
@@ -428,8 +422,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded quad scene, generated on the host, uploaded before timing)",
         "config": {"workload": cfg["name"], "width": W, "height": H, "input_spp": spp, "out_spp": M,
-                   "records": n_in, "seed": args.seed},
+                   "records": n_in, "seed": seed},
         "output_samples_per_s": W * H * M / dt,
+        "upload": {"ms": upload_s * 1e3, "device_bytes": 48 * n_in,
+                   "note": "host AoS->SoA transpose + pageable copies, once per buffer, outside the timed steps",
+                   "mpix_s_including_upload_each_step": W * H / (dt + upload_s) / 1e6},
         "with_image_to_host": {"ms_per_step": dt_host * 1e3, "mpix_s": W * H / dt_host / 1e6},
         "device_ms": {"build": b_ms, "query": q_ms, "query_fast": fast_ms, "query_general": gen_ms,
                       "combine": mean("combine_ms")},
@@ -449,12 +446,85 @@ def main():
                           "roofline_ms": (b_in + b_build + b_query + b_out) / (HBM_PEAK_GBS * 1e9) * 1e3,
                           "frac": (b_in + b_build + b_query + b_out) / (HBM_PEAK_GBS * 1e9) / dt},
     }
+    if args.config == 4:
+        res["gate"] = gate_leg(args, ctx, sbuf, step, timed, dt, log)
     if not args.no_cpu:
-        print("cpu baseline ...", file=sys.stderr, flush=True)
+        log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(cfg, sbuf, args.cpu_rows)
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res))
+
+
+def gate_leg(args, ctx, sbuf, step, timed, dt_plain, log):
+    """The learned gate on the same buffer: committed parameters, timed steps,
+    error of the gated image against the plain one."""
+    g = load_gate()
+    if g is None:
+        return {"status": "no committed parameters (python3 bench.py --gate-train)"}
+    params, meta = g
+    ctx.gate_set_params(params)
+    ctx.gate_set_threshold(meta["threshold"])
+    W, H = sbuf.width, sbuf.height
+    dt, sts = timed(gate=True)
+    log("gated: %.2f ms/step" % (dt * 1e3))
+    M = CONFIGS[args.config]["out_spp"]
+    ctx.build()
+    plain, _ = ctx.reconstruct(out_spp=M)
+    gated, sg = ctx.reconstruct(out_spp=M, gate=True)
+    err = np.abs(gated - plain)[..., :3]
+    mean = lambda k: float(np.mean([x[k] for x in sts]))
+    return {"mpix_s": W * H / dt / 1e6, "ms_per_step": dt * 1e3, "speedup_vs_plain": dt_plain / dt,
+            "mean_abs_vs_plain": float(err.mean()), "max_abs_vs_plain": float(err.max()),
+            "budget_mean_abs": 1e-3, "threshold": meta["threshold"],
+            "gated_fraction": sg["gate_accepted"] / max(sg["gate_candidates"], 1.0)
+            if "gate_accepted" in sg else None,
+            "device_ms": {"build": mean("build_ms"), "query": mean("query_ms"), "fast": mean("fast_ms"),
+                          "gate": mean("gate_ms"), "general": mean("general_ms")},
+            "params": os.path.relpath(GATE_PARAMS, ROOT), "trained_by": meta.get("command")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS))
+    ap.add_argument("--seed", type=int, default=None, help="scene seed (default: the workload's)")
+    ap.add_argument("--cpu-rows", type=int, default=-1,
+                    help="scanlines in the CPU baseline sample (0: the whole frame; default: a quarter at "
+                         "1920x1080, the whole frame otherwise)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gate-train", type=int, default=0, metavar="SCENES",
+                    help="train the gate on SCENES 960x540 scenes, calibrate it, evaluate it and write "
+                         "gate_params.f32 / gate_params.json")
+    ap.add_argument("--gate-eval", type=int, default=20, help="--gate-train: held-out evaluation scenes")
+    ap.add_argument("--path", type=int, default=0, choices=(0, 2),
+                    help="2: the reference's GPU-variant method instead of the plain one")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the counter-calibration kernels over MIB MiB")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.seed is None:
+        args.seed = CONFIGS[args.config].get("seed", 2011)
+    if args.cpu_rows < 0:
+        args.cpu_rows = CONFIGS[args.config]["H"] // 4 if CONFIGS[args.config]["H"] >= 1080 else 0
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gate_train:
+        bench_gate(args)
+        return
+    if args.config == 5:
+        bench_batch(args)
+        return
+    if args.path == 2:
+        bench_refgpu(args)
+        return
+    bench_single(args, CONFIGS[args.config])
 
 
 if __name__ == "__main__":

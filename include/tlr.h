@@ -130,7 +130,8 @@ tlr_status  tlr_build(tlr_ctx* ctx, float aperture_adjust, float focal_adjust);
 /* reconstructDofMotion (Reconstruction.cpp:221-295): rgba_out receives
  * width*height*4 floats (row-major, the buffer's y order); on_device selects
  * a device or host destination.  rgba_out NULL (with out_on_device 0): the
- * image stays in the context's device buffer.  stats may be NULL. */
+ * image stays in the context's device buffer; NULL with out_on_device 1 is
+ * TLR_E_INVALID.  stats may be NULL. */
 tlr_status  tlr_reconstruct(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba_out, int out_on_device,
                             tlr_stats* stats);
 
@@ -208,6 +209,19 @@ tlr_status  tlr_copy_query_colors(tlr_ctx* ctx, float* out4);
 tlr_status  tlr_copy_query_stages(tlr_ctx* ctx, int32_t* out);
 /* output pattern: 64*M templates x,y,u,v,t (host computation) */
 int         tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, float* out5);
+
+/* ---- Profiler counter calibration (no reference counterpart) ------------
+ * One kernel launch (on the context's stream, then a sync) that moves a known
+ * number of bytes, so that rocprofv3's FETCH_SIZE / WRITE_SIZE readings of
+ * the query kernels can be scaled to bytes with the same access pattern:
+ *   kind 0: streaming copy, 16 bytes per lane, `bytes` read and written;
+ *   kind 1: gather of 32-byte records (the query kernels' per-sample record)
+ *           in a fixed pseudo-random order, `bytes` read, bytes/2 written;
+ *   kind 2: gather of 64-byte records (a hierarchy node), `bytes` read,
+ *           bytes/4 written.
+ * `bytes` is rounded down to a power of two of at least 1 MiB.  *read and
+ * *written (optional) receive the bytes the kernel moves. */
+tlr_status  tlr_calibrate_traffic(tlr_ctx* ctx, int kind, int64_t bytes, int64_t* read, int64_t* written);
 
 /* ---- Learned gate (no reference counterpart) ------------------------------
  * A 24-64-64-1 MLP (BF16 operands, FP32 accumulation on MFMA) scores each

@@ -30,7 +30,7 @@ EXPORTED = [
     "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_sync",
     "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
     "tlr_copy_leaves", "tlr_copy_hierarchy", "tlr_copy_query_colors", "tlr_copy_query_stages",
-    "tlr_output_pattern",
+    "tlr_output_pattern", "tlr_calibrate_traffic",
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
@@ -155,6 +155,7 @@ def lib():
         L.tlr_copy_query_stages.argtypes = [V, V]
         L.tlr_output_pattern.restype = C.c_int
         L.tlr_output_pattern.argtypes = [C.c_int, C.c_int, V, V]
+        L.tlr_calibrate_traffic.argtypes = [V, C.c_int, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.tlr_gate_set_params.argtypes = [V, V]
         L.tlr_gate_get_params.argtypes = [V, V]
         L.tlr_gate_set_threshold.argtypes = [V, C.c_float]
@@ -346,6 +347,14 @@ class Ctx:
         st = BatchStats()
         self._chk(lib().tlr_reconstruct_batch(self.h, items, n, C.byref(st)))
         return imgs, [bms[k] for k in range(n)], st.as_dict()
+
+    def calibrate_traffic(self, kind, nbytes):
+        """One kernel moving a known number of bytes (profiler counter
+        scaling): kind 0 streaming copy, 1 / 2 gathers of 32 / 64-byte
+        records.  Returns (bytes read, bytes written)."""
+        r, w = C.c_int64(0), C.c_int64(0)
+        self._chk(lib().tlr_calibrate_traffic(self.h, kind, nbytes, C.byref(r), C.byref(w)))
+        return r.value, w.value
 
     def use_slot(self, slot):
         """The context's input becomes batch slot `slot` (for build/reconstruct)."""

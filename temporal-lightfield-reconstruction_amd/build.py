@@ -60,6 +60,13 @@ def build_tlr_small_lds():
     return build_tlr("libtlr_small_lds.so", ("TLR_GATHER_STAGE=4", "TLR_GATHER_LDS_STACK=64"))
 
 
+def build_tlr_tiny_cap():
+    """Test build: the general kernel holds at most 8 leaves per output
+    sample, so ordinary scenes reach the TLR_E_CAPACITY error path
+    (tests/test_gpu_edges.py)."""
+    return build_tlr("libtlr_tiny_cap.so", ("TLR_LEAF_CAP=8",))
+
+
 def build_facade_example():
     """C++ caller of the facade (tests/cpp/facade_example.cpp), host compiler only."""
     out = os.path.join(LIB, "facade_example")
@@ -70,7 +77,7 @@ def build_facade_example():
 
 
 def build_all():
-    return [build_scene(), build_tlr(), build_tlr_small_lds(), build_facade_example()]
+    return [build_scene(), build_tlr(), build_tlr_small_lds(), build_tlr_tiny_cap(), build_facade_example()]
 
 
 if __name__ == "__main__":

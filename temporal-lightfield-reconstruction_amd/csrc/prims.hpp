@@ -9,17 +9,27 @@
 namespace tlr {
 
 // Scratch allocator owned by a context: bump allocation inside one device
-// buffer that grows on demand between builds (never during one).
+// buffer that grows on demand between builds (never during one).  A failed
+// allocation is sticky: `failed` stays set until the next reset(), so a
+// caller can check once after a group of allocations (a later, smaller one
+// succeeding does not hide an earlier failure).
 struct DeviceArena {
     char* base = nullptr;
     size_t cap = 0;
     size_t top = 0;
+    bool failed = false;
     hipError_t reserve(size_t bytes);
-    void reset() { top = 0; }
+    void reset() {
+        top = 0;
+        failed = false;
+    }
     template <class T>
     T* alloc(size_t n) {
         size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
-        if (top + bytes > cap) return nullptr;
+        if (failed || top + bytes > cap) {
+            failed = true;
+            return nullptr;
+        }
         T* p = reinterpret_cast<T*>(base + top);
         top += bytes;
         return p;

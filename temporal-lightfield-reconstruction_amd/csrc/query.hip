@@ -1346,6 +1346,31 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ q
     reinterpret_cast<float4*>(out)[p] = o;
 }
 
+// Counter calibration kernels: a streaming copy and gathers of 32-byte and
+// 64-byte records in a fixed scattered order (i * odd mod n, a bijection for
+// n a power of two).
+__global__ __launch_bounds__(256) void k_calib_stream(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                      size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+template <int REC>
+__global__ __launch_bounds__(256) void k_calib_gather(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                      size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const size_t j = (i * (size_t)0x9E3779B1u) & (n - 1);
+    const float4* r = src + (size_t)REC * j;
+    float4 acc = r[0];
+#pragma unroll
+    for (int e = 1; e < REC; e++) {
+        const float4 v = r[e];
+        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+    }
+    dst[i] = acc;
+}
+
 // Graph-safe clear of small counters (no memset nodes in captured graphs).
 __global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -1410,6 +1435,18 @@ hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out, h
     return hipGetLastError();
 }
 
+
+hipError_t launch_calibrate(int kind, const float4* src, float4* dst, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (kind == 0)
+        k_calib_stream<<<blocks, 256, 0, s>>>(src, dst, n);
+    else if (kind == 1)
+        k_calib_gather<2><<<blocks, 256, 0, s>>>(src, dst, n);
+    else
+        k_calib_gather<4><<<blocks, 256, 0, s>>>(src, dst, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s) {
     if (n <= 0) return hipSuccess;

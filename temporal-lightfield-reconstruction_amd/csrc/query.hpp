@@ -8,8 +8,13 @@
 namespace tlr {
 
 // Capacities of one wave's scratch (per query).  A query that exceeds them
-// is reported as an overflow and the call fails with TLR_E_CAPACITY.
-constexpr int kLeafCap = 4096;
+// is reported as an overflow and the call fails with TLR_E_CAPACITY.  (The
+// leaf capacity can be set at compile time; a test build sets it tiny so
+// that the capacity error is reached on ordinary scenes.)
+#ifndef TLR_LEAF_CAP
+#define TLR_LEAF_CAP 4096
+#endif
+constexpr int kLeafCap = TLR_LEAF_CAP;
 constexpr int kSampleCap = 32768;   // gathered samples + merged-surface copies
 constexpr int kStackCap = 8192;
 
@@ -125,6 +130,10 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
 // surface clustering, plus kGateIn features; writes qcol and features.
 hipError_t launch_query_direct(const QueryParams& p, hipStream_t s);
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
+// Counter calibration: kind 0 streams n float4 from src to dst; kinds 1 / 2
+// gather n records of 2 / 4 float4 from src in the order (i * odd) mod n (n
+// a power of two) and write one float4 per record to dst.
+hipError_t launch_calibrate(int kind, const float4* src, float4* dst, size_t n, hipStream_t s);
 // Routing after the gate: queries whose logit is below the threshold, or
 // with no gathered sample, are appended to list (for the plain path).
 hipError_t launch_gate_route(const float* logits, const float* features, unsigned n, float threshold,

@@ -374,7 +374,6 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     HIPCHK(c, c->arena.reserve(bytes));
     DeviceArena& A = c->arena;
     A.reset();
-    auto need = [&](void* p) { return p != nullptr; };
 
     c->rx = A.alloc<float>(N); c->ry = A.alloc<float>(N); c->rw = A.alloc<float>(N); c->rt = A.alloc<float>(N);
     uint32_t* k0 = A.alloc<uint32_t>(N);
@@ -384,7 +383,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     int* hits = A.alloc<int>((size_t)W * H);
     uint32_t* hist = A.alloc<uint32_t>(hist_words);
     uint32_t* stmp = A.alloc<uint32_t>(scan_words + 64);
-    if (!need(c->rt) || !need(v1) || !need(hits) || !need(stmp)) return c->fail(TLR_E_HIP, "arena exhausted");
+    if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted");
 
     // ---- reprojection + grid ----
     HIPCHK(c, hipEventRecord(c->ev[0], s));
@@ -414,7 +413,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     uint32_t* own_flag = A.alloc<uint32_t>(cells);
     int* own_y1 = A.alloc<int>(cells);
     uint32_t* own_idx = A.alloc<uint32_t>(cells + 1);
-    if (!need(own_idx)) return c->fail(TLR_E_HIP, "arena exhausted (grid)");
+    if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (grid)");
 
     // ---- bucketing: stable sort of (cell, input index) ----
     HIPCHK(c, hipMemsetAsync(counts, 0, sizeof(uint32_t) * cells, s));
@@ -439,7 +438,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     c->cand = A.alloc<uint32_t>(std::max(nb, 1));
     float* ct = A.alloc<float>(std::max(nb, 1));
     float* cw = A.alloc<float>(std::max(nb, 1));
-    if (!need(cw)) return c->fail(TLR_E_HIP, "arena exhausted (candidates)");
+    if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (candidates)");
     launch_emit_initial(owner, own_flag, own_y1, own_idx, sat, gw, gh, c->ileaf_rect, c->ileaf_oc, s);
     launch_place(skeys, svals, nb, gw, owner, sat, cell_start, c->rt, c->rw, c->cand, ct, cw, s);
     launch_leaf_sort(c->ileaf_oc, nil, c->cand, ct, cw, s);
@@ -455,7 +454,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     float4* boxes = A.alloc<float4>(2 * (size_t)nb1);
     uint32_t* sflag = A.alloc<uint32_t>(nb1 + 1);
     uint32_t* lpos = A.alloc<uint32_t>(nb1 + 1);
-    if (!need(lpos)) return c->fail(TLR_E_HIP, "arena exhausted (points)");
+    if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (points)");
     launch_points(c->cand, nb, in, c->rx, c->ry, c->rw, c->rt, (float)W, (float)H, c->cc0, c->cc1, P, sb, boxes, s);
     HIPCHK(c, hipMemsetAsync(sflag, 0, sizeof(uint32_t) * (nb1 + 1), s));
     HIPCHK(c, hipMemsetAsync(P.pab + 2 * (size_t)nb, 0, sizeof(float4) * 2 * kPointPad, s));
@@ -469,7 +468,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     c->leaf_b = A.alloc<Bounds>(L1);
     uint32_t* vflag = A.alloc<uint32_t>(L1 + 1);
     uint32_t* vpos = A.alloc<uint32_t>(L1 + 1);
-    if (!need(vpos)) return c->fail(TLR_E_HIP, "arena exhausted (leaves)");
+    if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (leaves)");
     launch_leaf_emit(sflag, lpos, nb, sb, P, c->leaves, c->leaf_b, vflag, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, exclusive_scan_u32(vflag, vpos, vpos + L, (size_t)L, stmp, s));
@@ -495,7 +494,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
     int* split_owner = A.alloc<int>(nv1);
     c->tree_b = A.alloc<Bounds>(nv1);
     c->trav = A.alloc<TravNode>(2 * (size_t)nv1);
-    if (!need(c->trav)) return c->fail(TLR_E_HIP, "arena exhausted (hierarchy)");
+    if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (hierarchy)");
     if (nv > 0) {
         HIPCHK(c, hipMemsetAsync(parent, 0xFF, sizeof(int) * (2 * (size_t)nv), s));
         HIPCHK(c, launch_kdtree(vflag, vpos, L, c->leaf_b, nv, feat, vid_leaf, tk0, tv0, tk1, tv1, kinfo, kmn, kmx, hist, stmp,
@@ -542,9 +541,8 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
         c->logits = c->qarena.alloc<float>(nq);
         c->labels = c->qarena.alloc<float>(nq);
         c->route_list = c->qarena.alloc<unsigned>(nq);
-        if (!c->route_list) return c->fail(TLR_E_HIP, "query arena exhausted");
     }
-    if (!c->scratch) return c->fail(TLR_E_HIP, "query arena exhausted");
+    if (c->qarena.failed) return c->fail(TLR_E_HIP, "query arena exhausted");
     c->qcol_n = nq;
     c->num_waves = waves;
     c->last_M = M;
@@ -576,7 +574,10 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     Q.num_waves = waves;
     Q.fail_list = c->fail_list;
     Q.fail_count = c->d_fail_count;
+    // tile order pads every 8x8 tile to whole waves; the fast kernel's lane
+    // index is 32-bit, so very narrow images fall back to query order
     Q.tile_order = 1;
+    if (fast_slots(Q) >= ((size_t)1 << 32)) Q.tile_order = 0;
     return TLR_OK;
 }
 
@@ -628,7 +629,7 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
     unsigned* over2 = c->rarena.alloc<unsigned>(nq);
     c->ref_stage = c->rarena.alloc<int>(nq);
     float* wscratch = c->rarena.alloc<float>(waves * kRefWaveScratchWords);
-    if (!wscratch) return c->fail(TLR_E_HIP, "reference-method arena exhausted");
+    if (c->rarena.failed) return c->fail(TLR_E_HIP, "reference-method arena exhausted");
     HIPCHK(c, hipMemsetAsync(cnt, 0, NUM_COUNTERS * sizeof(unsigned), s));
 
     RefLaunch R;
@@ -664,7 +665,7 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
         W.point_cap = (int)point_cap;
         W.lists = c->rwide.alloc<int>(waves * 64 * leaf_cap);
         W.xy = point_cap ? c->rwide.alloc<float2>(waves * 64 * point_cap) : nullptr;
-        if (!W.lists || (point_cap && !W.xy)) return c->fail(TLR_E_HIP, "reference-method overflow arena exhausted");
+        if (c->rwide.failed) return c->fail(TLR_E_HIP, "reference-method overflow arena exhausted");
         HIPCHK(c, launch_ref_stage(W, stage, (int)(waves / 4), s));
         return TLR_OK;
     };
@@ -723,6 +724,7 @@ static tlr_status run_refgpu(tlr_ctx* c, const QueryParams& Q, int list_cap, uns
 
 tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, int out_on_device, tlr_stats* stats) {
     if (!c || !p) return TLR_E_INVALID;
+    if (out_on_device && !out) return c->fail(TLR_E_INVALID, "out_on_device set with a NULL rgba_out");
     if (p->gate && !c->has_gate) return c->fail(TLR_E_STATE, "gate requested but no gate parameters set");
     if (p->path < 0 || p->path > 2) return c->fail(TLR_E_INVALID, "unknown path %d", p->path);
     if (p->path == 2 && p->gate) return c->fail(TLR_E_INVALID, "the gate applies to the plain method only");
@@ -931,6 +933,29 @@ tlr_status tlr_copy_query_stages(tlr_ctx* c, int32_t* out) {
     if (!c || !out) return TLR_E_INVALID;
     if (!c->ref_stage) return c->fail(TLR_E_STATE, "the last reconstruction did not use path 2");
     HIPCHK(c, hipMemcpy(out, c->ref_stage, c->qcol_n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_calibrate_traffic(tlr_ctx* c, int kind, int64_t bytes, int64_t* read, int64_t* written) {
+    if (!c) return TLR_E_INVALID;
+    if (kind < 0 || kind > 2) return c->fail(TLR_E_INVALID, "calibration kind %d: expected 0, 1 or 2", kind);
+    if (bytes < ((int64_t)1 << 20)) return c->fail(TLR_E_INVALID, "calibration size below 1 MiB");
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t b = (int64_t)1 << 20;
+    while (b * 2 <= bytes) b *= 2;
+    const int rec = kind == 0 ? 1 : (kind == 1 ? 2 : 4);     // float4 per record
+    const size_t n = (size_t)b / (16 * (size_t)rec);          // records (a power of two)
+    float4 *src = nullptr, *dst = nullptr;
+    HIPCHK(c, hipMalloc(&src, (size_t)b));
+    hipError_t e = hipMalloc(&dst, n * sizeof(float4));
+    if (e == hipSuccess) e = hipMemsetAsync(src, 0, (size_t)b, c->stream);
+    if (e == hipSuccess) e = launch_calibrate(kind, src, dst, n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e != hipSuccess) return c->fail(TLR_E_HIP, "calibration: %s", hipGetErrorString(e));
+    if (read) *read = b;
+    if (written) *written = (int64_t)(n * sizeof(float4));
     return TLR_OK;
 }
 
@@ -1229,7 +1254,7 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
     b->pattern = b->prod.alloc<float>(pat.size());
     b->qcol = b->prod.alloc<float4>(nq);
     b->img = b->prod.alloc<float>((size_t)W * H * 4);
-    if (!b->img) return c->fail(TLR_E_HIP, "batch slot arena exhausted");
+    if (b->prod.failed) return c->fail(TLR_E_HIP, "batch slot arena exhausted");
     hipStream_t s = c->stream;
     HIPCHK(c, hipMemcpyAsync(b->pts.pab, c->pts.pab, sizeof(float4) * 2 * (nb + kPointPad), hipMemcpyDeviceToDevice, s));
     if (nb) HIPCHK(c, hipMemcpyAsync(b->pts.pc, c->pts.pc, sizeof(float4) * nb, hipMemcpyDeviceToDevice, s));
@@ -1256,6 +1281,7 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
     Q.pts = b->pts;
     Q.qcol = b->qcol;
     Q.tile_order = 1;
+    if (fast_slots(Q) >= ((size_t)1 << 32)) Q.tile_order = 0;
     // the pattern copy reads host memory that goes away with `pat`
     HIPCHK(c, hipStreamSynchronize(s));
     return TLR_OK;
@@ -1300,7 +1326,6 @@ static tlr_status batch_chain(tlr_ctx* c, const std::vector<BatchSlot*>& its, un
         Q.stats = reinterpret_cast<unsigned long long*>(counters + kBatchStatsWord);
         Q.scratch = scratch;
         Q.num_waves = waves;
-        Q.tile_order = 1;
         HIPCHK(c, launch_query_fast(Q, false, s));
         Q.qlist = list;
         Q.qcount = pair + 1;
@@ -1351,7 +1376,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     unsigned* d_item = c->barena.alloc<unsigned>(total_q);
     unsigned* d_cnt = c->barena.alloc<unsigned>((size_t)kBatchChainWord + 2 * (size_t)n);
     float* d_scratch = c->barena.alloc<float>((size_t)waves * kWaveScratchWords);
-    if (!d_scratch) return c->fail(TLR_E_HIP, "batch arena exhausted");
+    if (c->barena.failed) return c->fail(TLR_E_HIP, "batch arena exhausted");
     std::vector<QueryParams> qp((size_t)n);
     for (int i = 0; i < n; i++) {
         qp[i] = its[i]->q;
@@ -1436,14 +1461,22 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     }
     const int v = c->tuned_variant;
     const BatchQuery B = make_bq(v);
-    // graph of the query phase, keyed by everything its launches capture
+    // graph of the query phase, keyed by everything its launches capture:
+    // the chained form passes each item's QueryParams by value (CoC
+    // coefficients, spp, tree size and root, every device pointer), so all
+    // of its bytes are part of the key, as are the shared buffers' addresses
     std::vector<uint64_t> key = {(uint64_t)n, (uint64_t)v, (uint64_t)c->tuned_general_blocks,
                                  (uint64_t)c->tuned_chain, (uint64_t)(uintptr_t)c->barena.base,
-                                 (uint64_t)B.total_blocks};
+                                 (uint64_t)B.total_blocks, (uint64_t)(uintptr_t)d_items,
+                                 (uint64_t)(uintptr_t)d_prefix, (uint64_t)(uintptr_t)d_cnt,
+                                 (uint64_t)(uintptr_t)d_scratch, (uint64_t)waves};
+    static_assert(sizeof(QueryParams) % sizeof(uint64_t) == 0, "QueryParams is keyed as 64-bit words");
     for (BatchSlot* b : its) {
-        key.push_back((uint64_t)(uintptr_t)b->qcol);
         key.push_back((uint64_t)(uintptr_t)b->img);
         key.push_back(((uint64_t)b->h.width << 40) | ((uint64_t)b->h.height << 20) | (uint64_t)b->M);
+        uint64_t words[sizeof(QueryParams) / sizeof(uint64_t)];
+        memcpy(words, &b->q, sizeof(QueryParams));
+        key.insert(key.end(), words, words + sizeof(QueryParams) / sizeof(uint64_t));
     }
     bool reused = c->bgraph && key == c->bkey;
     if (!reused) {

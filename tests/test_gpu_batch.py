@@ -108,3 +108,38 @@ def test_batch_errors(tlr):
         ctx.reconstruct_batch([(3, 8)])  # empty slot
     with pytest.raises(tlr.TlrError):
         ctx.batch_set_mode(2)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_new_parameters_are_not_replayed_stale(tlr, mode):
+    """A later call on the same slots with other CoC adjustments, and after a
+    slot is re-uploaded with a different scene of the same size, gives the
+    single-buffer path's images bit for bit (the captured graph must not
+    replay the previous call's parameters)."""
+    sbufs = [tlr.make_scene(p, W, H, spp, seed=900 + k) for k, (p, W, H, spp, m) in enumerate(BUFS[:3])]
+    ctx = tlr.Ctx(0)
+    ctx.batch_set_mode(mode)
+    for k, sb in enumerate(sbufs):
+        ctx.batch_set_samples(k, sb)
+    single = tlr.Ctx(0)
+
+    def check(jobs, bufs, label):
+        imgs, _, st = ctx.reconstruct_batch(jobs)
+        for k, job in enumerate(jobs):
+            single.set_samples(bufs[job[0]])
+            single.build(job[2] if len(job) > 2 else 1.0, job[3] if len(job) > 3 else 1.0)
+            ref, _ = single.reconstruct(out_spp=job[1])
+            assert np.array_equal(imgs[k], ref), "%s: item %d differs" % (label, k)
+        return st
+
+    print("form %d: plain call" % mode, flush=True)
+    jobs = [(k, BUFS[k][4]) for k in range(3)]
+    check(jobs, sbufs, "first call")
+    print("form %d: aperture 1.3 / focal 0.8 on two items" % mode, flush=True)
+    jobs2 = [(0, BUFS[0][4], 1.3, 1.0), (1, BUFS[1][4], 1.0, 0.8), (2, BUFS[2][4])]
+    check(jobs2, sbufs, "new adjustments")
+    print("form %d: slot 1 re-uploaded with another seed" % mode, flush=True)
+    p, W, H, spp, m = BUFS[1]
+    sbufs[1] = tlr.make_scene(p, W, H, spp, seed=1234)
+    ctx.batch_set_samples(1, sbufs[1])
+    check(jobs, sbufs, "re-uploaded slot")
