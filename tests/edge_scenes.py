@@ -31,9 +31,9 @@ def wcross_scene(tlr, W=64, H=48, spp=8, seed=41, block=(20, 14, 10, 8)):
     """A 960x540-type scene whose pixels in `block` (x0, y0, width, height)
     move in w at up to 4x their depth per shutter, so samples take each
     reprojection branch and some cross w = 0 inside the shutter; 1% of the
-    samples sit behind the camera with no motion (discarded at every t').  Samples whose chosen
-    P0.w would fall below 0.2 w keep their original motion, so reprojected
-    positions stay within a few image widths."""
+    samples sit behind the camera with no motion (discarded at every t').
+    Samples whose chosen P0.w would fall below 0.2 w keep their original
+    motion, so reprojected positions stay within a few image widths."""
     sb = tlr.make_scene(2, W, H, spp, seed=seed)
     rec = sb.records.reshape(H, W, spp, 16).copy()
     rng = np.random.default_rng(seed)
