@@ -135,6 +135,22 @@ tlr_status  tlr_build(tlr_ctx* ctx, float aperture_adjust, float focal_adjust);
 tlr_status  tlr_reconstruct(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba_out, int out_on_device,
                             tlr_stats* stats);
 
+/* Allocation-free variant of tlr_reconstruct.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The plain method only (path 0 or 1, no gate); rgba_out is device memory
+ * receiving width*height*4 floats.  Every query-phase buffer (pattern,
+ * per-output-sample colours, hand-off list, counters, the general kernel's
+ * per-wave scratch) lives in one fixed device array of 73.7 MB per GPU, held
+ * until the process ends, and the image is processed in row bands of at most
+ * 2^20 output samples; out_spp above 256, or an image row of more than 2^20
+ * output samples, is TLR_E_INVALID.  The general kernel runs 64 waves, so
+ * hand-off-heavy frames take longer than through tlr_reconstruct.  Images and
+ * stats equal tlr_reconstruct's.  All calls on one GPU share the array (it
+ * keeps the last call's contents; each call clears what it needs): calls that
+ * overlap in time on one GPU, from any context or thread, race. */
+tlr_status  tlr_reconstruct_noalloc(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba_out, tlr_stats* stats);
+
 /* Blocks until the context's stream is idle. */
 tlr_status  tlr_sync(tlr_ctx* ctx);
 

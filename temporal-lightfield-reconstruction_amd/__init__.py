@@ -27,7 +27,7 @@ STATUS = {0: "TLR_OK", 1: "TLR_E_INVALID", 2: "TLR_E_IO", 3: "TLR_E_ALL_DISCARDE
 
 EXPORTED = [
     "tlr_create", "tlr_destroy", "tlr_last_error", "tlr_load_sample_file", "tlr_save_sample_file",
-    "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_sync",
+    "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_reconstruct_noalloc", "tlr_sync",
     "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
     "tlr_copy_leaves", "tlr_copy_hierarchy", "tlr_copy_query_colors", "tlr_copy_query_stages",
     "tlr_output_pattern", "tlr_calibrate_traffic",
@@ -139,6 +139,7 @@ def lib():
         L.tlr_set_samples_soa.argtypes = [V, C.POINTER(Header), C.POINTER(Soa), C.c_int]
         L.tlr_build.argtypes = [V, C.c_float, C.c_float]
         L.tlr_reconstruct.argtypes = [V, C.POINTER(ReconParams), V, C.c_int, C.POINTER(Stats)]
+        L.tlr_reconstruct_noalloc.argtypes = [V, C.POINTER(ReconParams), V, C.POINTER(Stats)]
         L.tlr_sync.argtypes = [V]
         L.tlr_get_build_info.argtypes = [V, C.POINTER(BuildInfo)]
         L.tlr_copy_reprojected.argtypes = [V, V, V, V, V]
@@ -313,6 +314,16 @@ class Ctx:
         st = Stats()
         self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img) if img is not None else None, 0, C.byref(st)))
         return img, st.as_dict()
+
+    def reconstruct_noalloc(self, device_out, out_spp=16, override_uvt=None, lens_gaussian=False,
+                            time_gaussian=False, collect_stats=False, path=0):
+        """tlr_reconstruct_noalloc into device memory at address device_out
+        (W*H*4 floats); returns the stats."""
+        p = self._params(out_spp=out_spp, override_uvt=override_uvt, lens_gaussian=lens_gaussian,
+                         time_gaussian=time_gaussian, collect_stats=collect_stats, path=path)
+        st = Stats()
+        self._chk(lib().tlr_reconstruct_noalloc(self.h, C.byref(p), C.c_void_p(device_out), C.byref(st)))
+        return st.as_dict()
 
     def batch_set_samples(self, slot, sbuf):
         """Uploads a sample buffer into batch slot `slot` (device-resident)."""

@@ -485,7 +485,7 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
     const int M = Q.M;
     const unsigned pix = qi / (unsigned)M;
     const int i = (int)(qi - pix * (unsigned)M);
-    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
     const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
     const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
     OSample o;
@@ -764,7 +764,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
     const unsigned q = active ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
     const unsigned pix = q / (unsigned)M;
     const int i = (int)(q - pix * (unsigned)M);
-    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
     bool fail = false;
     bool single = false;
     bool needs2r = false;  // failed because the reference fetches 2R
@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
     const int M = Q.M;
     const unsigned pix = active ? q / (unsigned)M : 0u;
     const int i = (int)(q - pix * (unsigned)M);
-    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
     if (!active) return;
     const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
     const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);

@@ -67,6 +67,9 @@ struct QueryParams {
     // fast kernel over all queries: 0 = lanes in query order (pixel-major);
     // 1 = each wave takes one output-sample index over an 8x8 pixel tile
     int tile_order;
+    // first image row of this launch: its W x H pixels are rows [y0, y0+H)
+    // of the image (0 for a whole image); query ids and qcol are local
+    int y0;
 };
 
 // Lanes the fast kernel is launched with over all queries of Q (not a list):

@@ -20,7 +20,7 @@ struct OSample {
 __device__ __forceinline__ OSample make_osample(const QueryParams& Q, unsigned q) {
     const unsigned pix = q / (unsigned)Q.M;
     const int i = (int)(q - pix * (unsigned)Q.M);
-    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W);
+    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
     const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
     const float* tp = Q.pattern + 5 * ((size_t)pat * Q.M + i);
     OSample o;

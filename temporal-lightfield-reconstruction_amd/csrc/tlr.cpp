@@ -18,6 +18,7 @@
 #include "../../include/tlr.h"
 #include "build.hpp"
 #include "gate.hpp"
+#include "noalloc.hpp"
 #include "pattern.hpp"
 #include "prims.hpp"
 #include "query.hpp"
@@ -142,6 +143,11 @@ struct tlr_ctx {
     int tuned_chain = 0;                 // 1: per-buffer launches chained in the graph
     int batch_mode = -1;                 // tlr_batch_set_mode: -1 autotune, 0 merged, 1 chained
 
+    // tlr_reconstruct_noalloc: this device's fixed scratch array and the
+    // host copy of the output pattern last uploaded into it
+    char* noalloc = nullptr;
+    std::vector<float> noalloc_pat;
+
     tlr_status fail(tlr_status s, const char* fmt, ...) {
         char buf[512];
         va_list ap;
@@ -182,6 +188,10 @@ tlr_status tlr_create(tlr_ctx** out, int device) {
             return TLR_E_HIP;
         }
     c->events = true;
+    if (noalloc_scratch(device, &c->noalloc) != hipSuccess) {
+        delete c;
+        return TLR_E_HIP;
+    }
     if (hipMalloc(&c->d_gi, sizeof(GridInfo)) != hipSuccess ||
         hipMalloc(&c->d_stats, sizeof(unsigned long long) * kNumStats) != hipSuccess ||
         hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess ||
@@ -599,9 +609,11 @@ static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* l
     }
     HIPCHK(c, launch_query_fast(F, stats_mode, s));
     HIPCHK(c, hipEventRecord(c->ev[5], s));
+    // the general kernel takes the hand-offs from the list the fast kernel
+    // just filled (Q0's, which need not be the context's own)
     QueryParams G = Q0;
-    G.qlist = c->fail_list;
-    G.qcount = c->d_fail_count;
+    G.qlist = Q0.fail_list;
+    G.qcount = Q0.fail_count;
     G.labels = nullptr;
     HIPCHK(c, launch_query_general(G, s));
     return TLR_OK;
@@ -802,6 +814,108 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
             stats->count = (double)nq;
         }
         stats->total_ms = stats->build_ms + stats->query_ms + stats->combine_ms;
+        stats->num_leaves = c->L;
+        stats->num_samples = c->nb;
+    }
+    if (hs[ST_OVERFLOW])
+        return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",
+                       (unsigned long long)hs[ST_OVERFLOW]);
+    return TLR_OK;
+}
+
+// Allocation-free variant of tlr_reconstruct (plain method): every buffer of
+// the query phase lives in this device's fixed scratch array, and the image
+// is reconstructed in row bands of at most kNoAllocBandQueries output samples
+// (fast kernel, general kernel over its hand-offs, combine into the caller's
+// rows).  No allocation: counters are cleared by a fill kernel, the pattern
+// is copied from a host vector the context keeps.
+tlr_status tlr_reconstruct_noalloc(tlr_ctx* c, const tlr_recon_params* p, float* out, tlr_stats* stats) {
+    if (!c || !p) return TLR_E_INVALID;
+    if (!out) return c->fail(TLR_E_INVALID, "tlr_reconstruct_noalloc needs a device destination");
+    if (!c->built) return c->fail(TLR_E_STATE, "reconstruct before tlr_build");
+    if (p->gate || p->path == 2 || p->path < 0 || p->path > 2)
+        return c->fail(TLR_E_INVALID, "tlr_reconstruct_noalloc runs the plain method only (path 0 or 1, no gate)");
+    if (p->out_spp <= 0 && !p->override_uvt) return c->fail(TLR_E_INVALID, "out_spp must be positive");
+    if (!p->override_uvt && p->out_spp > kNoAllocMaxM)
+        return c->fail(TLR_E_INVALID, "tlr_reconstruct_noalloc supports at most %d output samples per pixel",
+                       kNoAllocMaxM);
+    const int W = c->h.width, H = c->h.height;
+    const int M = p->override_uvt ? 1 : p->out_spp;
+    if ((size_t)W * M > kNoAllocBandQueries)
+        return c->fail(TLR_E_INVALID, "an image row of %zu output samples exceeds the %zu of one band",
+                       (size_t)W * M, kNoAllocBandQueries);
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    c->noalloc_pat.clear();
+    make_output_pattern(p->out_spp, p->override_uvt != 0, p->uvt, c->noalloc_pat);
+    char* base = c->noalloc;
+    unsigned* counters = reinterpret_cast<unsigned*>(base + kNoAllocCounters);
+    unsigned long long* st = reinterpret_cast<unsigned long long*>(base + kNoAllocCounters + 32);
+    float* pattern = reinterpret_cast<float*>(base + kNoAllocPattern);
+    float4* qcol = reinterpret_cast<float4*>(base + kNoAllocQcol);
+    unsigned* fail = reinterpret_cast<unsigned*>(base + kNoAllocFail);
+    float* wscratch = reinterpret_cast<float*>(base + kNoAllocWave);
+    HIPCHK(c, hipMemcpyAsync(pattern, c->noalloc_pat.data(), c->noalloc_pat.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c, launch_fill_u32(counters, 64, 0u, s));
+    // rows per band: a multiple of 8 (whole 8x8 tiles) when the band holds 8
+    int rows = (int)(kNoAllocBandQueries / ((size_t)W * M));
+    if (rows >= 8) rows &= ~7;
+    rows = std::min(rows, H);
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    for (int y0 = 0; y0 < H; y0 += rows) {
+        const int bh = std::min(rows, H - y0);
+        QueryParams Q;
+        memset(&Q, 0, sizeof(Q));
+        Q.W = W; Q.H = bh; Q.M = M; Q.y0 = y0;
+        Q.spp_pow2 = c->spp_pow2;
+        Q.cc0 = c->cc0; Q.cc1 = c->cc1;
+        Q.lens_gauss = p->lens_gaussian;
+        Q.time_gauss = p->time_gaussian;
+        Q.collect_stats = p->collect_stats;
+        Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
+        Q.root = c->nv > 1 ? c->nv / 2 - 1 : 0;
+        Q.num_queries = (unsigned)((size_t)W * bh * M);
+        Q.pattern = pattern;
+        Q.trav = c->trav;
+        Q.leaves = c->leaves;
+        Q.leaf_b = c->leaf_b;
+        Q.pts = c->pts;
+        Q.qcol = qcol;
+        Q.stats = st;
+        Q.next_query = counters;
+        Q.fail_list = fail;
+        Q.fail_count = counters + 1;
+        Q.scratch = wscratch;
+        Q.num_waves = kNoAllocWaves;
+        Q.tile_order = 1;
+        if (y0 > 0) HIPCHK(c, launch_fill_u32(counters, 2, 0u, s));
+        tlr_status r = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0, p->path);
+        if (r != TLR_OK) return r;
+        HIPCHK(c, launch_combine(qcol, W, bh, M, out + (size_t)y0 * W * 4, s));
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    unsigned long long hs[kNumStats];
+    HIPCHK(c, hipMemcpyAsync(hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->leaf_nodes = (double)hs[ST_LEAF_NODES];
+        stats->samples_in_leaf_nodes = (double)hs[ST_SAMPLES_IN_LEAVES];
+        stats->samples_within_1r = (double)hs[ST_WITHIN_1R];
+        stats->samples_fetched = (double)hs[ST_FETCHED];
+        stats->surfaces = (double)hs[ST_SURFACES];
+        stats->num_2r = (double)hs[ST_2R];
+        stats->surfaces_merged = (double)hs[ST_MERGED];
+        stats->at_least_one = (double)hs[ST_AT_LEAST_ONE];
+        stats->count = (double)hs[ST_COUNT];
+        stats->overflow = (double)hs[ST_OVERFLOW];
+        stats->fast_leaf_nodes = (double)hs[ST_FAST_LEAVES];
+        stats->fast_samples_in_leaf_nodes = (double)hs[ST_FAST_IN_LEAVES];
+        stats->fast_samples_within_1r = (double)hs[ST_FAST_KEPT];
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
+        stats->total_ms = stats->build_ms + stats->query_ms;
         stats->num_leaves = c->L;
         stats->num_samples = c->nb;
     }
