@@ -55,7 +55,7 @@ typedef struct tlr_recon_params {
     int   lens_gaussian;       /* CameraParams::lensFilter == LENS_GAUSSIAN         */
     int   time_gaussian;       /* CameraParams::timeFilter == TIME_GAUSSIAN         */
     int   collect_stats;       /* fill the Stats counters (slower)                  */
-    int   gate;                /* 0: plain method; 1: learned gate                  */
+    int   gate;                /* 0: plain method; 1: learned gate (path 0 only)    */
     int   path;                /* 0: fast 1R kernel + general kernel for the rest;
                                   1: general kernel for every output sample;
                                   2: the reference's GPU-variant method
@@ -82,8 +82,9 @@ typedef struct tlr_stats {
     double num_leaves, num_samples;
     double fast_ms, general_ms;  /* split of query_ms: 1R lane kernel, general kernel */
     double general_queries;      /* output samples handled by the general kernel     */
-    double direct_ms, gate_ms;   /* gated path: direct filter + features, MLP + routing */
-    double gate_rejected;        /* output samples the gate sent to the plain method */
+    double gate_candidates;      /* gated path: hand-offs of the 1R lane kernel      */
+    double gate_ms;              /* gated path: MLP + routing                        */
+    double gate_rejected;        /* hand-offs the gate sent to the general kernel    */
     /* the 1R lane kernel's share of leaf_nodes, samples_in_leaf_nodes and
      * samples_within_1r (its algorithmic bytes) */
     double fast_leaf_nodes, fast_samples_in_leaf_nodes, fast_samples_within_1r;
@@ -240,27 +241,33 @@ int         tlr_output_pattern(int out_spp, int override_uvt, const float* uvt, 
 tlr_status  tlr_calibrate_traffic(tlr_ctx* ctx, int kind, int64_t bytes, int64_t* read, int64_t* written);
 
 /* ---- Learned gate (no reference counterpart) ------------------------------
- * A 24-64-64-1 MLP (BF16 operands, FP32 accumulation on MFMA) scores each
- * output sample from features of a streaming 1R gather; output samples it
- * accepts take the direct tent filter of that gather, the others the plain
- * method.  Parameter layout: W1[24][64], b1[64], W2[64][64], b2[64], w3[64],
- * b3 (TLR_GATE_PARAMS floats, row-major with the input index first). */
+ * The plain method's 1R lane kernel finishes most output samples and hands
+ * the rest (a 2R fetch, no surface, too many leaves) to the general kernel.
+ * With the gate on, the lane kernel also filters the first 1R surface of each
+ * hand-off that has one (a provisional colour) and writes 24 features per
+ * hand-off; a 24-64-64-1 MLP (BF16 operands, FP32 accumulation on MFMA)
+ * scores them, and hand-offs scoring at least the threshold keep the
+ * provisional colour instead of running the general kernel.  Parameter
+ * layout: W1[24][64], b1[64], W2[64][64], b2[64], w3[64], b3
+ * (TLR_GATE_PARAMS floats, row-major with the input index first). */
 #define TLR_GATE_FEATURES 24
 #define TLR_GATE_PARAMS 5825
 tlr_status  tlr_gate_set_params(tlr_ctx* ctx, const float* params);
 tlr_status  tlr_gate_get_params(tlr_ctx* ctx, float* params);
-/* output samples with logit >= threshold take the direct filter */
+/* hand-offs with logit >= threshold keep their provisional colour */
 tlr_status  tlr_gate_set_threshold(tlr_ctx* ctx, float logit_threshold);
 /* host arrays: logits of n feature rows */
 tlr_status  tlr_gate_forward(tlr_ctx* ctx, const float* features, int64_t n, float* logits);
 /* host arrays: summed gradient of the weighted BCE loss over n rows */
 tlr_status  tlr_gate_grad(tlr_ctx* ctx, const float* features, const float* labels, int64_t n, float pos_weight,
                           float* grad, float* loss);
-/* For every stride-th output sample of the current build (query order):
- * features, label (1 = one surface at 1R, no 2R fetch), direct-filter and
- * plain-method colours.  Buffers may be NULL; *n receives the row count. */
-tlr_status  tlr_gate_collect(tlr_ctx* ctx, const tlr_recon_params* p, int stride, float* features, float* labels,
-                             float* direct_rgba, float* plain_rgba, int64_t* n);
+/* Over the current build (plain method, path 0): every stride-th hand-off
+ * entry's features, label (1 = the provisional colour is within 0.001 of the
+ * plain method's in every channel), provisional colour, plain colour and
+ * output-sample index ((y*W+x)*M+i).  At most `cap` rows are written; *n
+ * receives the row count (which may exceed cap).  Buffers may be NULL. */
+tlr_status  tlr_gate_collect(tlr_ctx* ctx, const tlr_recon_params* p, int stride, int64_t cap, float* features,
+                             float* labels, float* prov_rgba, float* plain_rgba, uint32_t* query, int64_t* n);
 /* Same rows appended to a device-resident training set. */
 tlr_status  tlr_gate_append_training(tlr_ctx* ctx, const tlr_recon_params* p, int stride);
 tlr_status  tlr_gate_clear_training(tlr_ctx* ctx);

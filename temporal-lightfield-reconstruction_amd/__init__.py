@@ -68,7 +68,7 @@ class Stats(C.Structure):
         "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched",
         "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count", "overflow",
         "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
-        "fast_ms", "general_ms", "general_queries", "direct_ms", "gate_ms", "gate_rejected",
+        "fast_ms", "general_ms", "general_queries", "gate_candidates", "gate_ms", "gate_rejected",
         "fast_leaf_nodes", "fast_samples_in_leaf_nodes", "fast_samples_within_1r")]
 
     def as_dict(self):
@@ -162,7 +162,8 @@ def lib():
         L.tlr_gate_set_threshold.argtypes = [V, C.c_float]
         L.tlr_gate_forward.argtypes = [V, V, C.c_int64, V]
         L.tlr_gate_grad.argtypes = [V, V, V, C.c_int64, C.c_float, V, V]
-        L.tlr_gate_collect.argtypes = [V, C.POINTER(ReconParams), C.c_int, V, V, V, V, C.POINTER(C.c_int64)]
+        L.tlr_gate_collect.argtypes = [V, C.POINTER(ReconParams), C.c_int, C.c_int64, V, V, V, V, V,
+                                       C.POINTER(C.c_int64)]
         L.tlr_gate_append_training.argtypes = [V, C.POINTER(ReconParams), C.c_int]
         L.tlr_gate_clear_training.argtypes = [V]
         L.tlr_gate_training_size.restype = C.c_int64
@@ -430,17 +431,21 @@ class Ctx:
         return g, float(loss[0])
 
     def gate_collect(self, out_spp=16, stride=1, **kw):
+        """Every stride-th hand-off entry of the current build: (features,
+        labels, provisional colours, plain colours, output-sample indices)."""
         p = self._params(out_spp=out_spp, **kw)
         M = 1 if p.override_uvt else out_spp
-        nq = self.sbuf.width * self.sbuf.height * M
-        rows = (nq + stride - 1) // stride
-        X = np.empty((rows, GATE_FEATURES), np.float32)
-        y = np.empty(rows, np.float32)
-        d = np.empty((rows, 4), np.float32)
-        pl = np.empty((rows, 4), np.float32)
+        cap = (self.sbuf.width * self.sbuf.height * M + stride - 1) // stride
+        X = np.empty((cap, GATE_FEATURES), np.float32)
+        y = np.empty(cap, np.float32)
+        pv = np.empty((cap, 4), np.float32)
+        pl = np.empty((cap, 4), np.float32)
+        qi = np.empty(cap, np.uint32)
         n = C.c_int64(0)
-        self._chk(lib().tlr_gate_collect(self.h, C.byref(p), stride, _p(X), _p(y), _p(d), _p(pl), C.byref(n)))
-        return X, y, d, pl
+        self._chk(lib().tlr_gate_collect(self.h, C.byref(p), stride, cap, _p(X), _p(y), _p(pv), _p(pl), _p(qi),
+                                         C.byref(n)))
+        k = min(n.value, cap)
+        return X[:k], y[:k], pv[:k], pl[:k], qi[:k]
 
     def gate_append_training(self, out_spp=16, stride=1, **kw):
         p = self._params(out_spp=out_spp, **kw)

@@ -168,7 +168,9 @@ constexpr int kFwdBlock = 256;
 constexpr int kFwdWaves = kFwdBlock / 64;
 
 __global__ __launch_bounds__(kFwdBlock) void k_gate_forward(const float* __restrict__ X, int n,
+                                                            const unsigned* __restrict__ n_dev,
                                                             const float* __restrict__ P, float* __restrict__ logits) {
+    if (n_dev) n = (int)*n_dev;
     __shared__ float H1s[kFwdWaves][16 * kGateHid];
     const int lane = threadIdx.x & 63;
     float* H1 = H1s[threadIdx.x >> 6];
@@ -390,7 +392,16 @@ hipError_t launch_gate_forward(const float* X, int n, const float* params, float
     if (n <= 0) return hipSuccess;
     int blocks = (n + 16 * kFwdWaves - 1) / (16 * kFwdWaves);
     if (blocks > 4096) blocks = 4096;
-    k_gate_forward<<<blocks, kFwdBlock, 0, s>>>(X, n, params, logits);
+    k_gate_forward<<<blocks, kFwdBlock, 0, s>>>(X, n, nullptr, params, logits);
+    return hipGetLastError();
+}
+
+hipError_t launch_gate_forward_dev(const float* X, const unsigned* n_dev, size_t max_n, const float* params,
+                                   float* logits, hipStream_t s) {
+    if (max_n == 0) return hipSuccess;
+    size_t blocks = (max_n + 16 * kFwdWaves - 1) / (16 * kFwdWaves);
+    if (blocks > 4096) blocks = 4096;
+    k_gate_forward<<<(unsigned)blocks, kFwdBlock, 0, s>>>(X, 0, n_dev, params, logits);
     return hipGetLastError();
 }
 

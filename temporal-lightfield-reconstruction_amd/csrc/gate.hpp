@@ -21,6 +21,9 @@ constexpr int kGateParams = kGateOffB3 + 1;  // 5825
 
 // Logits z for n feature rows (24 floats each).
 hipError_t launch_gate_forward(const float* X, int n, const float* params, float* logits, hipStream_t s);
+// Same for *n_dev rows (a device-side count, at most max_n).
+hipError_t launch_gate_forward_dev(const float* X, const unsigned* n_dev, size_t max_n, const float* params,
+                                   float* logits, hipStream_t s);
 // Accumulates d(loss)/d(params) over rows idx[0..n) into grad (zeroed by the
 // caller) and the summed loss into *loss.  Loss: weighted binary cross
 // entropy on logits, weight pos_w for label 1.

@@ -21,8 +21,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
-// Marks a query in the hand-over list whose 1R pass already decided on 2R.
-constexpr unsigned kNeeds2R = 0x80000000u;
 enum { XPOS = 1, YPOS = 2, XNEG = 4, YNEG = 8 };
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -754,9 +752,79 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
 // otherwise a leaf's scan stops at its first sample within R and the surface
 // pass stops when the second surface starts.
 // ---------------------------------------------------------------------------
+// Gate features of a hand-off entry (kGateFeatures floats), from what the
+// fast kernel's 1R passes found; `prov` false (no first-surface colour: no
+// surface at 1R, or more leaves than the lane's list holds) marks the entry
+// with f[0] = -1.  Ratios that can be unbounded pass through a signed log;
+// everything is clamped to [-16, 16] and NaN becomes 0.
+//   0 log1p(leaves at 1R)  1 log1p(leaves with a sample within R)
+//   2 log1p(such leaves in the first surface)  3 quadrants of the first
+//   surface / 4  4..7 its quadrant bits  8 log1p(its samples within R)
+//   9 their mean tent weight  10 nearest distance / R  11 log1p(leaves in
+//   front of the first surface with no sample within R)  12 slog(first-surface
+//   key)  13 slog(relative key gap to the second surface)  14,15 centroid
+//   offset (x, y) of the first surface's samples / R  16 nearest sample of
+//   the leaves in front / R (16 if none)  17..19 output (u, v, t)
+//   20 slog(CoC at the first surface / 16)  21 fraction of kept leaves in
+//   the first surface  22 log1p(samples tested in its leaves)
+//   23 log1p(kept leaves behind it)
+struct GateAcc {
+    int cnt0 = 0, tested0 = 0;
+    float sdx = 0.f, sdy = 0.f;
+};
+
+__device__ __forceinline__ float slog(float v) { return copysignf(log1pf(fabsf(v)), v); }
+
+__device__ __forceinline__ void write_gate_features(float* out, bool prov, const GateAcc& g, int n, int nkept,
+                                                    int nkept0, int quad0, float aw, float min0, int nsurf,
+                                                    float key0, float key1, int front, float front_d2,
+                                                    const OSample& o, float R, float cc0, float cc1) {
+    float f[kGateFeatures];
+#pragma unroll
+    for (int k = 0; k < kGateFeatures; k++) f[k] = 0.0f;
+    if (prov) {
+        const float c = (float)(g.cnt0 > 0 ? g.cnt0 : 1);
+        f[0] = log1pf((float)n);
+        f[1] = log1pf((float)nkept);
+        f[2] = log1pf((float)nkept0);
+        f[3] = 0.25f * (float)__popc((unsigned)quad0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) f[4 + q] = (quad0 >> q) & 1 ? 1.0f : 0.0f;
+        f[8] = log1pf((float)g.cnt0);
+        f[9] = aw / c;
+        f[10] = min0 / R;
+        f[11] = log1pf((float)front);
+        f[12] = slog(key0);
+        f[13] = nsurf > 1 ? slog((key1 - key0) / fmaxf(fabsf(key0), 1e-6f)) : 0.0f;
+        f[14] = g.sdx / c / R;
+        f[15] = g.sdy / c / R;
+        f[16] = front ? sqrtf(front_d2) / R : 16.0f;
+        f[17] = o.u;
+        f[18] = o.v;
+        f[19] = o.t;
+        f[20] = slog((cc0 / key0 + cc1) / 16.0f);
+        f[21] = nkept ? (float)nkept0 / (float)nkept : 0.0f;
+        f[22] = log1pf((float)g.tested0);
+        f[23] = log1pf((float)(nkept - nkept0));
+#pragma unroll
+        for (int k = 0; k < kGateFeatures; k++) {
+            const float v = f[k];
+            f[k] = v == v ? fminf(fmaxf(v, -16.0f), 16.0f) : 0.0f;
+        }
+    } else {
+        f[0] = -1.0f;
+    }
+    float4* o4 = reinterpret_cast<float4*>(out);
+#pragma unroll
+    for (int k = 0; k < kGateFeatures / 4; k++) o4[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+}
+
 // One block of the fast path: `slot` is this lane's query index (or list
 // position).  lid: the block's LDS leaf lists, CAP entries per lane.
-template <bool STATS, int CAP, int BLOCK>
+// GATE: hand-offs that have a first surface at 1R still get its filtered
+// colour in qcol (a provisional result the learned gate may keep), and every
+// hand-off entry gets its gate features in Q.features (row = list position).
+template <bool STATS, int CAP, int BLOCK, bool GATE = false>
 __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, unsigned item, int (*lid)[BLOCK]) {
     const int tid = threadIdx.x;
     const int M = Q.M;
@@ -766,9 +834,15 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
     const int i = (int)(q - pix * (unsigned)M);
     const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
     bool fail = false;
-    bool single = false;
     bool needs2r = false;  // failed because the reference fetches 2R
     unsigned long long s_leaves = 0, s_in_leaves = 0, s_kept = 0, s_surf = 0;
+    // gate features (GATE only)
+    bool prov = false;
+    GateAcc gacc;
+    int g_front = 0;            // leaves in front of the first with a sample within R
+    float g_front_d2 = kFltMax; // their nearest sample, squared distance
+    int g_n = 0, g_nkept = 0, g_nkept0 = 0, g_quad0 = 0, g_nsurf = 0;
+    float g_aw = 0.f, g_min0 = 0.f, g_key0 = 0.f, g_key1 = 0.f;
     OSample o{0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
@@ -889,6 +963,9 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                 const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
                 s_in_leaves += (unsigned long long)(k1 - k0);
                 bool done_leaf = false;
+                // GATE: nearest sample (hardware-reciprocal estimate) of the
+                // leaves in front of the first one with a sample within R
+                float leaf_min = kFltMax;
                 for (int kg = k0; kg < k1 && !done_leaf; kg += 4) {
                     const float4* g = pab + 2 * (size_t)kg;
                     float4 ga[4], gb[4];
@@ -900,6 +977,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
                         if (kg + e >= k1) break;
+                        if (GATE && kmask == 0) leaf_min = fminf(leaf_min, approx_d2(ga[e], gb[e], o, Q.cc0, Q.cc1));
                         if (surely_outside(ga[e], gb[e], o, Q.cc0, Q.cc1, R)) continue;
                         float x, y;
                         reproject_point(ga[e], gb[e], o, Q.cc0, Q.cc1, x, y);
@@ -912,6 +990,10 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                             break;
                         }
                     }
+                }
+                if (GATE && kmask == 0) {
+                    g_front++;
+                    g_front_d2 = fminf(g_front_d2, leaf_min);
                 }
             }
             // 2. surfaces; surface 0 = kept leaves in [first0, end0)
@@ -951,6 +1033,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                 const int li = __builtin_ctzll(km);
                 const int id = lid[li][tid];
                 const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
+                if (GATE) gacc.tested0 += k1 - k0;
                 for (int kg = k0; kg < k1; kg += 4) {
                     const float4* g = pab + 2 * (size_t)kg;
                     float4 ga[4], gb[4];
@@ -971,6 +1054,11 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                         if (!(d2 <= R2)) continue;
                         quad0 |= 1 << ((dx >= 0 ? XPOS : 0) | (dy >= 0 ? YPOS : 0));
                         const float dist = sqrtf(d2);
+                        if (GATE) {
+                            gacc.cnt0++;
+                            gacc.sdx += dx;
+                            gacc.sdy += dy;
+                        }
                         const float4 col = pc[k];
                         if (dist < min0) {
                             min0 = dist;
@@ -988,12 +1076,24 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             }
             const bool stop = nsurf > 1;
             const bool f2 = nsurf == 0 || (quad0 != 0xF && (nsurf > 1 || stop));
-            single = nsurf == 1 && !stop;
             if (f2) {
                 fail = true;
                 needs2r = true;
-            } else {
-                s_surf = (unsigned long long)nsurf;
+            }
+            if (GATE && f2 && nsurf > 0) {
+                prov = true;
+                g_n = n;
+                g_nkept = __popcll(kmask);
+                g_nkept0 = __popcll(kmask & ((end0 < 64 ? (1ull << end0) : 0ull) - 1ull));
+                g_quad0 = quad0;
+                g_aw = aw;
+                g_min0 = min0;
+                g_nsurf = nsurf;
+                g_key0 = leaf_key(Q.leaves, lid[first0][tid], o.t);
+                g_key1 = nsurf > 1 ? leaf_key(Q.leaves, lid[end0][tid], o.t) : 0.0f;
+            }
+            if (!f2 || (GATE && nsurf > 0)) {
+                if (!f2) s_surf = (unsigned long long)nsurf;
                 const float oow = rcp(aw);
                 float cr = ar * oow, cg = ag * oow, cb = ab * oow, ca = aw * oow;
                 if (ca == 0) {
@@ -1015,7 +1115,6 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             }
         }
     }
-    if (Q.labels && active) Q.labels[q] = (!fail && single) ? 1.0f : 0.0f;
     // failures go to the general kernel (one atomic per wave)
     const bool hand = active && fail;
     const uint64_t fm = __ballot(hand);
@@ -1025,8 +1124,13 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
         if (lane == 0) base = atomicAdd(Q.fail_count, (unsigned)__popcll(fm));
         base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
         if (hand) {
-            Q.fail_list[base + lanes_below(fm)] = q | (needs2r ? kNeeds2R : 0u);
-            if (Q.fail_item) Q.fail_item[base + lanes_below(fm)] = item;
+            const unsigned e = base + lanes_below(fm);
+            Q.fail_list[e] = q | (needs2r ? kNeeds2R : 0u);
+            if (Q.fail_item) Q.fail_item[e] = item;
+            if (GATE)
+                write_gate_features(Q.features + (size_t)e * kGateFeatures, prov, gacc, g_n, g_nkept, g_nkept0,
+                                    g_quad0, g_aw, g_min0, g_nsurf, g_key0, g_key1, g_front, g_front_d2, o, R,
+                                    Q.cc0, Q.cc1);
         }
     }
     if (STATS) {
@@ -1078,6 +1182,14 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     fast_query<STATS, kFastLeafCap, kFastBlock>(Q, tile_slot(Q, slot), 0u, lid);
 }
 
+// Gated fast path (single buffer): as k_query_fast<false>, plus provisional
+// colours and gate features for the hand-offs.
+__global__ __launch_bounds__(kFastBlock, 7) void k_query_fast_gate(QueryParams Q) {
+    __shared__ int lid[kFastLeafCap][kFastBlock];
+    const unsigned slot = blockIdx.x * kFastBlock + threadIdx.x;
+    fast_query<false, kFastLeafCap, kFastBlock, true>(Q, tile_slot(Q, slot), 0u, lid);
+}
+
 // Batched fast path: one launch over every buffer of a batch.  Block b of the
 // grid is logical block b of the concatenated buffers (each buffer's lanes in
 // tile order, padded to whole blocks); its buffer is found by binary search
@@ -1097,234 +1209,59 @@ __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
                                   lid);
 }
 
-// ---------------------------------------------------------------------------
-// Direct filter + gate features (one lane per output sample, R = dispersion).
-// Leaves are visited in traversal order and every sample within R joins one
-// tent sum: exactly the plain result when the output sample sees a single
-// surface (ReconstructionOur.cpp:99,204,268-297), up to summation order.
-// Features (kGateIn floats; ratios 3,4,6-9,15 pass through sign(x)log(1+|x|),
-// all are clamped to [-16, 16]):
-//   0 log1p(kept)  1 log1p(leaves)  2 log1p(samples in leaves)
-//   3 depth range / min depth  4 depth std / mean  5 log(mean depth)
-//   6 CoC range  7 mean |CoC| / 16  8,9 screen motion range x,y / 16
-//   10 range of sample t  11..14 quadrant fractions
-//   15 leaf key range / min key  16 fraction of leaves whose key differs by
-//   more than 5% from the first leaf met  17,18,19 output (u, v, t)
-//   20 nearest distance / R  21 mean distance / R  22 tent weight sum / kept
-//   23 fraction of gathered samples whose depth differs by more than 5% from
-//   the first gathered sample
-// ---------------------------------------------------------------------------
-constexpr int kDirectBlock = 256;
-
-__global__ __launch_bounds__(kDirectBlock) void k_query_direct(QueryParams Q) {
-    // leaves found by the walk, in traversal order (scanned after the walk,
-    // so lanes in different phases do not serialise each other)
-    __shared__ int dl[kFastLeafCap][kDirectBlock];
-    const int tid = threadIdx.x;
-    const unsigned q = blockIdx.x * kDirectBlock + tid;
-    const bool active = q < Q.num_queries;
-    const int M = Q.M;
-    const unsigned pix = active ? q / (unsigned)M : 0u;
-    const int i = (int)(q - pix * (unsigned)M);
-    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
-    if (!active) return;
-    const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
-    const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
-    OSample o;
-    o.x = tp[0] + (float)px;
-    o.y = tp[1] + (float)py;
-    o.u = tp[2];
-    o.v = tp[3];
-    o.t = tp[4];
-    const float disp = dispersion_for_spp(Q.spp_pow2);
-    const float R = disp, R2 = R * R;
-    float ar = 0, ag = 0, ab = 0, aw = 0;
-    float dmin = kFltMax, dsum = 0;
-    float4 near = make_float4(0, 0, 0, 1);
-    int kept = 0, nleaf = 0, inleaf = 0;
-    float wmin = kFltMax, wmax = -kFltMax, wsum = 0, wsq = 0;
-    float cmin = kFltMax, cmax = -kFltMax, cabs = 0;
-    float mxmin = kFltMax, mxmax = -kFltMax, mymin = kFltMax, mymax = -kFltMax;
-    float tmin = kFltMax, tmax = -kFltMax;
-    float kmin = kFltMax, kmax = -kFltMax;
-    int quad[4] = {0, 0, 0, 0};
-    // depth separation relative to the first leaf / first kept sample met
-    float key_first = 0.0f, w_first = 0.0f;
-    int far_leaves = 0, off_depth = 0;
-    // The walk fills the list; when it is full (or the walk ends) the listed
-    // leaves are scanned and the walk resumes where it stopped.
-    const float4* __restrict__ pab = Q.pts.pab;
-    int node = Q.n_nodes > 0 ? Q.root : -1;
-    while (true) {
-        int nl = 0;
-        while (node >= 0) {
-            const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
-            const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
-            const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-            const int child = __float_as_int(d.x), escape = __float_as_int(d.y);
-            if (node_hit(p, o, R)) {
-                if (child >= 0) {
-                    node = child;
-                    continue;
-                }
-                dl[nl][tid] = -child - 1;
-                nl++;
-                if (nl == kFastLeafCap) {
-                    node = escape;
-                    break;
-                }
-            }
-            node = escape;
+// Learned-gate routing over the fast kernel's hand-off list: entry e keeps
+// its provisional first-surface colour when the gate's logit reaches the
+// threshold and the entry has one (feature 0 >= 0); otherwise the entry (with
+// its 2R marker) is appended to `list` for the general kernel.  A NaN logit
+// goes to the general kernel.
+__global__ __launch_bounds__(256) void k_gate_route(const float* __restrict__ logits,
+                                                    const float* __restrict__ features,
+                                                    const unsigned* __restrict__ entries,
+                                                    const unsigned* __restrict__ n_entries, float thr,
+                                                    unsigned* __restrict__ list, unsigned* __restrict__ count) {
+    const unsigned n = *n_entries;
+    for (unsigned e0 = blockIdx.x * 256; e0 < n; e0 += gridDim.x * 256) {
+        const unsigned e = e0 + threadIdx.x;
+        bool plain = false;
+        if (e < n) {
+            const float z = logits[e], f0 = features[(size_t)e * kGateFeatures];
+            plain = !(z >= thr && f0 >= 0.0f);
         }
-        for (int lj = 0; lj < nl; lj++) {
-            const int id = dl[lj][tid];
-            const int li = nleaf + lj;
-            const float4 r0 = *reinterpret_cast<const float4*>(Q.leaves + id);  // w0, t0, mvw0, s0
-            const int k0 = __float_as_int(r0.w), k1 = Q.leaves[id].s1;
-            const float key = r0.x + (o.t - r0.y) * r0.z;
-            if (li == 0) key_first = key;
-            far_leaves += (li > 0 && (key > 1.05f * key_first || key_first > 1.05f * key)) ? 1 : 0;
-            inleaf += k1 - k0;
-            kmin = fminf(kmin, key);
-            kmax = fmaxf(kmax, key);
-            for (int kg = k0; kg < k1; kg += 4) {
-                const float4* g = pab + 2 * (size_t)kg;
-                float4 ga[4], gb[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    ga[e] = g[2 * e];
-                    gb[e] = g[2 * e + 1];
-                }
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int k = kg + e;
-                    if (k >= k1) break;
-                    const float4 sa = ga[e], sb = gb[e];
-                    if (surely_outside(sa, sb, o, Q.cc0, Q.cc1, R)) continue;
-                    float x, y, w, coc, rw = 0.0f;
-                    if (sb.x != 0.0f || sb.y != 0.0f || sb.z != 0.0f) {
-                        const float dt = o.t - sa.w;
-                        const float Px = sa.x * sa.z + dt * sb.x;
-                        const float Py = sa.y * sa.z + dt * sb.y;
-                        const float Pw = 1.0f * sa.z + dt * sb.z;
-                        rw = rcp(Pw);
-                        coc = Q.cc0 / Pw + Q.cc1;
-                        x = Px * rw + coc * o.u;
-                        y = Py * rw + coc * o.v;
-                        w = Pw;
-                    } else {
-                        coc = Q.cc0 / sa.z + Q.cc1;
-                        x = sa.x + coc * o.u;
-                        y = sa.y + coc * o.v;
-                        w = sa.z;
-                    }
-                    const float dx = x - o.x, dy = y - o.y;
-                    const float d2 = dx * dx + dy * dy;
-                    if (!(d2 <= R2)) continue;
-                    const float dist = sqrtf(d2);
-                    if (kept == 0) w_first = w;
-                    off_depth += (w > 1.05f * w_first || w_first > 1.05f * w) ? 1 : 0;
-                    kept++;
-                    dsum += dist;
-                    const float4 col = Q.pts.pc[k];
-                    if (dist < dmin) {
-                        dmin = dist;
-                        near = col;
-                    }
-                    const float wgt = 1 - dist / disp;
-                    if (wgt > 0) {
-                        ar += wgt * col.x;
-                        ag += wgt * col.y;
-                        ab += wgt * col.z;
-                        aw += wgt * 1.0f;
-                    }
-                    wmin = fminf(wmin, w);
-                    wmax = fmaxf(wmax, w);
-                    wsum += w;
-                    wsq += w * w;
-                    cmin = fminf(cmin, coc);
-                    cmax = fmaxf(cmax, coc);
-                    cabs += fabsf(coc);
-                    // screen motion (zero for static samples)
-                    mxmin = fminf(mxmin, sb.x * rw);
-                    mxmax = fmaxf(mxmax, sb.x * rw);
-                    mymin = fminf(mymin, sb.y * rw);
-                    mymax = fmaxf(mymax, sb.y * rw);
-                    tmin = fminf(tmin, sa.w);
-                    tmax = fmaxf(tmax, sa.w);
-                    quad[(dx >= 0 ? 1 : 0) | (dy >= 0 ? 2 : 0)]++;
-                }
-            }
-        }
-        nleaf += nl;
-        if (node < 0) break;
-    }
-    // colour: tent result, or the nearest sample when all weights vanish
-    const float oow = rcp(aw);
-    float cr = ar * oow, cg = ag * oow, cb = ab * oow, ca = aw * oow;
-    if (ca == 0 && kept > 0) {
-        cr = near.x; cg = near.y; cb = near.z; ca = 1.0f;
-    }
-    Q.qcol[q] = make_float4(cr, cg, cb, ca);
-    if (Q.features) {
-        float f[kGateFeatures];
-        const float kf = (float)(kept > 0 ? kept : 1);
-        const float wm = wsum / kf;
-        f[0] = log1pf((float)kept);
-        f[1] = log1pf((float)nleaf);
-        f[2] = log1pf((float)inleaf);
-        f[3] = kept ? (wmax - wmin) / wmin : 0.0f;
-        f[4] = kept ? sqrtf(fmaxf(wsq / kf - wm * wm, 0.0f)) / wm : 0.0f;
-        f[5] = kept ? logf(wm) : 0.0f;
-        f[6] = kept ? (cmax - cmin) : 0.0f;
-        f[7] = kept ? cabs / kf / 16.0f : 0.0f;
-        f[8] = kept ? (mxmax - mxmin) / 16.0f : 0.0f;
-        f[9] = kept ? (mymax - mymin) / 16.0f : 0.0f;
-        f[10] = kept ? (tmax - tmin) : 0.0f;
-#pragma unroll
-        for (int k = 0; k < 4; k++) f[11 + k] = (float)quad[k] / kf;
-        f[15] = nleaf ? (kmax - kmin) / fmaxf(kmin, 1e-6f) : 0.0f;
-        f[16] = nleaf ? (float)far_leaves / (float)nleaf : 0.0f;
-        f[17] = o.u;
-        f[18] = o.v;
-        f[19] = o.t;
-        f[20] = kept ? dmin / R : 1.0f;
-        f[21] = kept ? dsum / kf / R : 1.0f;
-        f[22] = kept ? aw / kf : 0.0f;
-        f[23] = kept ? (float)off_depth / kf : 0.0f;
-        // ratios can be unbounded (a leaf key or depth near zero): signed log,
-        // then everything clamped to [-16, 16]
-#pragma unroll
-        for (int k = 0; k < kGateFeatures; k++) {
-            const bool ratio = k == 3 || k == 4 || k == 6 || k == 7 || k == 8 || k == 9 || k == 15;
-            float v = f[k];
-            if (ratio) v = copysignf(log1pf(fabsf(v)), v);
-            f[k] = fminf(fmaxf(v, -16.0f), 16.0f);
-        }
-        float* out = Q.features + (size_t)q * kGateFeatures;
-#pragma unroll
-        for (int k = 0; k < kGateFeatures; k++) out[k] = f[k];
+        const uint64_t m = __ballot(plain);
+        if (!m) continue;
+        unsigned base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (unsigned)__popcll(m));
+        base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
+        if (plain) list[base + lanes_below(m)] = entries[e];
     }
 }
 
-__global__ __launch_bounds__(256) void k_gate_route(const float* __restrict__ logits,
-                                                    const float* __restrict__ features, unsigned n, float thr,
-                                                    unsigned* __restrict__ list, unsigned* __restrict__ count) {
-    const unsigned q = blockIdx.x * 256 + threadIdx.x;
-    const bool act = q < n;
-    // feature 0 is log1p(kept): no gathered sample -> plain path; so does a
-    // NaN logit or feature
-    bool plain = false;
-    if (act) {
-        const float z = logits[q], f0 = features[(size_t)q * kGateFeatures];
-        plain = !(z >= thr && f0 > 0.0f);
+// Provisional colour of every hand-off entry (before the general kernel
+// overwrites qcol).
+__global__ __launch_bounds__(256) void k_gate_prov(const float4* __restrict__ qcol,
+                                                   const unsigned* __restrict__ entries,
+                                                   const unsigned* __restrict__ n_entries,
+                                                   float4* __restrict__ prov) {
+    const unsigned n = *n_entries;
+    for (unsigned e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256)
+        prov[e] = qcol[entries[e] & ~kNeeds2R];
+}
+
+// Training label of every hand-off entry: 1 when it has a provisional colour
+// and the plain method's final colour differs from it by at most tau in
+// every channel.
+__global__ __launch_bounds__(256) void k_gate_label(const float4* __restrict__ qcol,
+                                                    const unsigned* __restrict__ entries,
+                                                    const unsigned* __restrict__ n_entries,
+                                                    const float* __restrict__ features,
+                                                    const float4* __restrict__ prov, float tau,
+                                                    float* __restrict__ labels) {
+    const unsigned n = *n_entries;
+    for (unsigned e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+        const float4 a = qcol[entries[e] & ~kNeeds2R], b = prov[e];
+        const float d = fmaxf(fmaxf(fabsf(a.x - b.x), fabsf(a.y - b.y)), fmaxf(fabsf(a.z - b.z), fabsf(a.w - b.w)));
+        labels[e] = (features[(size_t)e * kGateFeatures] >= 0.0f && d <= tau) ? 1.0f : 0.0f;
     }
-    const uint64_t m = __ballot(plain);
-    if (!m) return;
-    unsigned base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (unsigned)__popcll(m));
-    base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-    if (plain) list[base + lanes_below(m)] = q;
 }
 
 // Pixel = sum of its output samples in order, normalised by alpha; red if
@@ -1415,16 +1352,39 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_query_direct(const QueryParams& p, hipStream_t s) {
+hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
-    k_query_direct<<<(p.num_queries + kDirectBlock - 1) / kDirectBlock, kDirectBlock, 0, s>>>(p);
+    const size_t slots = fast_slots(p);
+    k_query_fast_gate<<<(unsigned)((slots + kFastBlock - 1) / kFastBlock), kFastBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 
-hipError_t launch_gate_route(const float* logits, const float* features, unsigned n, float thr, unsigned* list,
-                             unsigned* count, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    k_gate_route<<<(n + 255) / 256, 256, 0, s>>>(logits, features, n, thr, list, count);
+// Launches over a device-side entry count use a grid that covers max_n
+// entries, capped; the kernels stride over the rest.
+static unsigned entry_grid(size_t max_n) {
+    const size_t b = (max_n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+hipError_t launch_gate_route(const float* logits, const float* features, const unsigned* entries,
+                             const unsigned* n_entries, size_t max_n, float thr, unsigned* list, unsigned* count,
+                             hipStream_t s) {
+    if (max_n == 0) return hipSuccess;
+    k_gate_route<<<entry_grid(max_n), 256, 0, s>>>(logits, features, entries, n_entries, thr, list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_gate_prov(const float4* qcol, const unsigned* entries, const unsigned* n_entries, size_t max_n,
+                            float4* prov, hipStream_t s) {
+    if (max_n == 0) return hipSuccess;
+    k_gate_prov<<<entry_grid(max_n), 256, 0, s>>>(qcol, entries, n_entries, prov);
+    return hipGetLastError();
+}
+
+hipError_t launch_gate_label(const float4* qcol, const unsigned* entries, const unsigned* n_entries, size_t max_n,
+                             const float* features, const float4* prov, float tau, float* labels, hipStream_t s) {
+    if (max_n == 0) return hipSuccess;
+    k_gate_label<<<entry_grid(max_n), 256, 0, s>>>(qcol, entries, n_entries, features, prov, tau, labels);
     return hipGetLastError();
 }
 

@@ -18,6 +18,10 @@ constexpr int kLeafCap = TLR_LEAF_CAP;
 constexpr int kSampleCap = 32768;   // gathered samples + merged-surface copies
 constexpr int kStackCap = 8192;
 
+// Marks an entry of the fast kernel's hand-off list whose 1R pass already
+// decided on 2R (the rest of the entry is the query id).
+constexpr unsigned kNeeds2R = 0x80000000u;
+
 constexpr int kNumStats = 13;
 enum StatIdx {
     ST_LEAF_NODES = 0, ST_SAMPLES_IN_LEAVES, ST_WITHIN_1R, ST_FETCHED, ST_SURFACES, ST_2R, ST_MERGED,
@@ -59,10 +63,8 @@ struct QueryParams {
     int fast_list;
     // batched launches: the buffer index of each fail_list entry (else null)
     unsigned* fail_item;
-    // fast kernel, optional: 1 where the output sample has exactly one
-    // surface at 1R and no 2R fetch (the gate's training label)
-    float* labels;
-    // direct kernel: per query features (kGateIn floats) and colours
+    // gated fast kernel: kGateFeatures floats per hand-off entry (row = the
+    // entry's position in fail_list)
     float* features;
     // fast kernel over all queries: 0 = lanes in query order (pixel-major);
     // 1 = each wave takes one output-sample index over an 8x8 pixel tile
@@ -128,19 +130,29 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s);
 // (ReconstructionOur.cpp:99,204: single surface, or a first surface covering
 // all four quadrants) and appends the others to fail_list.
 hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
-// Direct filter for the learned gate: one lane per output sample, 1R
-// gather in traversal order, tent filter over every gathered sample without
-// surface clustering, plus kGateIn features; writes qcol and features.
-hipError_t launch_query_direct(const QueryParams& p, hipStream_t s);
+// Gated fast kernel: launch_query_fast (no stats) plus, for every hand-off
+// with a first surface at 1R, its filtered colour in qcol (provisional) and,
+// for every hand-off entry, its gate features in p.features.
+hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s);
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
 // Counter calibration: kind 0 streams n float4 from src to dst; kinds 1 / 2
 // gather n records of 2 / 4 float4 from src in the order (i * odd) mod n (n
 // a power of two) and write one float4 per record to dst.
 hipError_t launch_calibrate(int kind, const float4* src, float4* dst, size_t n, hipStream_t s);
-// Routing after the gate: queries whose logit is below the threshold, or
-// with no gathered sample, are appended to list (for the plain path).
-hipError_t launch_gate_route(const float* logits, const float* features, unsigned n, float threshold,
-                             unsigned* list, unsigned* count, hipStream_t s);
+// Routing after the gate over the hand-off entries (*n_entries on the
+// device, at most max_n): entries whose logit is below the threshold, or with
+// no provisional colour, are appended to list (for the general kernel).
+hipError_t launch_gate_route(const float* logits, const float* features, const unsigned* entries,
+                             const unsigned* n_entries, size_t max_n, float threshold, unsigned* list,
+                             unsigned* count, hipStream_t s);
+// Provisional colour of each hand-off entry (copied before the general
+// kernel overwrites qcol).
+hipError_t launch_gate_prov(const float4* qcol, const unsigned* entries, const unsigned* n_entries, size_t max_n,
+                            float4* prov, hipStream_t s);
+// Label of each entry: 1 if it has a provisional colour within tau (every
+// channel) of the plain method's final colour.
+hipError_t launch_gate_label(const float4* qcol, const unsigned* entries, const unsigned* n_entries, size_t max_n,
+                             const float* features, const float4* prov, float tau, float* labels, hipStream_t s);
 
 // The reference's GPU-variant method (ReconstructionCudaKernels.cu:242-868):
 // stage launches over output samples and lists of them.

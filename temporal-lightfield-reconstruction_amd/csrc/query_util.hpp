@@ -85,4 +85,19 @@ __device__ __forceinline__ bool surely_outside(const float4 a, const float4 b, c
     return __builtin_fmaf(dx, dx, dy * dy) > lim * lim;
 }
 
+// Squared screen distance of a tree sample from the output sample at the
+// output's (u,v,t), with the hardware reciprocal: an estimate for features,
+// not for decisions.
+__device__ __forceinline__ float approx_d2(const float4 a, const float4 b, const OSample& o, float cc0, float cc1) {
+    const float dt = o.t - a.w;
+    const float Px = a.x * a.z + dt * b.x;
+    const float Py = a.y * a.z + dt * b.y;
+    const float Pw = a.z + dt * b.z;
+    const float r = __builtin_amdgcn_rcpf(Pw);
+    const float coc = cc0 * r + cc1;
+    const float dx = __builtin_fmaf(coc, o.u, Px * r) - o.x;
+    const float dy = __builtin_fmaf(coc, o.v, Py * r) - o.y;
+    return __builtin_fmaf(dx, dx, dy * dy);
+}
+
 }  // namespace tlr

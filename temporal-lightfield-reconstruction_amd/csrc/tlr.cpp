@@ -109,6 +109,7 @@ struct tlr_ctx {
     float* features = nullptr;      // gate: per query features
     float* logits = nullptr;
     float* labels = nullptr;
+    float4* prov = nullptr;         // gate: provisional colour per hand-off entry
     unsigned* route_list = nullptr;
     unsigned* d_route_count = nullptr;
     // reference GPU-variant method (path 2)
@@ -536,7 +537,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
     size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
                     (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
-    if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 3) + 4 * 256;
+    if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 3) + nq * sizeof(float4) + 5 * 256;
     HIPCHK(c, c->qarena.reserve(qbytes));
     c->qarena.reset();
     c->d_pattern = c->qarena.alloc<float>(pat.size());
@@ -545,11 +546,13 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     c->fail_list = c->qarena.alloc<unsigned>(nq);
     c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
     c->features = c->logits = c->labels = nullptr;
+    c->prov = nullptr;
     c->route_list = nullptr;
     if (gate_buffers) {
         c->features = c->qarena.alloc<float>(nq * kGateFeatures);
         c->logits = c->qarena.alloc<float>(nq);
         c->labels = c->qarena.alloc<float>(nq);
+        c->prov = c->qarena.alloc<float4>(nq);
         c->route_list = c->qarena.alloc<unsigned>(nq);
     }
     if (c->qarena.failed) return c->fail(TLR_E_HIP, "query arena exhausted");
@@ -614,7 +617,6 @@ static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* l
     QueryParams G = Q0;
     G.qlist = Q0.fail_list;
     G.qcount = Q0.fail_count;
-    G.labels = nullptr;
     HIPCHK(c, launch_query_general(G, s));
     return TLR_OK;
 }
@@ -751,17 +753,21 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
 
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     if (p->gate) {
-        // direct filter + features for every output sample, gate, then the
-        // plain method for the rejected ones
-        QueryParams D = Q;
-        D.features = c->features;
-        HIPCHK(c, launch_query_direct(D, s));
+        // the 1R lane kernel with provisional first-surface colours and
+        // features for its hand-offs; the gate scores the hand-offs; the
+        // general kernel runs only for the ones it does not accept
+        QueryParams F = Q;
+        F.features = c->features;
+        HIPCHK(c, launch_query_fast_gate(F, s));
+        HIPCHK(c, hipEventRecord(c->ev[5], s));
+        HIPCHK(c, launch_gate_forward_dev(c->features, c->d_fail_count, nq, c->gate_p, c->logits, s));
+        HIPCHK(c, launch_gate_route(c->logits, c->features, c->fail_list, c->d_fail_count, nq, c->gate_thr,
+                                    c->route_list, c->d_route_count, s));
         HIPCHK(c, hipEventRecord(c->ev[6], s));
-        HIPCHK(c, launch_gate_forward(c->features, (int)nq, c->gate_p, c->logits, s));
-        HIPCHK(c, launch_gate_route(c->logits, c->features, (unsigned)nq, c->gate_thr, c->route_list,
-                                    c->d_route_count, s));
-        HIPCHK(c, hipEventRecord(c->ev[7], s));
-        st = run_plain(c, Q, c->route_list, c->d_route_count, false, 0);
+        QueryParams G = Q;
+        G.qlist = c->route_list;
+        G.qcount = c->d_route_count;
+        HIPCHK(c, launch_query_general(G, s));
     } else if (p->path == 2) {
         st = run_refgpu(c, Q, p->list_cap, &n2r);
     } else {
@@ -799,16 +805,16 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) stats->fast_ms = ms;
         if (p->gate) {
-            if (hipEventElapsedTime(&ms, c->ev[2], c->ev[6]) == hipSuccess) stats->direct_ms = ms;
-            if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) stats->gate_ms = ms;
-            if (hipEventElapsedTime(&ms, c->ev[7], c->ev[5]) == hipSuccess) stats->fast_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) stats->gate_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev[6], c->ev[3]) == hipSuccess) stats->general_ms = ms;
+            stats->gate_candidates = (double)nfail;
             stats->gate_rejected = (double)nroute;
-        } else if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) {
-            stats->fast_ms = ms;
+        } else if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) {
+            stats->general_ms = ms;
         }
-        if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) stats->general_ms = ms;
-        stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)nfail;
+        stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)(p->gate ? nroute : nfail);
         if (p->path == 2) {
             stats->num_2r = stats->general_queries = (double)n2r;
             stats->count = (double)nq;
@@ -1149,47 +1155,61 @@ tlr_status tlr_gate_grad(tlr_ctx* c, const float* X, const float* y, int64_t n, 
     return TLR_OK;
 }
 
-// Runs the plain method (with labels) and the direct filter (with features)
-// over every output sample of the current build.
-static tlr_status gate_pass(tlr_ctx* c, const tlr_recon_params* p, float* plain_dev) {
+// Training data of the gate, over the current build: the gated 1R lane
+// kernel (provisional first-surface colours and features of its hand-offs),
+// the provisional colours kept aside, the general kernel over every hand-off
+// (the plain method's result), then each entry's label: 1 when its
+// provisional colour is within kGateLabelTau of the plain one in every
+// channel.  *n_entries: hand-off entries (rows).
+constexpr float kGateLabelTau = 1e-3f;
+
+static tlr_status gate_pass(tlr_ctx* c, const tlr_recon_params* p, unsigned* n_entries) {
+    if (p->path != 0) return c->fail(TLR_E_INVALID, "gate data come from the plain method (path 0)");
     tlr_recon_params pp = *p;
     pp.collect_stats = 0;
     pp.gate = 0;
     QueryParams Q;
     tlr_status st = prepare_queries(c, &pp, true, Q);
     if (st != TLR_OK) return st;
-    QueryParams L = Q;
-    L.labels = c->labels;
-    st = run_plain(c, L, nullptr, nullptr, false, 0);
-    if (st != TLR_OK) return st;
-    if (plain_dev)
-        HIPCHK(c, hipMemcpyAsync(plain_dev, c->qcol, c->qcol_n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    QueryParams D = Q;
-    D.features = c->features;
-    HIPCHK(c, launch_query_direct(D, c->stream));
+    hipStream_t s = c->stream;
+    const size_t nq = c->qcol_n;
+    QueryParams F = Q;
+    F.features = c->features;
+    HIPCHK(c, launch_query_fast_gate(F, s));
+    HIPCHK(c, launch_gate_prov(c->qcol, c->fail_list, c->d_fail_count, nq, c->prov, s));
+    QueryParams G = Q;
+    G.qlist = c->fail_list;
+    G.qcount = c->d_fail_count;
+    HIPCHK(c, launch_query_general(G, s));
+    HIPCHK(c, launch_gate_label(c->qcol, c->fail_list, c->d_fail_count, nq, c->features, c->prov, kGateLabelTau,
+                                c->labels, s));
+    *n_entries = read_u32(c, c->d_fail_count);
+    unsigned long long ovf = 0;
+    HIPCHK(c, hipMemcpy(&ovf, c->d_stats + ST_OVERFLOW, sizeof(ovf), hipMemcpyDeviceToHost));
+    if (ovf) return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity", ovf);
     return TLR_OK;
 }
 
-tlr_status tlr_gate_collect(tlr_ctx* c, const tlr_recon_params* p, int stride, float* X, float* y, float* direct,
-                            float* plain, int64_t* n) {
-    if (!c || !p || stride <= 0) return TLR_E_INVALID;
-    float* plain_dev = nullptr;
-    if (plain) {
-        const size_t nq = (size_t)c->h.width * c->h.height * (p->override_uvt ? 1 : p->out_spp);
-        HIPCHK(c, hipMalloc(&plain_dev, nq * sizeof(float4)));
-    }
-    tlr_status st = gate_pass(c, p, plain_dev);
-    if (st != TLR_OK) {
-        if (plain_dev) (void)hipFree(plain_dev);
-        return st;
-    }
+tlr_status tlr_gate_collect(tlr_ctx* c, const tlr_recon_params* p, int stride, int64_t cap, float* X, float* y,
+                            float* prov, float* plain, uint32_t* query, int64_t* n) {
+    if (!c || !p || stride <= 0 || cap < 0) return TLR_E_INVALID;
+    unsigned ne = 0;
+    tlr_status st = gate_pass(c, p, &ne);
+    if (st != TLR_OK) return st;
     hipStream_t s = c->stream;
-    const size_t rows = (c->qcol_n + stride - 1) / stride;
+    const size_t rows_all = ((size_t)ne + stride - 1) / stride;
+    const size_t rows = std::min(rows_all, (size_t)cap);
+    if (n) *n = (int64_t)rows_all;
+    if (rows == 0) return TLR_OK;
     hipError_t e = hipSuccess;
-    // every stride-th row, compacted on the device, then one copy per array
-    // (a pitched copy of millions of short rows is slow)
+    // every stride-th entry, compacted on the device, then one copy per
+    // array (a pitched copy of millions of short rows is slow); the plain
+    // colours are gathered per entry from qcol first
     void* tmp = nullptr;
+    float4* plain_e = nullptr;
     e = hipMalloc(&tmp, rows * sizeof(float) * kGateFeatures);
+    if (e == hipSuccess && plain) e = hipMalloc(&plain_e, (size_t)ne * sizeof(float4));
+    if (e == hipSuccess && plain) e = launch_gate_prov(c->qcol, c->fail_list, c->d_fail_count, ne, plain_e, s);
     auto take = [&](const void* src, void* dst, int words) {
         if (!dst || e != hipSuccess) return;
         e = launch_take_rows(src, tmp, rows, words, stride, s);
@@ -1198,21 +1218,24 @@ tlr_status tlr_gate_collect(tlr_ctx* c, const tlr_recon_params* p, int stride, f
     };
     take(c->features, X, kGateFeatures);
     take(c->labels, y, 1);
-    take(c->qcol, direct, 4);
-    if (plain_dev) take(plain_dev, plain, 4);
+    take(c->prov, prov, 4);
+    take(plain_e, plain, 4);
+    take(c->fail_list, query, 1);
     if (tmp) (void)hipFree(tmp);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (plain_dev) (void)hipFree(plain_dev);
+    if (plain_e) (void)hipFree(plain_e);
     if (e != hipSuccess) return c->fail(TLR_E_HIP, "gate collect: %s", hipGetErrorString(e));
-    if (n) *n = (int64_t)rows;
+    if (query)
+        for (size_t r = 0; r < rows; r++) query[r] &= ~kNeeds2R;
     return TLR_OK;
 }
 
 tlr_status tlr_gate_append_training(tlr_ctx* c, const tlr_recon_params* p, int stride) {
     if (!c || !p || stride <= 0) return TLR_E_INVALID;
-    tlr_status st = gate_pass(c, p, nullptr);
+    unsigned ne = 0;
+    tlr_status st = gate_pass(c, p, &ne);
     if (st != TLR_OK) return st;
-    const int64_t rows = (int64_t)((c->qcol_n + stride - 1) / stride);
+    const int64_t rows = (int64_t)(((size_t)ne + stride - 1) / stride);
+    if (rows == 0) return TLR_OK;
     if (c->train_n + rows > c->train_cap) {
         const int64_t cap = std::max<int64_t>(c->train_n + rows, c->train_cap * 2);
         float *nX = nullptr, *ny = nullptr;
