@@ -460,7 +460,7 @@ inline int tcmp(float a, float b, float eps) {
     else if (a < b) return -1;
     else return 1;
 }
-inline bool teq2(int a, int b) { return a == 0 || b == 0 || a == b; }
+inline bool teq2(int a, int b) { return ORC_M(2) ? a == b : (a == 0 || b == 0 || a == b); }
 inline bool teq4(int a, int b, int c, int d) {
     int lo = std::min(std::min(std::min(a, b), c), d);
     int hi = std::max(std::max(std::max(a, b), c), d);
@@ -483,6 +483,7 @@ bool same_surface(const TLB& n0, const TLB& n1, const float uvtmin[3], const flo
                teq2(tcmp(Xmax00, Xmax01, T), tcmp(Xmax10, Xmax11, T)) &&
                teq2(tcmp(Ymax00, Ymax01, T), tcmp(Ymax10, Ymax11, T));
     if (!coc) return false;
+    if (ORC_M(1)) return true;
     float xa0 = n0.eval(0, uvtmin[2], 0.0f), xb0 = n0.eval(0, uvtmax[2], 0.0f);
     float ya0 = n0.eval(2, uvtmin[2], 0.0f), yb0 = n0.eval(2, uvtmax[2], 0.0f);
     float Xa0 = n0.eval(1, uvtmin[2], 0.0f), Xb0 = n0.eval(1, uvtmax[2], 0.0f);
@@ -704,6 +705,77 @@ struct LeafScratch {
     std::vector<int> surf;
 };
 
+// Surface scan of one initial leaf's candidates (sorted by (t, w)):
+// ReconstructionTreeBuilder.cpp:222-340, keeping the cocbounds[0] typo at
+// :334,336.  tof(k): candidate k's time; ba/bb/ba1/bb1: its screen boxes
+// (2 floats each) at t=0 and t=1; surf[k] receives its surface id.
+template <class TOf>
+void surface_scan(int n, TOf tof, const float* ba, const float* bb, const float* ba1, const float* bb1, int* surf) {
+    if (n <= 0) return;
+    int sb = 0, cur = 0;
+    surf[0] = 0;
+    bool reset = true;
+    float cb0[2] = {0, 0}, cb1[2] = {0, 0}, mvmin[2] = {0, 0}, mvmax[2] = {0, 0};
+    float st = 0.0f;
+    for (int k = 1; k < n; ++k) {
+        if (reset) {
+            cb0[0] = cb0[1] = 0.5f * (ba[2 * sb] - bb[2 * sb]);
+            cb1[0] = cb1[1] = 0.5f * (ba1[2 * sb] - bb1[2 * sb]);
+            mvmin[0] = 0.5f * (ba1[2 * sb] + bb1[2 * sb]) - 0.5f * (ba[2 * sb] + bb[2 * sb]);
+            mvmin[1] = 0.5f * (ba1[2 * sb + 1] + bb1[2 * sb + 1]) - 0.5f * (ba[2 * sb + 1] + bb[2 * sb + 1]);
+            mvmax[0] = mvmin[0]; mvmax[1] = mvmin[1];
+            st = tof(sb);
+            reset = false;
+        }
+        bool conflict = false;
+        bool early = false;
+        const float T = 1 * 2.0f;
+        if (!ORC_M(4) && st != tof(k)) {
+            sb = k;
+            surf[k] = ++cur;
+            reset = true;
+            continue;
+        }
+        float c0 = 0.5f * (ba[2 * k] - bb[2 * k]);
+        float c1 = 0.5f * (ba1[2 * k] - bb1[2 * k]);
+        float mv[2];
+        mv[0] = 0.5f * (ba1[2 * k] + bb1[2 * k]) - 0.5f * (ba[2 * k] + bb[2 * k]);
+        mv[1] = 0.5f * (ba1[2 * k + 1] + bb1[2 * k + 1]) - 0.5f * (ba[2 * k + 1] + bb[2 * k + 1]);
+        float dmv_a = mx(std::fabs(mvmin[0] - mv[0]), std::fabs(mvmin[1] - mv[1]));
+        float dmv_b = mx(std::fabs(mvmax[0] - mv[0]), std::fabs(mvmax[1] - mv[1]));
+        if (mx(std::fabs(c0 - cb0[0]), std::fabs(c0 - cb0[1])) < 0.5f * T &&
+            mx(std::fabs(c1 - cb1[0]), std::fabs(c1 - cb1[1])) < 0.5f * T &&
+            mx(dmv_a, dmv_b) < T)
+            early = true;
+        if (!early) {
+            for (int j = sb; j < k; ++j) {
+                bool bX = teq4(tcmp(ba[2 * j], ba[2 * k], T), tcmp(bb[2 * j], bb[2 * k], T),
+                               tcmp(ba1[2 * j], ba1[2 * k], T), tcmp(bb1[2 * j], bb1[2 * k], T));
+                bool bY = teq4(tcmp(ba[2 * j + 1], ba[2 * k + 1], T), tcmp(bb[2 * j + 1], bb[2 * k + 1], T),
+                               tcmp(ba1[2 * j + 1], ba1[2 * k + 1], T),
+                               tcmp(bb1[2 * j + 1], bb1[2 * k + 1], T));
+                if (!(bX && bY)) {
+                    sb = k;
+                    surf[k] = ++cur;
+                    conflict = true;
+                    reset = true;
+                    break;
+                }
+            }
+        }
+        if (!conflict) {
+            surf[k] = cur;
+            cb0[0] = mn(c0, cb0[0]);
+            cb0[1] = ORC_M(3) ? mx(c0, cb0[1]) : mx(c0, cb0[0]);
+            cb1[0] = mn(c1, cb1[0]);
+            cb1[1] = ORC_M(3) ? mx(c1, cb1[1]) : mx(c1, cb1[0]);
+            mvmin[0] = mn(mvmin[0], mv[0]); mvmin[1] = mn(mvmin[1], mv[1]);
+            mvmax[0] = mx(mvmax[0], mv[0]); mvmax[1] = mx(mvmax[1], mv[1]);
+        }
+    }
+
+}
+
 void build_leaf_cell(orc_ctx* c, int inode, int offset, LeafScratch& sc, std::vector<Leaf>& out) {
     const INode& in = c->inodes[inode];
     std::vector<RS>& cand = sc.cand;
@@ -768,73 +840,8 @@ void build_leaf_cell(orc_ctx* c, int inode, int offset, LeafScratch& sc, std::ve
         }
     }
 
-    // Surface scan: ReconstructionTreeBuilder.cpp:222-340 (keeps the
-    // cocbounds[0] typo at :334,336).
-    int sb = 0, cur = 0;
-    sc.surf[0] = 0;
-    bool reset = true;
-    float cb0[2] = {0, 0}, cb1[2] = {0, 0}, mvmin[2] = {0, 0}, mvmax[2] = {0, 0};
-    float st = 0.0f;
-    const float* ba = sc.ba.data();
-    const float* bb = sc.bb.data();
-    const float* ba1 = sc.ba1.data();
-    const float* bb1 = sc.bb1.data();
-    for (int k = 1; k < n; ++k) {
-        if (reset) {
-            cb0[0] = cb0[1] = 0.5f * (ba[2 * sb] - bb[2 * sb]);
-            cb1[0] = cb1[1] = 0.5f * (ba1[2 * sb] - bb1[2 * sb]);
-            mvmin[0] = 0.5f * (ba1[2 * sb] + bb1[2 * sb]) - 0.5f * (ba[2 * sb] + bb[2 * sb]);
-            mvmin[1] = 0.5f * (ba1[2 * sb + 1] + bb1[2 * sb + 1]) - 0.5f * (ba[2 * sb + 1] + bb[2 * sb + 1]);
-            mvmax[0] = mvmin[0]; mvmax[1] = mvmin[1];
-            st = cand[sb].t;
-            reset = false;
-        }
-        bool conflict = false;
-        bool early = false;
-        const float T = 1 * 2.0f;
-        if (st != cand[k].t) {
-            sb = k;
-            sc.surf[k] = ++cur;
-            reset = true;
-            continue;
-        }
-        float c0 = 0.5f * (ba[2 * k] - bb[2 * k]);
-        float c1 = 0.5f * (ba1[2 * k] - bb1[2 * k]);
-        float mv[2];
-        mv[0] = 0.5f * (ba1[2 * k] + bb1[2 * k]) - 0.5f * (ba[2 * k] + bb[2 * k]);
-        mv[1] = 0.5f * (ba1[2 * k + 1] + bb1[2 * k + 1]) - 0.5f * (ba[2 * k + 1] + bb[2 * k + 1]);
-        float dmv_a = mx(std::fabs(mvmin[0] - mv[0]), std::fabs(mvmin[1] - mv[1]));
-        float dmv_b = mx(std::fabs(mvmax[0] - mv[0]), std::fabs(mvmax[1] - mv[1]));
-        if (mx(std::fabs(c0 - cb0[0]), std::fabs(c0 - cb0[1])) < 0.5f * T &&
-            mx(std::fabs(c1 - cb1[0]), std::fabs(c1 - cb1[1])) < 0.5f * T &&
-            mx(dmv_a, dmv_b) < T)
-            early = true;
-        if (!early) {
-            for (int j = sb; j < k; ++j) {
-                bool bX = teq4(tcmp(ba[2 * j], ba[2 * k], T), tcmp(bb[2 * j], bb[2 * k], T),
-                               tcmp(ba1[2 * j], ba1[2 * k], T), tcmp(bb1[2 * j], bb1[2 * k], T));
-                bool bY = teq4(tcmp(ba[2 * j + 1], ba[2 * k + 1], T), tcmp(bb[2 * j + 1], bb[2 * k + 1], T),
-                               tcmp(ba1[2 * j + 1], ba1[2 * k + 1], T),
-                               tcmp(bb1[2 * j + 1], bb1[2 * k + 1], T));
-                if (!(bX && bY)) {
-                    sb = k;
-                    sc.surf[k] = ++cur;
-                    conflict = true;
-                    reset = true;
-                    break;
-                }
-            }
-        }
-        if (!conflict) {
-            sc.surf[k] = cur;
-            cb0[0] = mn(c0, cb0[0]);
-            cb0[1] = mx(c0, cb0[0]);
-            cb1[0] = mn(c1, cb1[0]);
-            cb1[1] = mx(c1, cb1[0]);
-            mvmin[0] = mn(mvmin[0], mv[0]); mvmin[1] = mn(mvmin[1], mv[1]);
-            mvmax[0] = mx(mvmax[0], mv[0]); mvmax[1] = mx(mvmax[1], mv[1]);
-        }
-    }
+    surface_scan(n, [&](int k) { return cand[k].t; }, sc.ba.data(), sc.bb.data(), sc.ba1.data(), sc.bb1.data(),
+                 sc.surf.data());
 
     // Leaf emission: ReconstructionTreeBuilder.cpp:345-378
     int curIdx = offset;
@@ -1136,14 +1143,14 @@ struct Filt {
         res.nsurf1 = nsurf;
         res.n1 = n1;
         int n2 = 0, nN = 0;
-        const bool f2 = nsurf == 0 || (surf[0].quad != 0xF && nsurf > 1);
+        const bool f2 = nsurf == 0 || ((ORC_M(5) || surf[0].quad != 0xF) && nsurf > 1);
         if (f2) {
             n2 = collect(o, disp * 2, st);
             if (n2 == 0) {
                 st.v[7] += 1;
                 nN = n2;
                 for (int k = 0; k < 3 && !nN; k++) {
-                    disp *= 2.f;
+                    if (!ORC_M(7)) disp *= 2.f;
                     nN = collect(o, disp * 2, st);
                     res.emerg++;
                 }
@@ -1155,7 +1162,7 @@ struct Filt {
         res.nfinal = std::max(std::max(n1, n2), nN);
 
         int numSurfaces = nsurf;
-        if (f2)
+        if (f2 && !ORC_M(6))
             for (int si = 0; si < nsurf - 1; si++) {
                 Surface& src = surf[si];
                 if ((int)src.s.size() < 4) {
@@ -1231,7 +1238,7 @@ struct Filt {
         }
         const RSamp& na = surf[nI0].s[nI1];
         const RS& ns = c->samples[na.index];
-        if (col[3] == 0) {
+        if (col[3] == 0 && !ORC_M(8)) {
             col[0] = ns.r; col[1] = ns.g; col[2] = ns.b; col[3] = 1.0f;
             res.nearest = 1;
         }

@@ -681,6 +681,7 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
         bool skip1r = false;
         if (Q.qlist) {
             qi = (unsigned)uni((int)Q.qlist[qi]);
+            if (qi == kSkipEntry) continue;  // accepted by the learned gate
             // top bit: the fast kernel already found that 2R is needed
             skip1r = (qi & kNeeds2R) != 0;
             qi &= ~kNeeds2R;
@@ -724,10 +725,8 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
                     dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
             }
             do {
-                unsigned qi = (unsigned)uni((int)B.list[e]);
-                const bool skip1r = (qi & kNeeds2R) != 0;
-                qi &= ~kNeeds2R;
-                general_query(Q, m, qi, skip1r, st);
+                const unsigned ent = (unsigned)uni((int)B.list[e]);
+                if (ent != kSkipEntry) general_query(Q, m, ent & ~kNeeds2R, (ent & kNeeds2R) != 0, st);
                 e++;
             } while (e < end && uni((int)B.list_item[e]) == item);
         }
@@ -1209,30 +1208,28 @@ __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
                                   lid);
 }
 
-// Learned-gate routing over the fast kernel's hand-off list: entry e keeps
+// Learned-gate decision over the fast kernel's hand-off list: entry e keeps
 // its provisional first-surface colour when the gate's logit reaches the
-// threshold and the entry has one (feature 0 >= 0); otherwise the entry (with
-// its 2R marker) is appended to `list` for the general kernel.  A NaN logit
-// goes to the general kernel.
-__global__ __launch_bounds__(256) void k_gate_route(const float* __restrict__ logits,
-                                                    const float* __restrict__ features,
-                                                    const unsigned* __restrict__ entries,
-                                                    const unsigned* __restrict__ n_entries, float thr,
-                                                    unsigned* __restrict__ list, unsigned* __restrict__ count) {
+// threshold and the entry has one (feature 0 >= 0); it is then replaced by
+// kSkipEntry in place, so the general kernel walks the list in its original
+// (spatially coherent) order and skips it.  A NaN logit keeps the entry.
+// *accepted counts the skipped entries.
+__global__ __launch_bounds__(256) void k_gate_accept(const float* __restrict__ logits,
+                                                     const float* __restrict__ features,
+                                                     unsigned* __restrict__ entries,
+                                                     const unsigned* __restrict__ n_entries, float thr,
+                                                     unsigned* __restrict__ accepted) {
     const unsigned n = *n_entries;
     for (unsigned e0 = blockIdx.x * 256; e0 < n; e0 += gridDim.x * 256) {
         const unsigned e = e0 + threadIdx.x;
-        bool plain = false;
+        bool acc = false;
         if (e < n) {
             const float z = logits[e], f0 = features[(size_t)e * kGateFeatures];
-            plain = !(z >= thr && f0 >= 0.0f);
+            acc = z >= thr && f0 >= 0.0f;
         }
-        const uint64_t m = __ballot(plain);
-        if (!m) continue;
-        unsigned base = 0;
-        if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (unsigned)__popcll(m));
-        base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-        if (plain) list[base + lanes_below(m)] = entries[e];
+        if (acc) entries[e] = kSkipEntry;
+        const uint64_t m = __ballot(acc);
+        if (m && (threadIdx.x & 63) == 0) atomicAdd(accepted, (unsigned)__popcll(m));
     }
 }
 
@@ -1366,11 +1363,11 @@ static unsigned entry_grid(size_t max_n) {
     return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
 
-hipError_t launch_gate_route(const float* logits, const float* features, const unsigned* entries,
-                             const unsigned* n_entries, size_t max_n, float thr, unsigned* list, unsigned* count,
-                             hipStream_t s) {
+hipError_t launch_gate_accept(const float* logits, const float* features, unsigned* entries,
+                              const unsigned* n_entries, size_t max_n, float thr, unsigned* accepted,
+                              hipStream_t s) {
     if (max_n == 0) return hipSuccess;
-    k_gate_route<<<entry_grid(max_n), 256, 0, s>>>(logits, features, entries, n_entries, thr, list, count);
+    k_gate_accept<<<entry_grid(max_n), 256, 0, s>>>(logits, features, entries, n_entries, thr, accepted);
     return hipGetLastError();
 }
 

@@ -21,6 +21,9 @@ constexpr int kStackCap = 8192;
 // Marks an entry of the fast kernel's hand-off list whose 1R pass already
 // decided on 2R (the rest of the entry is the query id).
 constexpr unsigned kNeeds2R = 0x80000000u;
+// A hand-off entry the learned gate accepted (the general kernel skips it);
+// query ids stay below 2^31 - 1, so it never names a query.
+constexpr unsigned kSkipEntry = 0xFFFFFFFFu;
 
 constexpr int kNumStats = 13;
 enum StatIdx {
@@ -139,12 +142,12 @@ hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rg
 // gather n records of 2 / 4 float4 from src in the order (i * odd) mod n (n
 // a power of two) and write one float4 per record to dst.
 hipError_t launch_calibrate(int kind, const float4* src, float4* dst, size_t n, hipStream_t s);
-// Routing after the gate over the hand-off entries (*n_entries on the
-// device, at most max_n): entries whose logit is below the threshold, or with
-// no provisional colour, are appended to list (for the general kernel).
-hipError_t launch_gate_route(const float* logits, const float* features, const unsigned* entries,
-                             const unsigned* n_entries, size_t max_n, float threshold, unsigned* list,
-                             unsigned* count, hipStream_t s);
+// The gate's decision over the hand-off entries (*n_entries on the device,
+// at most max_n): entries whose logit reaches the threshold and that have a
+// provisional colour become kSkipEntry in place; *accepted counts them.
+hipError_t launch_gate_accept(const float* logits, const float* features, unsigned* entries,
+                              const unsigned* n_entries, size_t max_n, float threshold, unsigned* accepted,
+                              hipStream_t s);
 // Provisional colour of each hand-off entry (copied before the general
 // kernel overwrites qcol).
 hipError_t launch_gate_prov(const float4* qcol, const unsigned* entries, const unsigned* n_entries, size_t max_n,

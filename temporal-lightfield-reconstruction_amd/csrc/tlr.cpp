@@ -110,8 +110,7 @@ struct tlr_ctx {
     float* logits = nullptr;
     float* labels = nullptr;
     float4* prov = nullptr;         // gate: provisional colour per hand-off entry
-    unsigned* route_list = nullptr;
-    unsigned* d_route_count = nullptr;
+    unsigned* d_accept_count = nullptr;
     // reference GPU-variant method (path 2)
     DeviceArena rarena;             // lists, counters, stage codes, first 2R lists
     DeviceArena rwide;              // lists of the overflow launches
@@ -197,7 +196,7 @@ tlr_status tlr_create(tlr_ctx** out, int device) {
         hipMalloc(&c->d_stats, sizeof(unsigned long long) * kNumStats) != hipSuccess ||
         hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&c->d_fail_count, sizeof(unsigned)) != hipSuccess ||
-        hipMalloc(&c->d_route_count, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->d_accept_count, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&c->gate_p, sizeof(float) * kGateParams) != hipSuccess ||
         hipMalloc(&c->gate_m, sizeof(float) * kGateParams) != hipSuccess ||
         hipMalloc(&c->gate_v, sizeof(float) * kGateParams) != hipSuccess ||
@@ -223,7 +222,7 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_counter) (void)hipFree(c->d_counter);
     if (c->d_fail_count) (void)hipFree(c->d_fail_count);
-    if (c->d_route_count) (void)hipFree(c->d_route_count);
+    if (c->d_accept_count) (void)hipFree(c->d_accept_count);
     for (float* p : {c->gate_p, c->gate_m, c->gate_v, c->gate_g, c->gate_loss, c->train_X, c->train_y})
         if (p) (void)hipFree(p);
     if (c->train_idx) (void)hipFree(c->train_idx);
@@ -537,7 +536,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
     size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
                     (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
-    if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 3) + nq * sizeof(float4) + 5 * 256;
+    if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 2) + nq * sizeof(float4) + 5 * 256;
     HIPCHK(c, c->qarena.reserve(qbytes));
     c->qarena.reset();
     c->d_pattern = c->qarena.alloc<float>(pat.size());
@@ -547,13 +546,11 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
     c->features = c->logits = c->labels = nullptr;
     c->prov = nullptr;
-    c->route_list = nullptr;
     if (gate_buffers) {
         c->features = c->qarena.alloc<float>(nq * kGateFeatures);
         c->logits = c->qarena.alloc<float>(nq);
         c->labels = c->qarena.alloc<float>(nq);
         c->prov = c->qarena.alloc<float4>(nq);
-        c->route_list = c->qarena.alloc<unsigned>(nq);
     }
     if (c->qarena.failed) return c->fail(TLR_E_HIP, "query arena exhausted");
     c->qcol_n = nq;
@@ -563,7 +560,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
     HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
     HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
-    HIPCHK(c, hipMemsetAsync(c->d_route_count, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(c->d_accept_count, 0, sizeof(unsigned), s));
 
     memset(&Q, 0, sizeof(Q));
     Q.W = W; Q.H = H; Q.M = M;
@@ -761,12 +758,12 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         HIPCHK(c, launch_query_fast_gate(F, s));
         HIPCHK(c, hipEventRecord(c->ev[5], s));
         HIPCHK(c, launch_gate_forward_dev(c->features, c->d_fail_count, nq, c->gate_p, c->logits, s));
-        HIPCHK(c, launch_gate_route(c->logits, c->features, c->fail_list, c->d_fail_count, nq, c->gate_thr,
-                                    c->route_list, c->d_route_count, s));
+        HIPCHK(c, launch_gate_accept(c->logits, c->features, c->fail_list, c->d_fail_count, nq, c->gate_thr,
+                                     c->d_accept_count, s));
         HIPCHK(c, hipEventRecord(c->ev[6], s));
         QueryParams G = Q;
-        G.qlist = c->route_list;
-        G.qcount = c->d_route_count;
+        G.qlist = c->fail_list;
+        G.qcount = c->d_fail_count;
         HIPCHK(c, launch_query_general(G, s));
     } else if (p->path == 2) {
         st = run_refgpu(c, Q, p->list_cap, &n2r);
@@ -781,10 +778,10 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     if (!out_on_device && out)
         HIPCHK(c, hipMemcpyAsync(out, c->d_img, (size_t)W * H * 16, hipMemcpyDeviceToHost, s));
     unsigned long long hs[kNumStats];
-    unsigned nfail = 0, nroute = 0;
+    unsigned nfail = 0, naccept = 0;
     HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(&nfail, c->d_fail_count, sizeof(nfail), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(&nroute, c->d_route_count, sizeof(nroute), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&naccept, c->d_accept_count, sizeof(naccept), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (stats) {
         memset(stats, 0, sizeof(*stats));
@@ -810,11 +807,11 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
             if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) stats->gate_ms = ms;
             if (hipEventElapsedTime(&ms, c->ev[6], c->ev[3]) == hipSuccess) stats->general_ms = ms;
             stats->gate_candidates = (double)nfail;
-            stats->gate_rejected = (double)nroute;
+            stats->gate_rejected = (double)(nfail - naccept);
         } else if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) {
             stats->general_ms = ms;
         }
-        stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)(p->gate ? nroute : nfail);
+        stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)(p->gate ? nfail - naccept : nfail);
         if (p->path == 2) {
             stats->num_2r = stats->general_queries = (double)n2r;
             stats->count = (double)nq;
