@@ -35,12 +35,40 @@
  *       nearest search is nearest over every leaf its bounds reach; 1R first
  *       surface and 2R surfaces equal float64 tent means on a single plane;
  *       near plane first with two planes present         [test_oracle_refgpu]
- *       The 2R surface split (MIN_SAMPLES_PER_SURFACE) has no separate
- *       check beyond these.
+ *   same_surface                        hand-built bounds: parallel within
+ *       and beyond 0.1, crossing in (u,v), one-sided difference, crossing in
+ *       time                                             [test_oracle_pins]
+ *   leaf surface scan                   hand-built boxes: time split, the
+ *       2.0 threshold, crossing motion, a CoC sequence the cocbounds[0]
+ *       typo decides                                     [test_oracle_pins]
+ *   Filterer::reconstruct control       constructed static scenes with
+ *       closed-form outcomes: fetch2R + coverage both ways, no fetch2R with
+ *       a four-quadrant first surface, emergency pass -> nearest sample,
+ *       3-sample surface merged into the next            [test_oracle_pins]
+ *   output pattern composition          recomposed from the pinned
+ *       primitives, bit for bit                          [test_oracle_pins]
+ *   RefGpu 2R surface split             a 3-point front surface takes the
+ *       next leaf in                                     [test_oracle_pins]
+ *   Each of the last five rules is broken on purpose in a mutant build
+ *   (ORC_MUTANT below) and the pins are checked to fail [test_oracle_mutants].
  * The reference itself cannot be built here (Win32-only headers), so no
  * function is pinned against reference outputs.
  */
 #include "tlr_oracle.h"
+
+// Mutants for the pin tests: built with -DORC_MUTANT=k, the oracle breaks one
+// rule on purpose (tests/test_oracle_mutants.py checks that the pins catch
+// each); the oracle proper is built with ORC_MUTANT 0.
+//   1 sameSurface without its time test      2 ternaryEqual needs equal signs
+//   3 leaf scan without the cocbounds[0] typo 4 leaf scan ignores t changes
+//   5 fetch2R ignores the quadrant mask       6 no tiny-surface merge
+//   7 emergency passes keep the dispersion    8 no nearest-sample fallback
+//   9 output times without the jitter         10 GPU-variant 2R surfaces
+//                                                split at 3 points
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define ORC_M(k) (ORC_MUTANT == (k))
 
 #include <algorithm>
 #include <atomic>
@@ -1277,7 +1305,7 @@ int make_pattern(int out_spp, int override_uvt, const float* uvt, std::vector<Tm
             float x = sobol(3, i + 1);
             float y = lp(i + 1);
             float u = s2[0], v = s2[1];
-            float t = (i + tj) / M;
+            float t = (i + (ORC_M(9) ? 0.0f : tj)) / M;
             float xy0 = x + xo, xy1 = y + yo, uv0 = u + uo, uv1 = v + vo;
             if (xy0 >= 1.f) xy0 -= 1.f;
             if (xy1 >= 1.f) xy1 -= 1.f;
@@ -1517,7 +1545,7 @@ struct RefGpu {
             do {
                 const Leaf& L = c->leaves[keyed[lidx].second];
                 bool end = false;
-                if (np >= 4)
+                if (np >= (ORC_M(10) ? 3 : 4))
                     for (int j = sfirst; j < lidx; j++)
                         if (!same_surface(L.b, c->leaves[keyed[j].second].b, umin, umax)) {
                             end = true;
@@ -2122,6 +2150,12 @@ void orc_fw_sort(int n, float* keys, int32_t* payload) {
 
 int orc_triangle_covers(const float* r0, const float* r1, const float* r2, const float* o) {
     return triangle_covers(r0[0], r0[1], r1[0], r1[1], r2[0], r2[1], o[0], o[1]) ? 1 : 0;
+}
+
+int orc_surface_scan(int n, const float* t, const float* ba, const float* bb, const float* ba1, const float* bb1,
+                     int32_t* surf) {
+    surface_scan(n, [&](int k) { return t[k]; }, ba, bb, ba1, bb1, surf);
+    return n;
 }
 
 int orc_same_surface(const float* n0, const float* n1, const float* o5) {

@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# TLR_ORACLE_LIB selects another build of the same source (the pin tests'
+# mutants, tests/test_oracle_mutants.py)
+LIB_PATH = os.environ.get("TLR_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 _lib = None
 
@@ -118,6 +120,7 @@ def lib():
         L.orc_fw_sort.argtypes = [C.c_int, fp, ip]
         L.orc_triangle_covers.argtypes = [fp, fp, fp, fp]
         L.orc_same_surface.argtypes = [fp, fp, fp]
+        L.orc_surface_scan.argtypes = [C.c_int, fp, fp, fp, fp, fp, ip]
         _lib = L
     return _lib
 
@@ -286,6 +289,26 @@ class Oracle:
         lv = np.zeros(max_leaves, np.int32)
         n = lib().orc_leaves_bruteforce(self.h, o, R, _ptr(lv), max_leaves)
         return lv[:n]
+
+
+def surface_scan(t, ba, bb, ba1, bb1):
+    """Leaf surface scan (ReconstructionTreeBuilder.cpp:222-340) of candidates
+    with times t and screen boxes ba, bb (t=0) and ba1, bb1 (t=1), (n, 2)
+    each: surface id per candidate."""
+    t = np.ascontiguousarray(t, np.float32)
+    n = t.size
+    arrs = [np.ascontiguousarray(a, np.float32).reshape(n, 2) for a in (ba, bb, ba1, bb1)]
+    out = np.zeros(n, np.int32)
+    lib().orc_surface_scan(n, t, *arrs, out)
+    return out
+
+
+def same_surface(b0, b1, o5):
+    """sameSurface (Reconstruction.hpp:457-525) of two 14-float bounds (12
+    plane coefficients: xmin, xmax, ymin, ymax planes as (t, uv, 1)
+    coefficients; then the time span) around output sample o5 = (x,y,u,v,t)."""
+    return bool(lib().orc_same_surface(np.ascontiguousarray(b0, np.float32), np.ascontiguousarray(b1, np.float32),
+                                       np.ascontiguousarray(o5, np.float32)))
 
 
 def output_pattern(out_spp, override_uvt=None):

@@ -173,6 +173,12 @@ void     orc_fw_sort(int n, float* keys, int32_t* payload);
 int      orc_triangle_covers(const float* r0, const float* r1, const float* r2, const float* o);
 /* sameSurface on two 14-float bounds at output (u,v,t). */
 int      orc_same_surface(const float* n0, const float* n1, const float* o5);
+/* Leaf surface scan of n candidates sorted by (t, w), given each one's time
+ * and its screen boxes (x, y pairs): corners at u=-1 / u=+1 at t=0 (ba, bb)
+ * and t=1 (ba1, bb1) (ReconstructionTreeBuilder.cpp:222-340).  surf[k]
+ * receives candidate k's surface id. */
+int orc_surface_scan(int n, const float* t, const float* ba, const float* bb, const float* ba1, const float* bb1,
+                     int32_t* surf);
 
 #ifdef __cplusplus
 }
