@@ -70,7 +70,11 @@ typedef struct tlr_recon_params {
                                   1R and 2R launches (output samples beyond it are
                                   run again with lists sized for them; results do
                                   not change)                                       */
-    int   reserved[2];
+    int   debug_image;         /* also make the reference's debug image (#surfaces,
+                                  #samples within 1R of each pixel's last output
+                                  sample; ReconstructionOur.cpp:312-314), read with
+                                  tlr_copy_debug_image; path 0 or 1, no gate     */
+    int   reserved;
 } tlr_recon_params;
 
 /* TreeGather::Stats (Reconstruction.hpp:134-147) as sums over output samples,
@@ -127,6 +131,12 @@ tlr_status  tlr_set_samples_soa(tlr_ctx* ctx, const tlr_header* h, const tlr_soa
 /* TreeGather ctor (Reconstruction.cpp:43-215): spp rounding, CoC adjust,
  * reprojection, bucketing, leaves and hierarchy, on the device. */
 tlr_status  tlr_build(tlr_ctx* ctx, float aperture_adjust, float focal_adjust);
+/* The same with CameraParams::overrideRefocusDistance
+ * (Reconstruction.cpp:88-95): coc_override (2 floats, C0 and C1 of the
+ * refocused camera, CameraParams::getCocCoeffs) replaces the adjusted
+ * coefficients for the tree bounds and the filter; the reprojection to the
+ * (u,v,t) centre keeps the header's.  NULL: as tlr_build. */
+tlr_status  tlr_build_ex(tlr_ctx* ctx, float aperture_adjust, float focal_adjust, const float* coc_override);
 
 /* reconstructDofMotion (Reconstruction.cpp:221-295): rgba_out receives
  * width*height*4 floats (row-major, the buffer's y order); on_device selects
@@ -220,6 +230,21 @@ tlr_status  tlr_copy_leaves(tlr_ctx* ctx, int32_t* s01, float* bounds14);
 tlr_status  tlr_copy_hierarchy(tlr_ctx* ctx, int32_t* leaf_order, int32_t* children2, float* bounds14);
 /* per output sample colour of the last reconstruct, (y,x,i) order, 4 floats */
 tlr_status  tlr_copy_query_colors(tlr_ctx* ctx, float* out4);
+/* The debug image of the last reconstruct made with debug_image set:
+ * width*height*4 floats, (#surfaces, #samples within 1R, 0, 0) per pixel
+ * (TreeGather::reconstructDofMotion's debugImage, Reconstruction.cpp:276). */
+tlr_status  tlr_copy_debug_image(tlr_ctx* ctx, float* rgba_out);
+/* Gathered sets of n output samples o5 (x,y,u,v,t each, host) at `radius`
+ * over the current build, as the plain method's general kernel gathers them
+ * (collectInputSamples2, ReconstructionOur.cpp:412-511): per output sample j,
+ * leaves[j*leaf_cap ..] its leaf ids front to back (by key, then id),
+ * n_leaves[j] their count; samples[j*sample_cap ..] the tree-layout index of
+ * each gathered sample in gather order, surface_of[..] its surface, and
+ * n_samples[j] the count (-1 for both counts past the kernel's capacity).
+ * Rows are cut at leaf_cap / sample_cap; the counts are not. */
+tlr_status  tlr_copy_query_gather(tlr_ctx* ctx, const float* o5, int n, float radius, int leaf_cap, int32_t* leaves,
+                                  int32_t* n_leaves, int sample_cap, int32_t* samples, int32_t* surface_of,
+                                  int32_t* n_samples);
 /* After a path-2 reconstruction: per output sample, the stage that produced
  * its colour (0: recKernel 1R, 1: failKernel 2R, 2: nearestKernel),
  * ReconstructionCudaKernels.cu:242,476,634. */

@@ -594,6 +594,8 @@ struct orc_ctx {
     bool built = false;
     int spp_pow2 = 1;
     float cc[2] = {0, 0};
+    bool coc_override = false;        // overrideRefocusDistance's coefficients
+    float coc_ov[2] = {0, 0};
     std::vector<RS> repro;            // kept, input order
     float bbmin[2], bbmax[2], scale = 0;
     int bbmin_i[2], gw = 0, gh = 0, max_hits = 0, num_bucketed = 0;
@@ -1691,6 +1693,11 @@ int orc_build(orc_ctx* c, float ap_adj, float foc_adj, int threads) {
         cx = -a;
         cy = a * rcpf(f);
         c->cc[0] = cx; c->cc[1] = cy;
+        // overrideRefocusDistance: Reconstruction.cpp:88-95
+        if (c->coc_override) {
+            c->cc[0] = c->coc_ov[0];
+            c->cc[1] = c->coc_ov[1];
+        }
     }
     const float cc0[2] = {c->C0, c->C1};
 
@@ -1897,6 +1904,19 @@ int orc_output_pattern(int out_spp, int override_uvt, const float* uvt, float* o
 }
 
 int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image, float* qcol, orc_stats* stats) {
+    return orc_reconstruct_debug(c, p, image, qcol, stats, nullptr);
+}
+
+void orc_set_coc_override(orc_ctx* c, const float* cc) {
+    c->coc_override = cc != nullptr;
+    if (cc) {
+        c->coc_ov[0] = cc[0];
+        c->coc_ov[1] = cc[1];
+    }
+}
+
+int orc_reconstruct_debug(orc_ctx* c, const orc_recon_params* p, float* image, float* qcol, orc_stats* stats,
+                          float* debug4) {
     if (!c->built) return 1;
     std::vector<Tmpl> pat;
     const int M = make_pattern(p->out_spp, p->override_uvt, p->uvt, pat);
@@ -1923,6 +1943,14 @@ int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image, float* 
                 o.u = tp.u; o.v = tp.v; o.t = tp.t;
                 Filt::Result r = F.reconstruct(o, p, st[w]);
                 for (int k = 0; k < 4; k++) pc[k] += r.color[k];
+                // debugColor: ReconstructionOur.cpp:312-314, the last output
+                // sample's stays (Reconstruction.cpp:276)
+                if (debug4 && i == M - 1) {
+                    float* d = debug4 + 4 * ((size_t)y * W + x);
+                    d[0] = (float)F.nsurf;
+                    d[1] = (float)r.n1;
+                    d[2] = d[3] = 0.0f;
+                }
                 if (qcol) {
                     size_t q = ((size_t)y * W + x) * M + i;
                     for (int k = 0; k < 4; k++) qcol[4 * q + k] = r.color[k];

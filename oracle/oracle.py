@@ -92,6 +92,9 @@ def lib():
         L.orc_output_pattern.argtypes = [C.c_int, C.c_int, fp, C.c_void_p]
         L.orc_reconstruct.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
                                       C.POINTER(Stats)]
+        L.orc_reconstruct_debug.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
+                                            C.POINTER(Stats), C.c_void_p]
+        L.orc_set_coc_override.argtypes = [P, C.c_void_p]
         L.orc_reconstruct_refgpu.argtypes = [P, C.POINTER(ReconParams), C.c_void_p, C.c_void_p,
                                              C.c_void_p]
         L.orc_refgpu_query.argtypes = [P, fp, C.c_void_p, C.POINTER(C.c_int32)]
@@ -150,7 +153,9 @@ class Oracle:
         except Exception:
             pass
 
-    def build(self, aperture_adjust=1.0, focal_adjust=1.0, threads=0):
+    def build(self, aperture_adjust=1.0, focal_adjust=1.0, threads=0, coc_override=None):
+        cc = None if coc_override is None else np.ascontiguousarray(coc_override, np.float32)
+        lib().orc_set_coc_override(self.h, _ptr(cc))
         rc = lib().orc_build(self.h, aperture_adjust, focal_adjust, threads)
         if rc != 0:
             raise RuntimeError("oracle build failed (%d): %s" % (rc, lib().orc_error(self.h).decode()))
@@ -214,15 +219,19 @@ class Oracle:
         p.row_stride = row_stride
         return p
 
-    def reconstruct(self, out_spp=16, want_queries=False, **kw):
+    def reconstruct(self, out_spp=16, want_queries=False, want_debug=False, **kw):
+        """(image, per-output-sample colours or None, stats[, debug image])."""
         p = self.params(out_spp=out_spp, **kw)
         M = 1 if p.override_uvt else out_spp
         img = np.zeros((self.H, self.W, 4), np.float32)
         q = np.zeros((self.H, self.W, M, 4), np.float32) if want_queries else None
+        dbg = np.zeros((self.H, self.W, 4), np.float32) if want_debug else None
         st = Stats()
-        rc = lib().orc_reconstruct(self.h, C.byref(p), _ptr(img), _ptr(q), C.byref(st))
+        rc = lib().orc_reconstruct_debug(self.h, C.byref(p), _ptr(img), _ptr(q), C.byref(st), _ptr(dbg))
         if rc != 0:
             raise RuntimeError("oracle reconstruct failed (%d)" % rc)
+        if want_debug:
+            return img, q, st.as_dict(), dbg
         return img, q, st.as_dict()
 
     def reconstruct_refgpu(self, out_spp=16, want_queries=False, **kw):

@@ -106,6 +106,14 @@ int orc_output_pattern(int out_spp, int override_uvt, const float* uvt, float* o
  * W*H*out_spp*4 floats in (y, x, i) order. */
 int orc_reconstruct(orc_ctx* c, const orc_recon_params* p, float* image,
                     float* query_colors, orc_stats* stats);
+/* orc_reconstruct plus the debug image (optional, W*H*4): per pixel, the last
+ * output sample's (#surfaces of its last gather, #samples within 1R, 0, 0)
+ * (ReconstructionOur.cpp:312-314, Reconstruction.cpp:276). */
+int orc_reconstruct_debug(orc_ctx* c, const orc_recon_params* p, float* image, float* query_colors,
+                          orc_stats* stats, float* debug4);
+/* overrideRefocusDistance (Reconstruction.cpp:88-95): later builds use cc
+ * (C0, C1) in place of the adjusted coefficients; NULL clears it. */
+void orc_set_coc_override(orc_ctx* c, const float* cc);
 
 /* The reference's GPU-variant algorithm (recKernel, failKernel and
  * nearestKernel of ReconstructionCudaKernels.cu) over the same hierarchy.

@@ -27,9 +27,10 @@ STATUS = {0: "TLR_OK", 1: "TLR_E_INVALID", 2: "TLR_E_IO", 3: "TLR_E_ALL_DISCARDE
 
 EXPORTED = [
     "tlr_create", "tlr_destroy", "tlr_last_error", "tlr_load_sample_file", "tlr_save_sample_file",
-    "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_reconstruct", "tlr_reconstruct_noalloc", "tlr_sync",
+    "tlr_free", "tlr_set_samples", "tlr_set_samples_soa", "tlr_build", "tlr_build_ex", "tlr_reconstruct", "tlr_reconstruct_noalloc", "tlr_sync",
     "tlr_get_build_info", "tlr_copy_reprojected", "tlr_copy_layout", "tlr_copy_initial_leaves",
     "tlr_copy_leaves", "tlr_copy_hierarchy", "tlr_copy_query_colors", "tlr_copy_query_stages",
+    "tlr_copy_debug_image", "tlr_copy_query_gather",
     "tlr_output_pattern", "tlr_calibrate_traffic",
     "tlr_gate_set_params", "tlr_gate_get_params", "tlr_gate_set_threshold", "tlr_gate_forward",
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
@@ -60,7 +61,7 @@ class ReconParams(C.Structure):
     _fields_ = [("out_spp", C.c_int), ("override_uvt", C.c_int), ("uvt", C.c_float * 3),
                 ("lens_gaussian", C.c_int), ("time_gaussian", C.c_int),
                 ("collect_stats", C.c_int), ("gate", C.c_int), ("path", C.c_int),
-                ("list_cap", C.c_int), ("reserved", C.c_int * 2)]
+                ("list_cap", C.c_int), ("debug_image", C.c_int), ("reserved", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -138,6 +139,9 @@ def lib():
         L.tlr_set_samples.argtypes = [V, C.POINTER(Header), V, C.c_int64]
         L.tlr_set_samples_soa.argtypes = [V, C.POINTER(Header), C.POINTER(Soa), C.c_int]
         L.tlr_build.argtypes = [V, C.c_float, C.c_float]
+        L.tlr_build_ex.argtypes = [V, C.c_float, C.c_float, V]
+        L.tlr_copy_debug_image.argtypes = [V, V]
+        L.tlr_copy_query_gather.argtypes = [V, V, C.c_int, C.c_float, C.c_int, V, V, C.c_int, V, V, V]
         L.tlr_reconstruct.argtypes = [V, C.POINTER(ReconParams), V, C.c_int, C.POINTER(Stats)]
         L.tlr_reconstruct_noalloc.argtypes = [V, C.POINTER(ReconParams), V, C.POINTER(Stats)]
         L.tlr_sync.argtypes = [V]
@@ -249,14 +253,53 @@ class UVTSampleBuffer:
         return Header(self.width, self.height, self.spp, self.coc[0], self.coc[1])
 
 
-class CameraParams:
-    """The CameraParams fields the reconstruction reads (CameraParams.hpp:92-107)."""
+def coc_coeffs(projection, window_width, aperture_radius, focus_distance):
+    """CameraParams::getCocCoeffs (CameraParams.hpp:97-118) in FP32 with the
+    framework's operation order: m = projection * getDofMatrix((0.5, 0)) -
+    projection (products summed over k = 0..3 from 0, Math.hpp:1024-1035),
+    coefficients (-m03, m02) * window width."""
+    f = np.float32
+    P = np.asarray(projection, np.float32).reshape(4, 4)
+    A, fd = f(aperture_radius), f(focus_distance)
+    dof = np.eye(4, dtype=np.float32)
+    dof[0, 2] = f(f(f(0.5) * A) / fd)     # uv * apertureRadius / focusDistance, uv = (1,0) / 2
+    dof[1, 2] = f(f(f(0.0) * A) / fd)
+    dof[0, 3] = f(f(0.5) * A)              # uv * apertureRadius
+    dof[1, 3] = f(f(0.0) * A)
+    m = np.zeros((4, 4), np.float32)
+    for i in range(4):
+        for j in range(4):
+            rr = f(0.0)
+            for k in range(4):
+                rr = f(rr + f(P[i, k] * dof[k, j]))
+            m[i, j] = f(rr - P[i, j])
+    ww = f(window_width)
+    return np.array([f(-m[0, 3] * ww), f(m[0, 2] * ww)], np.float32)
 
-    def __init__(self, out_spp=128, override_uvt=None, lens_gaussian=False, time_gaussian=False):
+
+class CameraParams:
+    """The CameraParams fields the reconstruction reads (CameraParams.hpp:92-107).
+    override_refocus_distance (with projection, window_width and
+    aperture_radius, which it needs) refocuses through new CoC coefficients
+    (Reconstruction.cpp:88-95)."""
+
+    def __init__(self, out_spp=128, override_uvt=None, lens_gaussian=False, time_gaussian=False,
+                 override_refocus_distance=None, projection=None, window_width=None, aperture_radius=None):
         self.out_spp = out_spp
         self.override_uvt = override_uvt
         self.lens_gaussian = lens_gaussian
         self.time_gaussian = time_gaussian
+        self.override_refocus_distance = override_refocus_distance
+        self.projection = projection
+        self.window_width = window_width
+        self.aperture_radius = aperture_radius
+
+    def refocus_coc(self):
+        if self.override_refocus_distance is None:
+            return None
+        if self.projection is None or self.window_width is None or self.aperture_radius is None:
+            raise ValueError("overrideRefocusDistance needs projection, window_width and aperture_radius")
+        return coc_coeffs(self.projection, self.window_width, self.aperture_radius, self.override_refocus_distance)
 
 
 class Ctx:
@@ -288,11 +331,44 @@ class Ctx:
         self._chk(lib().tlr_set_samples(self.h, C.byref(sbuf.header()), _p(sbuf.records),
                                         sbuf.records.shape[0]))
 
-    def build(self, aperture_adjust=1.0, focal_adjust=1.0):
-        self._chk(lib().tlr_build(self.h, aperture_adjust, focal_adjust))
+    def build(self, aperture_adjust=1.0, focal_adjust=1.0, coc_override=None):
+        if coc_override is None:
+            self._chk(lib().tlr_build(self.h, aperture_adjust, focal_adjust))
+        else:
+            cc = np.ascontiguousarray(coc_override, np.float32)
+            self._chk(lib().tlr_build_ex(self.h, aperture_adjust, focal_adjust, _p(cc)))
+
+    def debug_image(self):
+        """Debug image of the last reconstruct(debug_image=True)."""
+        out = np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
+        self._chk(lib().tlr_copy_debug_image(self.h, _p(out)))
+        return out
+
+    def query_gather(self, o5, radius, leaf_cap=4096, sample_cap=1 << 15):
+        """The plain method's gather of output samples o5 ((n, 5): x,y,u,v,t)
+        at `radius`: per output sample (leaf ids front to back, gathered
+        sample indices in tree layout, their surfaces); None past capacity."""
+        o5 = np.ascontiguousarray(o5, np.float32).reshape(-1, 5)
+        n = o5.shape[0]
+        lv = np.zeros((n, leaf_cap), np.int32)
+        sm = np.zeros((n, sample_cap), np.int32)
+        sf = np.zeros((n, sample_cap), np.int32)
+        nl = np.zeros(n, np.int32)
+        ns = np.zeros(n, np.int32)
+        self._chk(lib().tlr_copy_query_gather(self.h, _p(o5), n, float(radius), leaf_cap, _p(lv), _p(nl), sample_cap,
+                                              _p(sm), _p(sf), _p(ns)))
+        out = []
+        for j in range(n):
+            if nl[j] < 0:
+                out.append(None)
+            else:
+                out.append((lv[j, :min(nl[j], leaf_cap)], sm[j, :min(ns[j], sample_cap)],
+                            sf[j, :min(ns[j], sample_cap)]))
+        return out
 
     def reconstruct(self, out_spp=16, override_uvt=None, lens_gaussian=False, time_gaussian=False,
-                    collect_stats=False, gate=False, path=0, out=None, list_cap=0, device_only=False):
+                    collect_stats=False, gate=False, path=0, out=None, list_cap=0, device_only=False,
+                    debug_image=False):
         """Reconstructs into `out` (host, H x W x 4 float32; allocated when
         None).  device_only: the image stays in the context's device buffer
         and (None, stats) is returned."""
@@ -308,6 +384,7 @@ class Ctx:
         p.gate = int(gate)
         p.path = int(path)
         p.list_cap = int(list_cap)
+        p.debug_image = int(debug_image)
         if device_only:
             img = None
         else:
@@ -521,14 +598,19 @@ class TreeGather:
         self.params = params
         self.ctx = Ctx(device)
         self.ctx.set_samples(sbuf)
-        self.ctx.build(aperture_adjust, focal_adjust)
+        self.ctx.build(aperture_adjust, focal_adjust, coc_override=params.refocus_coc())
 
-    def reconstruct_dof_motion(self, image=None):
+    def reconstruct_dof_motion(self, image=None, debug_image=None):
+        """Fills image (or returns a new one); debug_image, if given, receives
+        (#surfaces, #samples within 1R, 0, 0) per pixel (Reconstruction.hpp:46)."""
         if image is not None and (image.shape[0] < self.sbuf.height or image.shape[1] < self.sbuf.width):
             raise TlrError(5, "TreeGather::reconstructDofMotion image smaller than sample buffer")
         p = self.params
         img, st = self.ctx.reconstruct(out_spp=p.out_spp, override_uvt=p.override_uvt,
-                                       lens_gaussian=p.lens_gaussian, time_gaussian=p.time_gaussian)
+                                       lens_gaussian=p.lens_gaussian, time_gaussian=p.time_gaussian,
+                                       debug_image=debug_image is not None)
+        if debug_image is not None:
+            debug_image[:self.sbuf.height, :self.sbuf.width] = self.ctx.debug_image()
         if image is not None:
             image[:self.sbuf.height, :self.sbuf.width] = img
             return image

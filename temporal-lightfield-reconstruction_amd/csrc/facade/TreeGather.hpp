@@ -115,7 +115,9 @@ class UVTSampleBuffer {
     std::vector<float> rec_;
 };
 
-// The CameraParams fields the reconstruction reads.
+// The CameraParams fields the reconstruction reads.  projection (row-major,
+// projection[4*row+col]), windowWidth and apertureRadius are needed only with
+// overrideRefocusDistance.
 class CameraParams {
   public:
     LensFilter lensFilter = LENS_BOX;
@@ -123,6 +125,30 @@ class CameraParams {
     Vec3f overrideUVT = Vec3f(FLT_MAX);
     float overrideRefocusDistance = FLT_MAX;
     bool enableCuda = true;
+    float projection[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    int windowWidth = 0;
+    float apertureRadius = 0;
+
+    // getCocCoeffs at focus distance f (CameraParams.hpp:97-118): m =
+    // projection * getDofMatrix((0.5, 0)) - projection, products summed over k
+    // from 0 as the framework's Mat4f does (Math.hpp:1024-1035); coefficients
+    // (-m03, m02) * window width.
+    void cocCoeffsAt(float f, float out[2]) const {
+        float dof[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        dof[0 * 4 + 2] = (0.5f * apertureRadius) / f;
+        dof[1 * 4 + 2] = (0.0f * apertureRadius) / f;
+        dof[0 * 4 + 3] = 0.5f * apertureRadius;
+        dof[1 * 4 + 3] = 0.0f * apertureRadius;
+        float m[16];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                float rr = 0.0f;
+                for (int k = 0; k < 4; k++) rr += projection[4 * i + k] * dof[4 * k + j];
+                m[4 * i + j] = rr - projection[4 * i + j];
+            }
+        out[0] = -m[0 * 4 + 3] * (float)windowWidth;
+        out[1] = m[0 * 4 + 2] * (float)windowWidth;
+    }
 };
 
 class TreeGather {
@@ -132,12 +158,17 @@ class TreeGather {
     TreeGather(const UVTSampleBuffer& sbuf, const CameraParams& params, float apertureAdjust = 1.f,
                float focalDistanceAdjust = 1.f, int device = 0)
         : sbuf_(&sbuf), params_(&params) {
-        if (params.overrideRefocusDistance != FLT_MAX)
-            throw ReconstructionError(TLR_E_INVALID, "overrideRefocusDistance is not supported");
         check(tlr_create(&ctx_, device), "tlr_create");
         const tlr_header& h = sbuf.header();
         check(tlr_set_samples(ctx_, &h, sbuf.records().data(), (int64_t)h.width * h.height * h.spp), "set samples");
-        check(tlr_build(ctx_, apertureAdjust, focalDistanceAdjust), "build");
+        if (params.overrideRefocusDistance != FLT_MAX) {
+            // Reconstruction.cpp:88-95: the refocused camera's coefficients
+            float cc[2];
+            params.cocCoeffsAt(params.overrideRefocusDistance, cc);
+            check(tlr_build_ex(ctx_, apertureAdjust, focalDistanceAdjust, cc), "build");
+        } else {
+            check(tlr_build(ctx_, apertureAdjust, focalDistanceAdjust), "build");
+        }
     }
     ~TreeGather() { tlr_destroy(ctx_); }
     TreeGather(const TreeGather&) = delete;
@@ -148,7 +179,6 @@ class TreeGather {
     void setOutputSpp(int spp) { out_spp_ = spp; }
 
     void reconstructDofMotion(Image& image, Image* debugImage = nullptr) {
-        (void)debugImage;
         if (image.getSize().x < sbuf_->getWidth() || image.getSize().y < sbuf_->getHeight())
             throw ReconstructionError(TLR_E_IMAGE_TOO_SMALL,
                                       "TreeGather::reconstructDofMotion image smaller than < sample buffer");
@@ -159,9 +189,21 @@ class TreeGather {
         p.uvt[0] = c.overrideUVT.x; p.uvt[1] = c.overrideUVT.y; p.uvt[2] = c.overrideUVT.z;
         p.lens_gaussian = c.lensFilter == LENS_GAUSSIAN;
         p.time_gaussian = c.timeFilter == TIME_GAUSSIAN;
+        p.debug_image = debugImage != nullptr;
         const int W = sbuf_->getWidth(), H = sbuf_->getHeight();
         std::vector<float> rgba((size_t)W * H * 4);
         check(tlr_reconstruct(ctx_, &p, rgba.data(), 0, &stats_), "reconstruct");
+        if (debugImage) {
+            // (#surfaces, #samples within 1R) of each pixel's last output
+            // sample (ReconstructionOur.cpp:312-314, Reconstruction.cpp:276)
+            std::vector<float> dbg((size_t)W * H * 4);
+            check(tlr_copy_debug_image(ctx_, dbg.data()), "debug image");
+            for (int y = 0; y < H; y++)
+                for (int x = 0; x < W; x++) {
+                    const float* q = &dbg[4 * ((size_t)y * W + x)];
+                    debugImage->setVec4f(Vec2i(x, y), Vec4f(q[0], q[1], q[2], q[3]));
+                }
+        }
         for (int y = 0; y < H; y++)
             for (int x = 0; x < W; x++) {
                 const float* q = &rgba[4 * ((size_t)y * W + x)];

@@ -121,13 +121,14 @@ struct Gather {
     int nsurf;     // surfaces
     int used;      // sample slots in use (>= total after merges)
     int overflow;
+    int nl;        // leaves found (sorted ids: first kStage in LDS, the rest in m.sid)
 };
 
 // collectInputSamples2 for one output sample at radius R (wave-uniform).
 __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& o, float R,
                          unsigned long long* st) {
     const int lane = threadIdx.x & 63;
-    Gather g{0, 0, 0, 0};
+    Gather g{0, 0, 0, 0, 0};
     int nl = 0;
     // leaves found (traversal order) and sorted, the first kStage in LDS,
     // the rest in the wave's scratch: key, id; sorted id, s0, s1
@@ -374,6 +375,7 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     g.total = total;
     g.nsurf = nsurf;
     g.used = total;
+    g.nl = nl;
     return g;
 }
 
@@ -496,7 +498,7 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
 
     // Filterer::reconstruct: ReconstructionOur.cpp:50-159
     float disp = dispersion_for_spp(Q.spp_pow2);
-    Gather g{0, 0, 0, 0};
+    Gather g{0, 0, 0, 0, 0};
     int overflow = 0;
     int n1 = 0;
     bool f2 = true;
@@ -647,7 +649,16 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
         }
     }
     if (overflow) st[ST_OVERFLOW] += 1;
-    if (lane == 0) Q.qcol[qi] = col;
+    if (lane == 0) {
+        Q.qcol[qi] = col;
+        // debug image values (Reconstruction.hpp:46, ReconstructionOur.cpp:312-314):
+        // surfaces of the last gather, samples within 1R (known to the fast
+        // kernel when it handed the 2R fetch over)
+        if (Q.qdebug) {
+            Q.qdebug[qi].x = (float)g.nsurf;
+            if (!skip1r) Q.qdebug[qi].y = (float)n1;
+        }
+    }
     wave_sync_mem();
 }
 
@@ -1091,8 +1102,8 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                 g_key0 = leaf_key(Q.leaves, lid[first0][tid], o.t);
                 g_key1 = nsurf > 1 ? leaf_key(Q.leaves, lid[end0][tid], o.t) : 0.0f;
             }
-            if (!f2 || (GATE && nsurf > 0)) {
-                if (!f2) s_surf = (unsigned long long)nsurf;
+            if (!fail || prov) {
+                if (!fail) s_surf = (unsigned long long)nsurf;
                 const float oow = rcp(aw);
                 float cr = ar * oow, cg = ag * oow, cb = ab * oow, ca = aw * oow;
                 if (ca == 0) {
@@ -1137,6 +1148,10 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
         // fetch report their (complete) 1R pass, the general kernel the rest
         const bool done = active && !fail;
         const bool one = done || (active && needs2r);
+        if (Q.qdebug && one) {
+            if (done) Q.qdebug[q] = make_float2((float)s_surf, (float)s_kept);
+            else Q.qdebug[q].y = (float)s_kept;
+        }
         unsigned long long v[6] = {one ? s_leaves : 0ull, one ? s_in_leaves : 0ull, one ? s_kept : 0ull,
                                    done ? s_surf : 0ull, done ? 1ull : 0ull, one ? 1ull : 0ull};
 #pragma unroll
@@ -1261,6 +1276,63 @@ __global__ __launch_bounds__(256) void k_gate_label(const float4* __restrict__ q
     }
 }
 
+// debugImage of reconstructDofMotion (Reconstruction.cpp:276): a pixel holds
+// its last output sample's (#surfaces, #samples within 1R, 0, 0)
+// (ReconstructionOur.cpp:312-314; every output sample overwrites it).
+__global__ __launch_bounds__(256) void k_debug_image(const float2* __restrict__ qdebug, int W, int H, int M,
+                                                     float4* __restrict__ out) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= W * H) return;
+    const float2 d = qdebug[(size_t)p * M + (M - 1)];
+    out[p] = make_float4(d.x, d.y, 0.0f, 0.0f);
+}
+
+// The gather of listed output samples (collectInputSamples2 at radius R,
+// ReconstructionOur.cpp:412-511), as the general kernel runs it: for output
+// sample j, its leaves front to back (leaf_cap per row), and its gathered
+// samples (tree layout index, surface) in gather order (sample_cap per row).
+// Counts are -1 when the gather exceeded the kernel's capacities.
+__global__ __launch_bounds__(kBlock) void k_query_gather(QueryParams Q, const float* __restrict__ o5, int n, float R,
+                                                         int leaf_cap, int* __restrict__ leaves,
+                                                         int* __restrict__ n_leaves, int sample_cap,
+                                                         int* __restrict__ samples, int* __restrict__ surface_of,
+                                                         int* __restrict__ n_samples) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int waves = gridDim.x * kWavesPerBlock;
+    const WaveMem m = wave_mem(Q.scratch + (size_t)wave * kWaveScratchWords);
+    unsigned long long st[kNumStats];
+#pragma unroll
+    for (int k = 0; k < kNumStats; k++) st[k] = 0;
+    for (int j = wave; j < n; j += waves) {
+        OSample o;
+        o.x = o5[5 * j]; o.y = o5[5 * j + 1]; o.u = o5[5 * j + 2]; o.v = o5[5 * j + 3]; o.t = o5[5 * j + 4];
+        const Gather g = gather(Q, m, o, R, st);
+        if (g.overflow) {
+            if (lane == 0) n_leaves[j] = n_samples[j] = -1;
+            continue;
+        }
+        const int* Sid = reinterpret_cast<const int*>(wave_lds()) + 2 * kStage;
+        for (int li = lane; li < g.nl && li < leaf_cap; li += 64)
+            leaves[(size_t)j * leaf_cap + li] = li < kStage ? Sid[li] : m.sid[li];
+        for (int si = 0; si < g.nsurf; si++) {
+            const int fs = uni(m.f_start[si]), fn = uni(m.f_size[si]);
+            for (int k = lane; k < fn; k += 64)
+                if (fs + k < sample_cap) {
+                    samples[(size_t)j * sample_cap + fs + k] = m.sidx[fs + k];
+                    surface_of[(size_t)j * sample_cap + fs + k] = si;
+                }
+        }
+        if (lane == 0) {
+            n_leaves[j] = g.nl;
+            n_samples[j] = g.total;
+        }
+        // the next gather rewrites the wave's LDS area
+        __builtin_amdgcn_wave_barrier();
+        wave_sync_mem();
+    }
+}
+
 // Pixel = sum of its output samples in order, normalised by alpha; red if
 // nothing was found (Reconstruction.cpp:665-719).
 __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ qcol, int W, int H, int M,
@@ -1382,6 +1454,23 @@ hipError_t launch_gate_label(const float4* qcol, const unsigned* entries, const 
                              const float* features, const float4* prov, float tau, float* labels, hipStream_t s) {
     if (max_n == 0) return hipSuccess;
     k_gate_label<<<entry_grid(max_n), 256, 0, s>>>(qcol, entries, n_entries, features, prov, tau, labels);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_image(const float2* qdebug, int W, int H, int M, float* out, hipStream_t s) {
+    const int n = W * H;
+    if (n == 0) return hipSuccess;
+    k_debug_image<<<(n + 255) / 256, 256, 0, s>>>(qdebug, W, H, M, reinterpret_cast<float4*>(out));
+    return hipGetLastError();
+}
+
+hipError_t launch_query_gather(const QueryParams& p, const float* o5, int n, float R, int leaf_cap, int* leaves,
+                               int* n_leaves, int sample_cap, int* samples, int* surface_of, int* n_samples,
+                               hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int blocks = p.num_waves / kWavesPerBlock;
+    k_query_gather<<<blocks, kBlock, 0, s>>>(p, o5, n, R, leaf_cap, leaves, n_leaves, sample_cap, samples,
+                                             surface_of, n_samples);
     return hipGetLastError();
 }
 

@@ -75,6 +75,9 @@ struct QueryParams {
     // first image row of this launch: its W x H pixels are rows [y0, y0+H)
     // of the image (0 for a whole image); query ids and qcol are local
     int y0;
+    // optional, with stats: per query (#surfaces of the last gather, #samples
+    // within 1R), the values of the reference's debug image
+    float2* qdebug;
 };
 
 // Lanes the fast kernel is launched with over all queries of Q (not a list):
@@ -138,6 +141,14 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
 // for every hand-off entry, its gate features in p.features.
 hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s);
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
+// Debug image from QueryParams::qdebug: pixel = (qdebug of its last output
+// sample, 0, 0).
+hipError_t launch_debug_image(const float2* qdebug, int W, int H, int M, float* out_rgba, hipStream_t s);
+// The general kernel's gather for n listed output samples (o5: x,y,u,v,t
+// each) at radius R; p.num_waves waves with p.scratch; see k_query_gather.
+hipError_t launch_query_gather(const QueryParams& p, const float* o5, int n, float R, int leaf_cap, int* leaves,
+                               int* n_leaves, int sample_cap, int* samples, int* surface_of, int* n_samples,
+                               hipStream_t s);
 // Counter calibration: kind 0 streams n float4 from src to dst; kinds 1 / 2
 // gather n records of 2 / 4 float4 from src in the order (i * odd) mod n (n
 // a power of two) and write one float4 per record to dst.

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <numeric>
 #include <random>
@@ -111,6 +112,9 @@ struct tlr_ctx {
     float* labels = nullptr;
     float4* prov = nullptr;         // gate: provisional colour per hand-off entry
     unsigned* d_accept_count = nullptr;
+    float2* qdebug = nullptr;       // debug image values per output sample
+    float* d_debug = nullptr;       // debug image (W*H*4) of the last reconstruct with debug_image
+    bool has_debug = false;
     // reference GPU-variant method (path 2)
     DeviceArena rarena;             // lists, counters, stage codes, first 2R lists
     DeviceArena rwide;              // lists of the overflow launches
@@ -327,7 +331,19 @@ static uint32_t read_u32(tlr_ctx* c, const uint32_t* d) {
     return v;
 }
 
+static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adjust, const float* coc_override);
+
 tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
+    return build_impl(c, aperture_adjust, focal_adjust, nullptr);
+}
+
+tlr_status tlr_build_ex(tlr_ctx* c, float aperture_adjust, float focal_adjust, const float* coc_override) {
+    if (coc_override && !(std::isfinite(coc_override[0]) && std::isfinite(coc_override[1])))
+        return c ? c->fail(TLR_E_INVALID, "non-finite CoC coefficients") : TLR_E_INVALID;
+    return build_impl(c, aperture_adjust, focal_adjust, coc_override);
+}
+
+static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adjust, const float* coc_override) {
     if (!c) return TLR_E_INVALID;
     if (!c->has_samples) return c->fail(TLR_E_STATE, "tlr_build before tlr_set_samples");
     HIPCHK(c, hipSetDevice(c->device));
@@ -353,6 +369,13 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
         f *= focal_adjust;
         c->cc0 = -a;
         c->cc1 = a * rcp(f);
+        // overrideRefocusDistance (Reconstruction.cpp:88-95): the new
+        // coefficients replace the adjusted ones; the reprojection to the
+        // (u,v,t) centre keeps the header's (Reconstruction.cpp:519)
+        if (coc_override) {
+            c->cc0 = coc_override[0];
+            c->cc1 = coc_override[1];
+        }
     }
 
     // ---- device memory for this build ----
@@ -522,6 +545,7 @@ tlr_status tlr_build(tlr_ctx* c, float aperture_adjust, float focal_adjust) {
 
 // Allocates the per-reconstruction buffers and fills the query parameters.
 static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool gate_buffers, QueryParams& Q) {
+    const bool debug = p->debug_image != 0;
     if (!c->built) return c->fail(TLR_E_STATE, "reconstruct before tlr_build");
     if (p->out_spp <= 0 && !p->override_uvt) return c->fail(TLR_E_INVALID, "out_spp must be positive");
     HIPCHK(c, hipSetDevice(c->device));
@@ -537,6 +561,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
                     (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
     if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 2) + nq * sizeof(float4) + 5 * 256;
+    if (debug) qbytes += nq * sizeof(float2) + (size_t)W * H * 16 + 2 * 256;
     HIPCHK(c, c->qarena.reserve(qbytes));
     c->qarena.reset();
     c->d_pattern = c->qarena.alloc<float>(pat.size());
@@ -546,6 +571,13 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
     c->features = c->logits = c->labels = nullptr;
     c->prov = nullptr;
+    c->qdebug = nullptr;
+    c->d_debug = nullptr;
+    c->has_debug = false;
+    if (debug) {
+        c->qdebug = c->qarena.alloc<float2>(nq);
+        c->d_debug = c->qarena.alloc<float>((size_t)W * H * 4);
+    }
     if (gate_buffers) {
         c->features = c->qarena.alloc<float>(nq * kGateFeatures);
         c->logits = c->qarena.alloc<float>(nq);
@@ -584,6 +616,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     Q.num_waves = waves;
     Q.fail_list = c->fail_list;
     Q.fail_count = c->d_fail_count;
+    Q.qdebug = c->qdebug;
     // tile order pads every 8x8 tile to whole waves; the fast kernel's lane
     // index is 32-bit, so very narrow images fall back to query order
     Q.tile_order = 1;
@@ -739,6 +772,8 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     if (p->gate && !c->has_gate) return c->fail(TLR_E_STATE, "gate requested but no gate parameters set");
     if (p->path < 0 || p->path > 2) return c->fail(TLR_E_INVALID, "unknown path %d", p->path);
     if (p->path == 2 && p->gate) return c->fail(TLR_E_INVALID, "the gate applies to the plain method only");
+    if (p->debug_image && (p->gate || p->path == 2))
+        return c->fail(TLR_E_INVALID, "the debug image comes from the plain method (path 0 or 1, no gate)");
     c->ref_stage = nullptr;
     QueryParams Q;
     tlr_status st = prepare_queries(c, p, p->gate != 0, Q);
@@ -768,9 +803,13 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     } else if (p->path == 2) {
         st = run_refgpu(c, Q, p->list_cap, &n2r);
     } else {
-        st = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0, p->path);
+        st = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0 || p->debug_image != 0, p->path);
     }
     if (st != TLR_OK) return st;
+    if (p->debug_image) {
+        HIPCHK(c, launch_debug_image(c->qdebug, W, H, M, c->d_debug, s));
+        c->has_debug = true;
+    }
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     float* dst = out_on_device ? out : c->d_img;
     HIPCHK(c, launch_combine(c->qcol, W, H, M, dst, s));
@@ -1043,6 +1082,68 @@ tlr_status tlr_copy_leaves(tlr_ctx* c, int32_t* s01, float* b14) {
 tlr_status tlr_copy_query_colors(tlr_ctx* c, float* out4) {
     if (!c || !c->qcol || !out4) return TLR_E_STATE;
     HIPCHK(c, hipMemcpy(out4, c->qcol, c->qcol_n * sizeof(float4), hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_debug_image(tlr_ctx* c, float* out4) {
+    if (!c || !out4) return TLR_E_INVALID;
+    if (!c->has_debug) return c->fail(TLR_E_STATE, "the last reconstruction did not make a debug image");
+    HIPCHK(c, hipMemcpy(out4, c->d_debug, (size_t)c->h.width * c->h.height * 16, hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+tlr_status tlr_copy_query_gather(tlr_ctx* c, const float* o5, int n, float radius, int leaf_cap, int32_t* leaves,
+                                 int32_t* n_leaves, int sample_cap, int32_t* samples, int32_t* surface_of,
+                                 int32_t* n_samples) {
+    if (!c || (!o5 && n > 0) || n < 0 || leaf_cap < 0 || sample_cap < 0 || !n_leaves || !n_samples)
+        return TLR_E_INVALID;
+    if ((leaf_cap > 0 && !leaves) || (sample_cap > 0 && (!samples || !surface_of))) return TLR_E_INVALID;
+    if (!(radius > 0.0f)) return c->fail(TLR_E_INVALID, "gather radius must be positive");
+    if (!c->built) return c->fail(TLR_E_STATE, "gather before tlr_build");
+    if (n == 0) return TLR_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int waves = std::min(n, 256);
+    const int wv = (waves + 3) / 4 * 4;
+    char* buf = nullptr;
+    const size_t b_o5 = (size_t)n * 5 * 4, b_l = (size_t)n * leaf_cap * 4, b_s = (size_t)n * sample_cap * 4,
+                 b_c = (size_t)n * 4, b_w = (size_t)wv * kWaveScratchWords * 4;
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = up(b_o5) + up(b_l) + 2 * up(b_s) + 2 * up(b_c) + up(b_w);
+    HIPCHK(c, hipMalloc(&buf, total));
+    char* q = buf;
+    auto take = [&](size_t b) { char* r = q; q += up(b); return r; };
+    float* d_o5 = reinterpret_cast<float*>(take(b_o5));
+    int* d_l = reinterpret_cast<int*>(take(b_l));
+    int* d_s = reinterpret_cast<int*>(take(b_s));
+    int* d_f = reinterpret_cast<int*>(take(b_s));
+    int* d_nl = reinterpret_cast<int*>(take(b_c));
+    int* d_ns = reinterpret_cast<int*>(take(b_c));
+    float* d_w = reinterpret_cast<float*>(take(b_w));
+    QueryParams Q;
+    memset(&Q, 0, sizeof(Q));
+    Q.W = c->h.width; Q.H = c->h.height; Q.M = 1;
+    Q.spp_pow2 = c->spp_pow2;
+    Q.cc0 = c->cc0; Q.cc1 = c->cc1;
+    Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
+    Q.root = c->nv > 1 ? c->nv / 2 - 1 : 0;
+    Q.trav = c->trav;
+    Q.leaves = c->leaves;
+    Q.leaf_b = c->leaf_b;
+    Q.pts = c->pts;
+    Q.scratch = d_w;
+    Q.num_waves = wv;
+    hipError_t e = hipMemcpyAsync(d_o5, o5, b_o5, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_query_gather(Q, d_o5, n, radius, leaf_cap, d_l, d_nl, sample_cap, d_s, d_f, d_ns, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipMemcpy(n_leaves, d_nl, b_c, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(n_samples, d_ns, b_c, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && leaf_cap) e = hipMemcpy(leaves, d_l, b_l, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sample_cap) e = hipMemcpy(samples, d_s, b_s, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sample_cap) e = hipMemcpy(surface_of, d_f, b_s, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return c->fail(TLR_E_HIP, "query gather: %s", hipGetErrorString(e));
     return TLR_OK;
 }
 
