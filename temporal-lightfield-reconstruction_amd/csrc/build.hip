@@ -411,20 +411,66 @@ __device__ void fwsort(const CandView& v, int start, int end) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_leaf_sort(const int2* __restrict__ offcnt, int nil, uint32_t* cand,
-                                                      float* ct, float* cw) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= nil) return;
-    const int2 oc = offcnt[k];
-    CandView v{cand + oc.x, ct + oc.x, cw + oc.x};
-    fwsort(v, 0, oc.y);
+// One wave of 64 initial leaves (one thread each).  The wave copies its
+// leaves into LDS one leaf at a time (each copy a contiguous, coalesced
+// range), every thread runs the sort on its own leaf there, and the sorted
+// input indices go back the same way; the sorted (t, w) copies are not needed
+// afterwards.  A wave whose leaves hold more than kSortLds candidates sorts in
+// global memory instead.  Either way the permutation is FW::sort's.
+constexpr int kSortBlock = 64;
+constexpr int kSortLds = 2304;
+
+__global__ __launch_bounds__(kSortBlock) void k_leaf_sort(const int2* __restrict__ offcnt, int nil, uint32_t* cand,
+                                                          float* ct, float* cw) {
+    __shared__ uint32_t s_idx[kSortLds];
+    __shared__ float s_t[kSortLds];
+    __shared__ float s_w[kSortLds];
+    const int lane = threadIdx.x;
+    const int k = blockIdx.x * kSortBlock + lane;
+    const int2 oc = k < nil ? offcnt[k] : make_int2(0, 0);
+    // exclusive prefix of the leaf sizes across the wave
+    int incl = oc.y;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int base = incl - oc.y;
+    const int total = __shfl(incl, 63, 64);
+    if (total > kSortLds) {
+        if (k < nil) fwsort(CandView{cand + oc.x, ct + oc.x, cw + oc.x}, 0, oc.y);
+        return;
+    }
+    for (int j = 0; j < kSortBlock; j++) {
+        const int off = __shfl(oc.x, j, 64), n = __shfl(oc.y, j, 64), b = __shfl(base, j, 64);
+        for (int e = lane; e < n; e += 64) {
+            s_idx[b + e] = cand[off + e];
+            s_t[b + e] = ct[off + e];
+            s_w[b + e] = cw[off + e];
+        }
+    }
+    __syncthreads();
+    if (k < nil) fwsort(CandView{s_idx + base, s_t + base, s_w + base}, 0, oc.y);
+    __syncthreads();
+    for (int j = 0; j < kSortBlock; j++) {
+        const int off = __shfl(oc.x, j, 64), n = __shfl(oc.y, j, 64), b = __shfl(base, j, 64);
+        for (int e = lane; e < n; e += 64) cand[off + e] = s_idx[b + e];
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Per-sample tree data in layout order: points, bounds and t=0/t=1 boxes
+// Per-sample tree data in layout order: points and t=0/t=1 boxes
 // (ReconstructionTreeBuilder.cpp:170-213).  Boxes of samples with an empty
-// time span are zero.
+// time span are zero.  A sample's bounds: TimeLensBounds from xywt0 =
+// (xy*w, w) - t*mv and mv (:174-175).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void sample_bounds(const float4 a, const float4 m, float W, float H, float cc0, float cc1,
+                                              Bounds& b) {
+    const float x = a.x, y = a.y, w = a.z, t = a.w;
+    const float px = x * w - t * m.x, py = y * w - t * m.y, pw = w - t * m.z;
+    tux_bounds(px, py, pw, m.x, m.y, m.z, W, H, cc0, cc1, b);
+}
+
 __global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ cand, int nb, InSoA in,
                                                    const float* __restrict__ rx, const float* __restrict__ ry,
                                                    const float* __restrict__ rw, const float* __restrict__ rt,
@@ -435,13 +481,13 @@ __global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ 
     const uint32_t i = cand[k];
     const float x = rx[i], y = ry[i], w = rw[i], t = rt[i];
     const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
-    pts.pab[2 * (size_t)k] = make_float4(x, y, w, t);
-    pts.pab[2 * (size_t)k + 1] = make_float4(mvx, mvy, mvw, 0.0f);
+    const float4 pa = make_float4(x, y, w, t), pm = make_float4(mvx, mvy, mvw, 0.0f);
+    pts.pab[2 * (size_t)k] = pa;
+    pts.pab[2 * (size_t)k + 1] = pm;
     pts.pc[k] = make_float4(in.r[i], in.g[i], in.b[i], 1.0f);
 
     Bounds b;
-    const float px = x * w - t * mvx, py = y * w - t * mvy, pw = w - t * mvw;
-    tux_bounds(px, py, pw, mvx, mvy, mvw, W, H, cc0, cc1, b);
+    sample_bounds(pa, pm, W, H, cc0, cc1, b);
     float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
     if (bounds_valid(b)) {
         const float ta = b.ts0, tb = b.ts1;
@@ -536,23 +582,59 @@ __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict_
     }
 }
 
-// One thread per leaf start: leaf id = scanned flag, bounds folded in sample
-// order with node.tlb = TimeLensBounds(bounds[k], node.tlb).
-__global__ __launch_bounds__(kBlock) void k_leaf_emit(const uint32_t* __restrict__ start_flag,
-                                                      const uint32_t* __restrict__ leaf_pos, int nb,
-                                                      const Bounds* __restrict__ bounds, Points pts,
-                                                      LeafRec* __restrict__ leaves, Bounds* __restrict__ leaf_b,
-                                                      uint32_t* __restrict__ valid_flag) {
+// Start sample of every leaf (leaf id = the scanned start flag), and nb
+// after the last.
+__global__ __launch_bounds__(kBlock) void k_leaf_starts(const uint32_t* __restrict__ start_flag,
+                                                        const uint32_t* __restrict__ leaf_pos, int nb, int L,
+                                                        int* __restrict__ leaf_start) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= nb || !start_flag[k]) return;
-    const uint32_t id = leaf_pos[k];
-    int e = k + 1;
-    while (e < nb && !start_flag[e]) e++;
+    if (k < nb && start_flag[k]) leaf_start[leaf_pos[k]] = k;
+    if (k == 0) leaf_start[L] = nb;
+}
+
+// One thread per leaf: bounds folded in sample order with node.tlb =
+// TimeLensBounds(bounds[k], node.tlb) (ReconstructionTreeBuilder.cpp:
+// 345-378).  The 64 leaves of a block cover one contiguous sample range; the
+// block copies its samples' bounds into LDS (coalesced) and each thread folds
+// its leaf there.  A block whose range exceeds kEmitLds samples folds from
+// global memory.
+constexpr int kEmitBlock = 64;
+constexpr int kEmitLds = 512;
+
+__global__ __launch_bounds__(kEmitBlock) void k_leaf_emit(const int* __restrict__ leaf_start, int L,
+                                                          const Bounds* __restrict__ bounds, Points pts,
+                                                          LeafRec* __restrict__ leaves, Bounds* __restrict__ leaf_b,
+                                                          uint32_t* __restrict__ valid_flag) {
+    __shared__ float4 sb[kEmitLds * 4];
+    const int lane = threadIdx.x;
+    const int l0 = blockIdx.x * kEmitBlock;
+    const int id = l0 + lane;
+    const int lend = min(l0 + kEmitBlock, L);
+    const int r0 = leaf_start[l0], r1 = leaf_start[lend];
+    const bool staged = r1 - r0 <= kEmitLds;
+    if (staged) {
+        const float4* src = reinterpret_cast<const float4*>(bounds + r0);
+        for (int e = lane; e < 4 * (r1 - r0); e += kEmitBlock) sb[e] = src[e];
+    }
+    __syncthreads();
+    if (id >= L) return;
+    const int k = leaf_start[id], e = leaf_start[id + 1];
     Bounds acc;
     bounds_zero(acc);
     for (int j = k; j < e; j++) {
-        Bounds nb2;
-        bounds_merge(bounds[j], acc, nb2);
+        Bounds bj, nb2;
+        if (staged) {
+            const float4* q = sb + 4 * (j - r0);
+            float* d = reinterpret_cast<float*>(&bj);
+            const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+            d[0] = q0.x; d[1] = q0.y; d[2] = q0.z; d[3] = q0.w;
+            d[4] = q1.x; d[5] = q1.y; d[6] = q1.z; d[7] = q1.w;
+            d[8] = q2.x; d[9] = q2.y; d[10] = q2.z; d[11] = q2.w;
+            d[12] = q3.x; d[13] = q3.y; d[14] = q3.z; d[15] = q3.w;
+        } else {
+            bj = bounds[j];
+        }
+        bounds_merge(bj, acc, nb2);
         acc = nb2;
     }
     LeafRec r;
@@ -1014,7 +1096,7 @@ void launch_place(const uint32_t* skeys, const uint32_t* svals, int nb, int gw, 
 }
 
 void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, float* cw, hipStream_t s) {
-    if (nil > 0) k_leaf_sort<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, cand, ct, cw);
+    if (nil > 0) k_leaf_sort<<<(nil + kSortBlock - 1) / kSortBlock, kSortBlock, 0, s>>>(offcnt, nil, cand, ct, cw);
 }
 
 void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
@@ -1028,9 +1110,13 @@ void launch_surface_scan(const int2* offcnt, int nil, const float4* pab, const f
     if (nil > 0) k_surface_scan<<<TLR_GRID(nil), 0, s>>>(offcnt, nil, pab, boxes, flags);
 }
 
-void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
-                      const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s) {
-    if (nb > 0) k_leaf_emit<<<TLR_GRID(nb), 0, s>>>(flags, leaf_pos, nb, bounds, pts, leaves, leaf_b, valid_flag);
+void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, int L, int* leaf_start,
+                      const Bounds* bounds, const Points& pts, LeafRec* leaves, Bounds* leaf_b,
+                      uint32_t* valid_flag, hipStream_t s) {
+    if (nb <= 0 || L <= 0) return;
+    k_leaf_starts<<<TLR_GRID(nb), 0, s>>>(flags, leaf_pos, nb, L, leaf_start);
+    k_leaf_emit<<<(L + kEmitBlock - 1) / kEmitBlock, kEmitBlock, 0, s>>>(leaf_start, L, bounds, pts, leaves, leaf_b,
+                                                                         valid_flag);
 }
 
 hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,

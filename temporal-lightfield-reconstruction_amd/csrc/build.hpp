@@ -26,8 +26,11 @@ void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* r
                    Bounds* bounds, float4* boxes, hipStream_t s);
 void launch_surface_scan(const int2* offcnt, int nil, const float4* pab, const float4* boxes, uint32_t* flags,
                          hipStream_t s);
-void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, const Bounds* bounds,
-                      const Points& pts, LeafRec* leaves, Bounds* leaf_b, uint32_t* valid_flag, hipStream_t s);
+// Leaves from the surface-scan start flags (L leaves; leaf_start: L+1 ints of
+// scratch): records, and bounds folded in sample order.
+void launch_leaf_emit(const uint32_t* flags, const uint32_t* leaf_pos, int nb, int L, int* leaf_start,
+                      const Bounds* bounds, const Points& pts, LeafRec* leaves, Bounds* leaf_b,
+                      uint32_t* valid_flag, hipStream_t s);
 // Median-split tree over the nv valid leaves: ids[k] = leaf id of slot k,
 // internal nodes g in [0, nv-1) with children / parent / range / split owner.
 hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, int L, const Bounds* leaf_b,

@@ -389,6 +389,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     words += 12 * (size_t)N + 8 * kPointPad;  // points: pab (2 x float4, padded), pc
     words += 16 * (size_t)N;          // per-sample bounds
     words += 8 * (size_t)N;           // boxes
+    words += (size_t)N + 1;           // leaf starts
     words += 2 * (size_t)N;           // start flags, leaf pos
     words += 8 * (size_t)N;           // leaf records
     words += 16 * (size_t)N;          // leaf bounds
@@ -501,8 +502,9 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     c->leaf_b = A.alloc<Bounds>(L1);
     uint32_t* vflag = A.alloc<uint32_t>(L1 + 1);
     uint32_t* vpos = A.alloc<uint32_t>(L1 + 1);
+    int* lstart = A.alloc<int>((size_t)L1 + 1);
     if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (leaves)");
-    launch_leaf_emit(sflag, lpos, nb, sb, P, c->leaves, c->leaf_b, vflag, s);
+    launch_leaf_emit(sflag, lpos, nb, L, lstart, sb, P, c->leaves, c->leaf_b, vflag, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, exclusive_scan_u32(vflag, vpos, vpos + L, (size_t)L, stmp, s));
     const int nv = L > 0 ? (int)read_u32(c, vpos + L) : 0;
