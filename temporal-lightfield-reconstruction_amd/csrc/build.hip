@@ -61,10 +61,14 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 // ---------------------------------------------------------------------------
 constexpr int kReduceBlocks = 2048;
 
+// Besides the per-field arrays, every kept sample gets one packed 48-byte
+// record (x, y, w, t | mv_x, mv_y, mv_w, r | g, b, 0, 0) that the tree-order
+// gathers (k_place, k_points) read in one or two cache lines.
 __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, int H, float C0, float C1,
                                                       float* __restrict__ rx, float* __restrict__ ry,
                                                       float* __restrict__ rw, float* __restrict__ rt,
-                                                      int* __restrict__ hits, GridInfo* gi) {
+                                                      float4* __restrict__ packed, int* __restrict__ hits,
+                                                      GridInfo* gi) {
     int kmin_x = 0x7FFFFFFF, kmin_y = 0x7FFFFFFF;
     int kmax_x = (int)0x80000000, kmax_y = (int)0x80000000;
     unsigned nok = 0, nbad = 0;
@@ -94,6 +98,9 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
             if (isfinite(ox) && isfinite(oy)) {
                 ok = true;
                 rx[i] = ox; ry[i] = oy; rw[i] = Wt; rt[i] = nt;
+                packed[3 * (size_t)i] = make_float4(ox, oy, Wt, nt);
+                packed[3 * (size_t)i + 1] = make_float4(mvx, mvy, mvw, in.r[i]);
+                packed[3 * (size_t)i + 2] = make_float4(in.g[i], in.b[i], 0.0f, 0.0f);
                 const int sx = floor_i(ox), sy = floor_i(oy);
                 if (sx >= 0 && sy >= 0 && sx < W && sy < H) atomicAdd(&hits[sy * W + sx], 1);
                 kmin_x = min(kmin_x, order_key(ox));
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_place(const uint32_t* __restrict__ s
                                                   const int4* __restrict__ owner,
                                                   const uint32_t* __restrict__ sat,
                                                   const uint32_t* __restrict__ cell_start,
-                                                  const float* __restrict__ rt, const float* __restrict__ rw,
+                                                  const float4* __restrict__ packed,
                                                   uint32_t* __restrict__ cand, float* __restrict__ ct,
                                                   float* __restrict__ cw) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
@@ -333,9 +340,10 @@ __global__ __launch_bounds__(kBlock) void k_place(const uint32_t* __restrict__ s
     pos += rect_count(sat, gw, o.x, o.y, o.z, cy);   // full rows above
     pos += rect_count(sat, gw, o.x, cy, cx, cy + 1);  // cells to the left
     pos += (uint32_t)s - cell_start[key];
+    const float4 a = packed[3 * (size_t)v];   // x, y, w, t
     cand[pos] = v;
-    ct[pos] = rt[v];
-    cw[pos] = rw[v];
+    ct[pos] = a.w;
+    cw[pos] = a.z;
 }
 
 // ---------------------------------------------------------------------------
@@ -471,20 +479,20 @@ __device__ __forceinline__ void sample_bounds(const float4 a, const float4 m, fl
     tux_bounds(px, py, pw, m.x, m.y, m.z, W, H, cc0, cc1, b);
 }
 
-__global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ cand, int nb, InSoA in,
-                                                   const float* __restrict__ rx, const float* __restrict__ ry,
-                                                   const float* __restrict__ rw, const float* __restrict__ rt,
-                                                   float W, float H, float cc0, float cc1, Points pts,
-                                                   Bounds* __restrict__ bounds, float4* __restrict__ boxes) {
+__global__ __launch_bounds__(kBlock) void k_points(const uint32_t* __restrict__ cand, int nb,
+                                                   const float4* __restrict__ packed, float W, float H, float cc0,
+                                                   float cc1, Points pts, Bounds* __restrict__ bounds,
+                                                   float4* __restrict__ boxes) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= nb) return;
     const uint32_t i = cand[k];
-    const float x = rx[i], y = ry[i], w = rw[i], t = rt[i];
-    const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
+    const float4 r0 = packed[3 * (size_t)i], r1 = packed[3 * (size_t)i + 1], r2 = packed[3 * (size_t)i + 2];
+    const float x = r0.x, y = r0.y, w = r0.z, t = r0.w;
+    const float mvx = r1.x, mvy = r1.y, mvw = r1.z;
     const float4 pa = make_float4(x, y, w, t), pm = make_float4(mvx, mvy, mvw, 0.0f);
     pts.pab[2 * (size_t)k] = pa;
     pts.pab[2 * (size_t)k + 1] = pm;
-    pts.pc[k] = make_float4(in.r[i], in.g[i], in.b[i], 1.0f);
+    pts.pc[k] = make_float4(r1.w, r2.x, r2.y, 1.0f);
 
     Bounds b;
     sample_bounds(pa, pm, W, H, cc0, cc1, b);
@@ -1059,9 +1067,9 @@ __global__ __launch_bounds__(kBlock) void k_tree_trav(const uint32_t* __restrict
 #define TLR_GRID(n) dim3((unsigned)(((n) + kBlock - 1) / kBlock)), dim3(kBlock)
 
 void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, float* rx, float* ry, float* rw,
-                      float* rt, int* hits, GridInfo* gi, hipStream_t s) {
+                      float* rt, float4* packed, int* hits, GridInfo* gi, hipStream_t s) {
     const unsigned rb = (unsigned)std::min<long long>(kReduceBlocks, ((long long)n + kBlock - 1) / kBlock);
-    if (n > 0) k_reproject<<<rb, kBlock, 0, s>>>(in, n, W, H, C0, C1, rx, ry, rw, rt, hits, gi);
+    if (n > 0) k_reproject<<<rb, kBlock, 0, s>>>(in, n, W, H, C0, C1, rx, ry, rw, rt, packed, hits, gi);
     const int nh = W * H;
     unsigned blocks = (unsigned)std::min(4096, (nh + kBlock - 1) / kBlock);
     if (blocks == 0) blocks = 1;
@@ -1090,19 +1098,18 @@ void launch_emit_initial(const int4* owner, const uint32_t* own_flag, const int*
 }
 
 void launch_place(const uint32_t* skeys, const uint32_t* svals, int nb, int gw, const int4* owner,
-                  const uint32_t* sat, const uint32_t* cell_start, const float* rt, const float* rw, uint32_t* cand,
-                  float* ct, float* cw, hipStream_t s) {
-    if (nb > 0) k_place<<<TLR_GRID(nb), 0, s>>>(skeys, svals, nb, gw, owner, sat, cell_start, rt, rw, cand, ct, cw);
+                  const uint32_t* sat, const uint32_t* cell_start, const float4* packed, uint32_t* cand, float* ct,
+                  float* cw, hipStream_t s) {
+    if (nb > 0) k_place<<<TLR_GRID(nb), 0, s>>>(skeys, svals, nb, gw, owner, sat, cell_start, packed, cand, ct, cw);
 }
 
 void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, float* cw, hipStream_t s) {
     if (nil > 0) k_leaf_sort<<<(nil + kSortBlock - 1) / kSortBlock, kSortBlock, 0, s>>>(offcnt, nil, cand, ct, cw);
 }
 
-void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
-                   const float* rw, const float* rt, float W, float H, float cc0, float cc1, const Points& pts,
-                   Bounds* bounds, float4* boxes, hipStream_t s) {
-    if (nb > 0) k_points<<<TLR_GRID(nb), 0, s>>>(cand, nb, in, rx, ry, rw, rt, W, H, cc0, cc1, pts, bounds, boxes);
+void launch_points(const uint32_t* cand, int nb, const float4* packed, float W, float H, float cc0, float cc1,
+                   const Points& pts, Bounds* bounds, float4* boxes, hipStream_t s) {
+    if (nb > 0) k_points<<<TLR_GRID(nb), 0, s>>>(cand, nb, packed, W, H, cc0, cc1, pts, bounds, boxes);
 }
 
 void launch_surface_scan(const int2* offcnt, int nil, const float4* pab, const float4* boxes, uint32_t* flags,

@@ -9,8 +9,11 @@
 
 namespace tlr {
 
+// Reprojection to the (u,v,t) centre: per-field arrays rx..rt (w = 0 marks a
+// discarded sample) and, for kept samples, packed 48-byte records
+// (x, y, w, t | mv_x, mv_y, mv_w, r | g, b, 0, 0) for the tree-order gathers.
 void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, float* rx, float* ry, float* rw,
-                      float* rt, int* hits, GridInfo* gi, hipStream_t s);
+                      float* rt, float4* packed, int* hits, GridInfo* gi, hipStream_t s);
 void launch_cell_keys(const float* rx, const float* ry, const float* rw, int n, GridInfo* gi, uint32_t* keys,
                       uint32_t* vals, uint32_t* counts, hipStream_t s);
 void launch_sat(const uint32_t* counts, int gw, int gh, uint32_t* sat, hipStream_t s);
@@ -18,12 +21,11 @@ void launch_kd(const uint32_t* sat, int gw, int gh, int4* owner, uint32_t* own_f
 void launch_emit_initial(const int4* owner, const uint32_t* own_flag, const int* own_y1, const uint32_t* idx,
                          const uint32_t* sat, int gw, int gh, int4* rect, int2* offcnt, hipStream_t s);
 void launch_place(const uint32_t* skeys, const uint32_t* svals, int nb, int gw, const int4* owner,
-                  const uint32_t* sat, const uint32_t* cell_start, const float* rt, const float* rw, uint32_t* cand,
-                  float* ct, float* cw, hipStream_t s);
+                  const uint32_t* sat, const uint32_t* cell_start, const float4* packed, uint32_t* cand, float* ct,
+                  float* cw, hipStream_t s);
 void launch_leaf_sort(const int2* offcnt, int nil, uint32_t* cand, float* ct, float* cw, hipStream_t s);
-void launch_points(const uint32_t* cand, int nb, const InSoA& in, const float* rx, const float* ry,
-                   const float* rw, const float* rt, float W, float H, float cc0, float cc1, const Points& pts,
-                   Bounds* bounds, float4* boxes, hipStream_t s);
+void launch_points(const uint32_t* cand, int nb, const float4* packed, float W, float H, float cc0, float cc1,
+                   const Points& pts, Bounds* bounds, float4* boxes, hipStream_t s);
 void launch_surface_scan(const int2* offcnt, int nil, const float4* pab, const float4* boxes, uint32_t* flags,
                          hipStream_t s);
 // Leaves from the surface-scan start flags (L leaves; leaf_start: L+1 ints of

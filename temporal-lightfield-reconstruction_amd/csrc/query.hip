@@ -479,6 +479,9 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
 
 // One output sample of the hand-off list, the whole algorithm
 // (Filterer::reconstruct, ReconstructionOur.cpp:50-358), wave-uniform.
+// DEBUG also writes the debug-image values (a separate instantiation: the
+// extra live values cost the plain one registers).
+template <bool DEBUG>
 __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMem& m, unsigned qi, bool skip1r,
                                               unsigned long long* st) {
     const int lane = threadIdx.x & 63;
@@ -654,7 +657,7 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
         // debug image values (Reconstruction.hpp:46, ReconstructionOur.cpp:312-314):
         // surfaces of the last gather, samples within 1R (known to the fast
         // kernel when it handed the 2R fetch over)
-        if (Q.qdebug) {
+        if (DEBUG) {
             Q.qdebug[qi].x = (float)g.nsurf;
             if (!skip1r) Q.qdebug[qi].y = (float)n1;
         }
@@ -676,6 +679,7 @@ __device__ __forceinline__ void flush_stats(unsigned long long* dst, const unsig
 // 7 waves per SIMD (the register limit this sets spills a little; the
 // kernel waits on memory and gains more from resident waves).  Persistent:
 // waves take queries from an atomic counter until the list is done.
+template <bool DEBUG>
 __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -697,7 +701,7 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
             skip1r = (qi & kNeeds2R) != 0;
             qi &= ~kNeeds2R;
         }
-        general_query(Q, m, qi, skip1r, st);
+        general_query<DEBUG>(Q, m, qi, skip1r, st);
     }
     flush_stats(Q.stats, st, Q.collect_stats != 0);
 }
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
             }
             do {
                 const unsigned ent = (unsigned)uni((int)B.list[e]);
-                if (ent != kSkipEntry) general_query(Q, m, ent & ~kNeeds2R, (ent & kNeeds2R) != 0, st);
+                if (ent != kSkipEntry) general_query<false>(Q, m, ent & ~kNeeds2R, (ent & kNeeds2R) != 0, st);
                 e++;
             } while (e < end && uni((int)B.list_item[e]) == item);
         }
@@ -1396,7 +1400,7 @@ __global__ void k_take_rows(const uint32_t* __restrict__ src, uint32_t* __restri
 
 int general_waves_per_cu() {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general<false>, kBlock, 0) != hipSuccess ||
         blocks < 1)
         blocks = 1;
     return blocks * kWavesPerBlock;
@@ -1405,7 +1409,10 @@ int general_waves_per_cu() {
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     const int blocks = p.num_waves / kWavesPerBlock;
-    k_query_general<<<blocks, kBlock, 0, s>>>(p);
+    if (p.qdebug)
+        k_query_general<true><<<blocks, kBlock, 0, s>>>(p);
+    else
+        k_query_general<false><<<blocks, kBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 

@@ -384,6 +384,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     const size_t scan_words = scan_tmp_words(std::max<size_t>(std::max<size_t>((size_t)N, cells_max), hist_words));
     size_t words = 0;
     words += 4 * (size_t)N;           // rx ry rw rt
+    words += 12 * (size_t)N;          // packed reprojected records
     words += 4 * (size_t)N;           // radix ping-pong
     words += 3 * (size_t)N;           // cand ct cw
     words += 12 * (size_t)N + 8 * kPointPad;  // points: pab (2 x float4, padded), pc
@@ -410,6 +411,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     A.reset();
 
     c->rx = A.alloc<float>(N); c->ry = A.alloc<float>(N); c->rw = A.alloc<float>(N); c->rt = A.alloc<float>(N);
+    float4* packed = A.alloc<float4>(3 * (size_t)std::max(N, 1));
     uint32_t* k0 = A.alloc<uint32_t>(N);
     uint32_t* v0 = A.alloc<uint32_t>(N);
     uint32_t* k1 = A.alloc<uint32_t>(N);
@@ -428,7 +430,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     HIPCHK(c, hipMemcpyAsync(c->d_gi, &g0, sizeof(g0), hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(hits, 0, sizeof(int) * (size_t)W * H, s));
     const InSoA in = in_view(c);
-    launch_reproject(in, N, W, H, c->h.coc_c0, c->h.coc_c1, c->rx, c->ry, c->rw, c->rt, hits, c->d_gi, s);
+    launch_reproject(in, N, W, H, c->h.coc_c0, c->h.coc_c1, c->rx, c->ry, c->rw, c->rt, packed, hits, c->d_gi, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&c->gi, c->d_gi, sizeof(GridInfo), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -474,7 +476,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     float* cw = A.alloc<float>(std::max(nb, 1));
     if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (candidates)");
     launch_emit_initial(owner, own_flag, own_y1, own_idx, sat, gw, gh, c->ileaf_rect, c->ileaf_oc, s);
-    launch_place(skeys, svals, nb, gw, owner, sat, cell_start, c->rt, c->rw, c->cand, ct, cw, s);
+    launch_place(skeys, svals, nb, gw, owner, sat, cell_start, packed, c->cand, ct, cw, s);
     launch_leaf_sort(c->ileaf_oc, nil, c->cand, ct, cw, s);
     HIPCHK(c, hipGetLastError());
 
@@ -489,7 +491,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     uint32_t* sflag = A.alloc<uint32_t>(nb1 + 1);
     uint32_t* lpos = A.alloc<uint32_t>(nb1 + 1);
     if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (points)");
-    launch_points(c->cand, nb, in, c->rx, c->ry, c->rw, c->rt, (float)W, (float)H, c->cc0, c->cc1, P, sb, boxes, s);
+    launch_points(c->cand, nb, packed, (float)W, (float)H, c->cc0, c->cc1, P, sb, boxes, s);
     HIPCHK(c, hipMemsetAsync(sflag, 0, sizeof(uint32_t) * (nb1 + 1), s));
     HIPCHK(c, hipMemsetAsync(P.pab + 2 * (size_t)nb, 0, sizeof(float4) * 2 * kPointPad, s));
     launch_surface_scan(c->ileaf_oc, nil, P.pab, boxes, sflag, s);
