@@ -1181,13 +1181,41 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
 // order each wave takes one output-sample index over an 8x8 pixel tile, so
 // its lanes' (u,v,t) come from one time stratum of the patterns; lanes of
 // tiles that overhang the image get no query.  Lists keep their own order.
-__device__ __forceinline__ unsigned tile_slot(const QueryParams& Q, unsigned slot) {
+// Tile order 2 keeps the whole device on one band of tile rows (the band's
+// samples stay in the last-level cache) while each XCD (workgroup b runs on
+// XCD b % 8) works on its own eighth of the band's columns, so that its L2
+// holds a window an eighth of the band's width.  The eighth moves by one
+// segment per tile row: a fixed segment per XCD would leave the XCDs with
+// the cheap parts of the image idle while the others finish theirs (XCDs
+// take their workgroups independently).  Measured at 1920x1080 x 16: the
+// L2's fetches fall from 310 to about 110 GB per launch; fixed segments
+// are 22% slower than tile order 1, rotated ones 2% faster.
+template <int BLOCK>
+__device__ __forceinline__ unsigned tile_slot(const QueryParams& Q, unsigned b, unsigned t) {
+    const unsigned slot = b * BLOCK + t;
     if (!Q.tile_order || Q.fast_list) return slot;
-    const unsigned lane = slot & 63u, grp = slot >> 6;
-    const unsigned i = grp % (unsigned)Q.M, tile = grp / (unsigned)Q.M;
-    const unsigned tiles_x = ((unsigned)Q.W + 7u) / 8u;
-    const unsigned x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
-    return (x < (unsigned)Q.W && y < (unsigned)Q.H) ? (y * (unsigned)Q.W + x) * (unsigned)Q.M + i : 0xFFFFFFFFu;
+    const unsigned lane = t & 63u, M = (unsigned)Q.M;
+    unsigned tx, ty, i;
+    if (Q.tile_order == 2) {
+        constexpr unsigned kWpb = BLOCK / 64;
+        static_assert(kWpb >= 1 && 16 % kWpb == 0, "a segment's waves are whole workgroups");
+        const unsigned seg_tiles = ((((unsigned)Q.W + 7u) / 8u + 15u) & ~15u) / 8u;
+        const unsigned seg_blocks = seg_tiles * M / kWpb;
+        const unsigned x = b & 7u, k = b >> 3;
+        ty = k / seg_blocks;
+        const unsigned ws = (k - ty * seg_blocks) * kWpb + (t >> 6);
+        tx = ((x + ty) & 7u) * seg_tiles + ws / M;
+        i = ws % M;
+    } else {
+        const unsigned grp = slot >> 6;
+        const unsigned tiles_x = ((unsigned)Q.W + 7u) / 8u;
+        i = grp % M;
+        const unsigned tile = grp / M;
+        tx = tile % tiles_x;
+        ty = tile / tiles_x;
+    }
+    const unsigned x = tx * 8u + (lane & 7u), y = ty * 8u + (lane >> 3);
+    return (x < (unsigned)Q.W && y < (unsigned)Q.H) ? (y * (unsigned)Q.W + x) * M + i : 0xFFFFFFFFu;
 }
 
 template <bool STATS>
@@ -1196,16 +1224,15 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     __shared__ int lid[kFastLeafCap][kFastBlock];
     // Queries in scan order (pixel-major): the whole device sweeps one band
     // of rows, whose samples stay resident in the last-level cache.
-    const unsigned slot = blockIdx.x * kFastBlock + threadIdx.x;
-    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, tile_slot(Q, slot), 0u, lid);
+    fast_query<STATS, kFastLeafCap, kFastBlock>(Q, tile_slot<kFastBlock>(Q, blockIdx.x, threadIdx.x), 0u, lid);
 }
 
 // Gated fast path (single buffer): as k_query_fast<false>, plus provisional
 // colours and gate features for the hand-offs.
 __global__ __launch_bounds__(kFastBlock, 7) void k_query_fast_gate(QueryParams Q) {
     __shared__ int lid[kFastLeafCap][kFastBlock];
-    const unsigned slot = blockIdx.x * kFastBlock + threadIdx.x;
-    fast_query<false, kFastLeafCap, kFastBlock, true>(Q, tile_slot(Q, slot), 0u, lid);
+    fast_query<false, kFastLeafCap, kFastBlock, true>(Q, tile_slot<kFastBlock>(Q, blockIdx.x, threadIdx.x), 0u,
+                                                      lid);
 }
 
 // Batched fast path: one launch over every buffer of a batch.  Block b of the
@@ -1223,8 +1250,7 @@ __global__ __launch_bounds__(BLOCK) void k_query_fast_batch(BatchQuery B) {
         else hi = mid - 1;
     }
     const QueryParams Q = B.items[lo];
-    fast_query<false, CAP, BLOCK>(Q, tile_slot(Q, (b - B.block_prefix[lo]) * BLOCK + threadIdx.x), (unsigned)lo,
-                                  lid);
+    fast_query<false, CAP, BLOCK>(Q, tile_slot<BLOCK>(Q, b - B.block_prefix[lo], threadIdx.x), (unsigned)lo, lid);
 }
 
 // Learned-gate decision over the fast kernel's hand-off list: entry e keeps

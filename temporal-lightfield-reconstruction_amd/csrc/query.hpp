@@ -37,6 +37,11 @@ enum StatIdx {
 constexpr size_t kWaveScratchWords =
     (size_t)kStackCap + 3 * (size_t)kLeafCap + 5 * (size_t)kSampleCap + 5 * (size_t)kLeafCap;
 
+// Tile order the single-buffer reconstruction uses (QueryParams::tile_order).
+#ifndef TLR_TILE_ORDER
+#define TLR_TILE_ORDER 2
+#endif
+
 struct QueryParams {
     int W, H, M;              // image size, output samples per pixel
     int spp_pow2;             // input spp rounded up to a power of two
@@ -70,7 +75,10 @@ struct QueryParams {
     // entry's position in fail_list)
     float* features;
     // fast kernel over all queries: 0 = lanes in query order (pixel-major);
-    // 1 = each wave takes one output-sample index over an 8x8 pixel tile
+    // 1 = each wave takes one output-sample index over an 8x8 pixel tile;
+    // 2 = tiles as in 1, each tile row cut into 8 column segments, segment
+    //     (x + row) % 8 run by the workgroups of XCD x (tile columns padded
+    //     to a multiple of 16)
     int tile_order;
     // first image row of this launch: its W x H pixels are rows [y0, y0+H)
     // of the image (0 for a whole image); query ids and qcol are local
@@ -85,8 +93,9 @@ struct QueryParams {
 // included, so the count rounds up past num_queries.
 inline size_t fast_slots(const QueryParams& Q) {
     if (!Q.tile_order) return Q.num_queries;
-    const size_t tiles = (size_t)((Q.W + 7) / 8) * (size_t)((Q.H + 7) / 8);
-    return tiles * (size_t)Q.M * 64;
+    size_t tiles_x = (size_t)((Q.W + 7) / 8);
+    if (Q.tile_order == 2) tiles_x = (tiles_x + 15) & ~(size_t)15;
+    return tiles_x * (size_t)((Q.H + 7) / 8) * (size_t)Q.M * 64;
 }
 
 // A batch of built buffers queried by one launch of each kernel.

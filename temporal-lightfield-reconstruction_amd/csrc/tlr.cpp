@@ -623,7 +623,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     Q.qdebug = c->qdebug;
     // tile order pads every 8x8 tile to whole waves; the fast kernel's lane
     // index is 32-bit, so very narrow images fall back to query order
-    Q.tile_order = 1;
+    Q.tile_order = TLR_TILE_ORDER;
     if (fast_slots(Q) >= ((size_t)1 << 32)) Q.tile_order = 0;
     return TLR_OK;
 }
