@@ -185,6 +185,13 @@ typedef struct tlr_batch_item {
     float* rgba_out;             /* W*H*4 floats, or NULL to keep on device   */
     int    out_on_device;        /* rgba_out is device memory                 */
     float* build_ms;             /* optional: this item's build time          */
+    int    gate;                 /* 1: learned gate (the context's parameters
+                                    and threshold, as tlr_recon_params.gate);
+                                    a batch with a gated item runs the chained
+                                    form                                      */
+    float* done_ms;              /* optional: time from the batch's start (its
+                                    first build) to this item's image being
+                                    complete on the device                    */
 } tlr_batch_item;
 
 typedef struct tlr_batch_stats {
@@ -197,6 +204,7 @@ typedef struct tlr_batch_stats {
     int    graph_reused;         /* 1: the previous call's graph was replayed */
     int    autotuned;            /* 1: this call ran the autotuner            */
     int    chained;              /* 1: chained form, 0: merged form           */
+    double gate_accepted;        /* hand-offs the gate accepted, all items    */
 } tlr_batch_stats;
 
 /* Uploads buffer `slot` (AoS records as for tlr_set_samples) to device

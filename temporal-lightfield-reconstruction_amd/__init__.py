@@ -79,14 +79,15 @@ class Stats(C.Structure):
 class BatchItem(C.Structure):
     _fields_ = [("slot", C.c_int), ("aperture_adjust", C.c_float), ("focal_adjust", C.c_float),
                 ("out_spp", C.c_int), ("rgba_out", C.c_void_p), ("out_on_device", C.c_int),
-                ("build_ms", C.POINTER(C.c_float))]
+                ("build_ms", C.POINTER(C.c_float)), ("gate", C.c_int), ("done_ms", C.POINTER(C.c_float))]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("query_ms", C.c_double), ("total_ms", C.c_double),
                 ("output_samples", C.c_double), ("general_queries", C.c_double),
                 ("fast_leaf_cap", C.c_int), ("fast_block", C.c_int), ("general_blocks", C.c_int),
-                ("graph_reused", C.c_int), ("autotuned", C.c_int), ("chained", C.c_int)]
+                ("graph_reused", C.c_int), ("autotuned", C.c_int), ("chained", C.c_int),
+                ("gate_accepted", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -411,13 +412,17 @@ class Ctx:
             self._slot_shapes = {}
         self._slot_shapes[slot] = (sbuf.height, sbuf.width)
 
-    def reconstruct_batch(self, jobs, outs=None):
+    def reconstruct_batch(self, jobs, outs=None, gate=False, done=None):
         """jobs: list of (slot, out_spp[, aperture_adjust, focal_adjust]);
-        outs: optional list of (H, W, 4) float32 arrays (host).  Returns
-        (images, per-item build ms, batch stats)."""
+        outs: optional list of (H, W, 4) float32 arrays (host); gate: the
+        learned gate for every item (or a list of flags); done: optional list
+        that receives each item's completion time (ms from the batch's start).
+        Returns (images, per-item build ms, batch stats)."""
         n = len(jobs)
         items = (BatchItem * n)()
         bms = (C.c_float * n)()
+        dms = (C.c_float * n)()
+        gates = list(gate) if isinstance(gate, (list, tuple)) else [bool(gate)] * n
         imgs = []
         for k, job in enumerate(jobs):
             slot, out_spp = job[0], job[1]
@@ -433,8 +438,12 @@ class Ctx:
             items[k].rgba_out = img.ctypes.data if img is not None else None
             items[k].out_on_device = 0
             items[k].build_ms = C.cast(C.addressof(bms) + 4 * k, C.POINTER(C.c_float))
+            items[k].gate = 1 if gates[k] else 0
+            items[k].done_ms = C.cast(C.addressof(dms) + 4 * k, C.POINTER(C.c_float))
         st = BatchStats()
         self._chk(lib().tlr_reconstruct_batch(self.h, items, n, C.byref(st)))
+        if done is not None:
+            done[:] = [dms[k] for k in range(n)]
         return imgs, [bms[k] for k in range(n)], st.as_dict()
 
     def calibrate_traffic(self, kind, nbytes):

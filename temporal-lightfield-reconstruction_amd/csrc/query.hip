@@ -1515,6 +1515,15 @@ hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out, h
 }
 
 
+__global__ void k_stamp(unsigned long long* dst) {
+    if (threadIdx.x == 0) *dst = wall_clock64();
+}
+
+hipError_t launch_stamp(unsigned long long* dst, hipStream_t s) {
+    k_stamp<<<1, 64, 0, s>>>(dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_calibrate(int kind, const float4* src, float4* dst, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n + 255) / 256);

@@ -150,6 +150,9 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
 // for every hand-off entry, its gate features in p.features.
 hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s);
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
+// Writes the device's constant-rate wall clock (hipDeviceAttributeWallClockRate
+// kHz) to *dst when the stream reaches it (also inside a captured graph).
+hipError_t launch_stamp(unsigned long long* dst, hipStream_t s);
 // Debug image from QueryParams::qdebug: pixel = (qdebug of its last output
 // sample, 0, 0).
 hipError_t launch_debug_image(const float2* qdebug, int W, int H, int M, float* out_rgba, hipStream_t s);

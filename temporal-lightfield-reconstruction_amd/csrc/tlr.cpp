@@ -53,7 +53,22 @@ struct BatchSlot {
     float* img = nullptr;
     int M = 0;
     size_t nq = 0;
+    int gate = 0;            // this batch: the learned gate on
     QueryParams q{};
+};
+
+// Learned-gate buffers of a batch's chained form, shared by its gated items
+// (they run one after another), and the per-item completion stamps.  Without
+// `gate` every item runs ungated whatever its flag (the autotuner's timing
+// runs have no gate buffers).
+struct BatchExtras {
+    bool gate = false;
+    float* features = nullptr;   // kGateFeatures per hand-off entry
+    float* logits = nullptr;
+    unsigned* accepted = nullptr;  // one word per item
+    float gate_thr = 0.0f;
+    const float* gate_p = nullptr;
+    unsigned long long* stamps = nullptr;  // per item, after its image
 };
 
 }  // namespace
@@ -127,6 +142,8 @@ struct tlr_ctx {
     float* gate_g = nullptr;
     float* gate_loss = nullptr;
     bool has_gate = false;
+    unsigned long long* d_stamps = nullptr;  // batch completion stamps (n + 1)
+    int stamps_cap = 0;
     float gate_thr = 0.0f;
     int gate_step = 0;
     float* train_X = nullptr;
@@ -225,6 +242,7 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->d_gi) (void)hipFree(c->d_gi);
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_counter) (void)hipFree(c->d_counter);
+    if (c->d_stamps) (void)hipFree(c->d_stamps);
     if (c->d_fail_count) (void)hipFree(c->d_fail_count);
     if (c->d_accept_count) (void)hipFree(c->d_accept_count);
     for (float* p : {c->gate_p, c->gate_m, c->gate_v, c->gate_g, c->gate_loss, c->train_X, c->train_y})
@@ -1536,12 +1554,17 @@ constexpr int kBatchChainWord = kBatchStatsWord + 2 * kNumStats;
 // Query phase of a batch, merged form: one launch of each query kernel over
 // every buffer, with a given fast variant and general grid.
 static tlr_status batch_queries(tlr_ctx* c, const BatchQuery& B, int variant, int general_blocks,
-                                const std::vector<BatchSlot*>& its, unsigned* counters) {
+                                const std::vector<BatchSlot*>& its, unsigned* counters,
+                                unsigned long long* stamps) {
     hipStream_t s = c->stream;
     HIPCHK(c, launch_fill_u32(counters, kBatchChainWord, 0u, s));
     HIPCHK(c, launch_query_fast_batch(B, variant, s));
     HIPCHK(c, launch_query_general_batch(B, general_blocks, s));
-    for (BatchSlot* b : its) HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
+    for (size_t i = 0; i < its.size(); i++) {
+        BatchSlot* b = its[i];
+        HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
+        if (stamps) HIPCHK(c, launch_stamp(stamps + i, s));
+    }
     return TLR_OK;
 }
 
@@ -1550,10 +1573,11 @@ static tlr_status batch_queries(tlr_ctx* c, const BatchQuery& B, int variant, in
 // the hand-off list and the general kernel's scratch.  No host work between
 // buffers, so in a graph it runs back to back.
 static tlr_status batch_chain(tlr_ctx* c, const std::vector<BatchSlot*>& its, unsigned* list, unsigned* counters,
-                              float* scratch, int waves) {
+                              float* scratch, int waves, const BatchExtras& x) {
     hipStream_t s = c->stream;
     const int n = (int)its.size();
     HIPCHK(c, launch_fill_u32(counters, kBatchChainWord + 2 * n, 0u, s));
+    if (x.gate) HIPCHK(c, launch_fill_u32(x.accepted, n, 0u, s));
     for (int i = 0; i < n; i++) {
         BatchSlot* b = its[i];
         QueryParams Q = b->q;
@@ -1565,11 +1589,23 @@ static tlr_status batch_chain(tlr_ctx* c, const std::vector<BatchSlot*>& its, un
         Q.stats = reinterpret_cast<unsigned long long*>(counters + kBatchStatsWord);
         Q.scratch = scratch;
         Q.num_waves = waves;
-        HIPCHK(c, launch_query_fast(Q, false, s));
+        if (b->gate && x.gate) {
+            // as tlr_reconstruct with the gate: provisional colours and
+            // features for the hand-offs, the gate's scores, the general
+            // kernel over the entries it does not accept
+            QueryParams F = Q;
+            F.features = x.features;
+            HIPCHK(c, launch_query_fast_gate(F, s));
+            HIPCHK(c, launch_gate_forward_dev(x.features, pair + 1, b->nq, x.gate_p, x.logits, s));
+            HIPCHK(c, launch_gate_accept(x.logits, x.features, list, pair + 1, b->nq, x.gate_thr, x.accepted + i, s));
+        } else {
+            HIPCHK(c, launch_query_fast(Q, false, s));
+        }
         Q.qlist = list;
         Q.qcount = pair + 1;
         HIPCHK(c, launch_query_general(Q, s));
         HIPCHK(c, launch_combine(b->qcol, b->h.width, b->h.height, b->M, b->img, s));
+        if (x.stamps) HIPCHK(c, launch_stamp(x.stamps + i, s));
     }
     return TLR_OK;
 }
@@ -1581,6 +1617,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     hipStream_t s = c->stream;
     std::vector<BatchSlot*> its((size_t)n);
     double build_ms = 0;
+    bool any_gate = false;
     for (int i = 0; i < n; i++) {
         const tlr_batch_item& it = items[i];
         if (it.slot < 0 || it.slot >= (int)c->slots.size() || c->slots[it.slot]->N == 0)
@@ -1588,7 +1625,22 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         if (it.out_spp <= 0) return c->fail(TLR_E_INVALID, "batch item %d: out_spp must be positive", i);
         for (int j = 0; j < i; j++)
             if (items[j].slot == it.slot) return c->fail(TLR_E_INVALID, "slot %d appears twice in the batch", it.slot);
+        any_gate = any_gate || it.gate != 0;
+    }
+    if (any_gate && !c->has_gate) return c->fail(TLR_E_STATE, "gate requested but no gate parameters set");
+    // completion stamps: [n] = the batch's start, [i] = item i's image done
+    if (c->stamps_cap < n + 1) {
+        if (c->d_stamps) HIPCHK(c, hipFree(c->d_stamps));
+        c->d_stamps = nullptr;
+        c->stamps_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_stamps, sizeof(unsigned long long) * (size_t)(n + 1)));
+        c->stamps_cap = n + 1;
+    }
+    HIPCHK(c, launch_stamp(c->d_stamps + n, s));
+    for (int i = 0; i < n; i++) {
+        const tlr_batch_item& it = items[i];
         its[i] = c->slots[it.slot];
+        its[i]->gate = it.gate != 0;
         float ms = 0;
         tlr_status st = batch_build(c, its[i], it, &ms);
         if (st != TLR_OK) return st;
@@ -1602,11 +1654,15 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
     const int cus = c->prop.multiProcessorCount;
     const int waves = std::max(4, cus * c->general_wpc);
-    for (int i = 0; i < n; i++) total_q += its[i]->nq;
+    size_t gate_nq = 0;  // largest gated item
+    for (int i = 0; i < n; i++) {
+        total_q += its[i]->nq;
+        if (its[i]->gate) gate_nq = std::max(gate_nq, its[i]->nq);
+    }
     if (total_q >= ((size_t)1 << 31)) return c->fail(TLR_E_INVALID, "batch has too many output samples");
-    const size_t bbytes = sizeof(QueryParams) * n + 4 * (size_t)(n + 1) * kNumFastVariants + 8 * total_q +
-                          4 * ((size_t)kBatchChainWord + 2 * (size_t)n) + (size_t)waves * kWaveScratchWords * 4 +
-                          16 * 256;
+    size_t bbytes = sizeof(QueryParams) * n + 4 * (size_t)(n + 1) * kNumFastVariants + 8 * total_q +
+                    4 * ((size_t)kBatchChainWord + 2 * (size_t)n) + (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
+    if (any_gate) bbytes += 4 * gate_nq * ((size_t)kGateFeatures + 1) + 4 * (size_t)n + 3 * 256;
     HIPCHK(c, c->barena.reserve(bbytes));
     c->barena.reset();
     QueryParams* d_items = c->barena.alloc<QueryParams>(n);
@@ -1615,6 +1671,17 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     unsigned* d_item = c->barena.alloc<unsigned>(total_q);
     unsigned* d_cnt = c->barena.alloc<unsigned>((size_t)kBatchChainWord + 2 * (size_t)n);
     float* d_scratch = c->barena.alloc<float>((size_t)waves * kWaveScratchWords);
+    BatchExtras xs;
+    xs.stamps = c->d_stamps;
+    if (any_gate) {
+        xs.gate = true;
+        xs.features = c->barena.alloc<float>(gate_nq * kGateFeatures);
+        xs.logits = c->barena.alloc<float>(gate_nq);
+        xs.accepted = c->barena.alloc<unsigned>((size_t)n);
+        xs.gate_thr = c->gate_thr;
+        xs.gate_p = c->gate_p;
+    }
+    BatchExtras xs_plain;  // what the autotuner runs: no gate, no stamps
     if (c->barena.failed) return c->fail(TLR_E_HIP, "batch arena exhausted");
     std::vector<QueryParams> qp((size_t)n);
     for (int i = 0; i < n; i++) {
@@ -1673,7 +1740,8 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         int best_v = 0, best_g = general_full, chain = c->batch_mode == 1 ? 1 : 0;
         if (c->batch_mode != 1) {
             for (int v = 0; v < kNumFastVariants; v++) {
-                tlr_status st = timed([&] { return batch_queries(c, make_bq(v), v, general_full, its, d_cnt); }, &ms);
+                tlr_status st =
+                    timed([&] { return batch_queries(c, make_bq(v), v, general_full, its, d_cnt, nullptr); }, &ms);
                 if (st != TLR_OK) return st;
                 if (ms < best) {
                     best = ms;
@@ -1681,7 +1749,8 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
                 }
             }
             const int g34 = std::max(1, general_full * 3 / 4);
-            tlr_status st = timed([&] { return batch_queries(c, make_bq(best_v), best_v, g34, its, d_cnt); }, &ms);
+            tlr_status st =
+                timed([&] { return batch_queries(c, make_bq(best_v), best_v, g34, its, d_cnt, nullptr); }, &ms);
             if (st != TLR_OK) return st;
             if (ms < best) {
                 best = ms;
@@ -1689,7 +1758,7 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
             }
         }
         if (c->batch_mode == -1) {
-            tlr_status st = timed([&] { return batch_chain(c, its, d_list, d_cnt, d_scratch, waves); }, &ms);
+            tlr_status st = timed([&] { return batch_chain(c, its, d_list, d_cnt, d_scratch, waves, xs_plain); }, &ms);
             if (st != TLR_OK) return st;
             if (ms < best) chain = 1;
         }
@@ -1700,19 +1769,27 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
     }
     const int v = c->tuned_variant;
     const BatchQuery B = make_bq(v);
+    // the gate runs in the chained form (the single-buffer gated kernels)
+    const int chain = any_gate ? 1 : c->tuned_chain;
     // graph of the query phase, keyed by everything its launches capture:
     // the chained form passes each item's QueryParams by value (CoC
     // coefficients, spp, tree size and root, every device pointer), so all
     // of its bytes are part of the key, as are the shared buffers' addresses
+    uint32_t thr_bits = 0;
+    memcpy(&thr_bits, &xs.gate_thr, sizeof(thr_bits));
     std::vector<uint64_t> key = {(uint64_t)n, (uint64_t)v, (uint64_t)c->tuned_general_blocks,
-                                 (uint64_t)c->tuned_chain, (uint64_t)(uintptr_t)c->barena.base,
+                                 (uint64_t)chain, (uint64_t)(uintptr_t)c->barena.base,
                                  (uint64_t)B.total_blocks, (uint64_t)(uintptr_t)d_items,
                                  (uint64_t)(uintptr_t)d_prefix, (uint64_t)(uintptr_t)d_cnt,
-                                 (uint64_t)(uintptr_t)d_scratch, (uint64_t)waves};
+                                 (uint64_t)(uintptr_t)d_scratch, (uint64_t)waves,
+                                 (uint64_t)(uintptr_t)xs.stamps, (uint64_t)(uintptr_t)xs.features,
+                                 (uint64_t)(uintptr_t)xs.logits, (uint64_t)(uintptr_t)xs.accepted,
+                                 (uint64_t)(uintptr_t)xs.gate_p, (uint64_t)thr_bits};
     static_assert(sizeof(QueryParams) % sizeof(uint64_t) == 0, "QueryParams is keyed as 64-bit words");
     for (BatchSlot* b : its) {
         key.push_back((uint64_t)(uintptr_t)b->img);
         key.push_back(((uint64_t)b->h.width << 40) | ((uint64_t)b->h.height << 20) | (uint64_t)b->M);
+        key.push_back((uint64_t)b->gate);
         uint64_t words[sizeof(QueryParams) / sizeof(uint64_t)];
         memcpy(words, &b->q, sizeof(QueryParams));
         key.insert(key.end(), words, words + sizeof(QueryParams) / sizeof(uint64_t));
@@ -1723,8 +1800,8 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         c->bgraph = nullptr;
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        tlr_status st = c->tuned_chain ? batch_chain(c, its, d_list, d_cnt, d_scratch, waves)
-                                       : batch_queries(c, B, v, c->tuned_general_blocks, its, d_cnt);
+        tlr_status st = chain ? batch_chain(c, its, d_list, d_cnt, d_scratch, waves, xs)
+                              : batch_queries(c, B, v, c->tuned_general_blocks, its, d_cnt, xs.stamps);
         hipError_t e = hipStreamEndCapture(s, &g);
         if (st != TLR_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -1747,13 +1824,24 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
                                  it.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
     std::vector<unsigned> cnt((size_t)kBatchChainWord + 2 * (size_t)n);
+    std::vector<unsigned long long> stamps((size_t)n + 1);
+    std::vector<unsigned> acc((size_t)n, 0u);
     HIPCHK(c, hipMemcpyAsync(cnt.data(), d_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(stamps.data(), c->d_stamps, 8 * stamps.size(), hipMemcpyDeviceToHost, s));
+    if (xs.accepted) HIPCHK(c, hipMemcpyAsync(acc.data(), xs.accepted, 4 * acc.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     unsigned long long st64[kNumStats];
     memcpy(st64, cnt.data() + kBatchStatsWord, sizeof(st64));
     double handed = cnt[1];
-    if (c->tuned_chain)
+    if (chain)
         for (int i = 0; i < n; i++) handed += cnt[(size_t)kBatchChainWord + 2 * (size_t)i + 1];
+    double accepted = 0;
+    for (unsigned a : acc) accepted += a;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) khz = 0;
+    for (int i = 0; i < n; i++)
+        if (items[i].done_ms)
+            *items[i].done_ms = khz ? (float)((double)(stamps[i] - stamps[n]) / (double)khz) : 0.0f;
     if (bs) {
         memset(bs, 0, sizeof(*bs));
         float ms = 0;
@@ -1762,12 +1850,13 @@ tlr_status tlr_reconstruct_batch(tlr_ctx* c, const tlr_batch_item* items, int n,
         bs->total_ms = bs->build_ms + bs->query_ms;
         bs->general_queries = handed;
         bs->output_samples = (double)total_q;
-        bs->fast_leaf_cap = c->tuned_chain ? kFastLeafCap : fast_variant_cap(v);
-        bs->fast_block = c->tuned_chain ? kFastBlock : fast_variant_block(v);
-        bs->general_blocks = c->tuned_chain ? general_full : c->tuned_general_blocks;
+        bs->fast_leaf_cap = chain ? kFastLeafCap : fast_variant_cap(v);
+        bs->fast_block = chain ? kFastBlock : fast_variant_block(v);
+        bs->general_blocks = chain ? general_full : c->tuned_general_blocks;
         bs->graph_reused = reused ? 1 : 0;
         bs->autotuned = tuned_now ? 1 : 0;
-        bs->chained = c->tuned_chain;
+        bs->chained = chain;
+        bs->gate_accepted = accepted;
     }
     if (st64[ST_OVERFLOW])
         return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",

@@ -143,3 +143,57 @@ def test_batch_new_parameters_are_not_replayed_stale(tlr, mode):
     sbufs[1] = tlr.make_scene(p, W, H, spp, seed=1234)
     ctx.batch_set_samples(1, sbufs[1])
     check(jobs, sbufs, "re-uploaded slot")
+
+
+def _committed_gate(tlr):
+    import json
+    import os
+    d = os.path.dirname(tlr.__file__)
+    params = np.fromfile(os.path.join(d, "gate_params.f32"), dtype="<f4")
+    with open(os.path.join(d, "gate_params.json")) as f:
+        return params, json.load(f)["threshold"]
+
+
+def test_gated_batch_equals_gated_single_buffer(tlr, batch):
+    """Items with the learned gate on (committed parameters) give the gated
+    single-buffer path's images bit for bit, the others the plain ones; the
+    batch runs the chained form, counts the hand-offs the gate accepted as the
+    single-buffer calls do, replays its graph, and recaptures it (no stale
+    threshold) when the threshold changes.  Completion times grow in item
+    order."""
+    sbufs, _, jobs, _, _ = batch
+    params, thr = _committed_gate(tlr)
+    flags = [True, False, True, True, False]
+    ctx = tlr.Ctx(0)
+    ctx.gate_set_params(params)
+    single = tlr.Ctx(0)
+    single.gate_set_params(params)
+    for k, sb in enumerate(sbufs):
+        ctx.batch_set_samples(k, sb)
+    for t in (thr, thr - 1.0):
+        ctx.gate_set_threshold(t)
+        single.gate_set_threshold(t)
+        print("gated batch, threshold %.4f" % t, flush=True)
+        done = []
+        imgs, _, st = ctx.reconstruct_batch(jobs, gate=flags, done=done)
+        imgs2, _, st2 = ctx.reconstruct_batch(jobs, gate=flags)
+        assert st["chained"] == 1 and st2["graph_reused"] == 1
+        accepted = 0.0
+        for k, sb in enumerate(sbufs):
+            single.set_samples(sb)
+            single.build()
+            ref, sst = single.reconstruct(out_spp=BUFS[k][4], gate=flags[k])
+            if flags[k]:
+                accepted += sst["gate_candidates"] - sst["gate_rejected"]
+            assert np.array_equal(imgs[k], ref), "threshold %g: buffer %d differs" % (t, k)
+            assert np.array_equal(imgs2[k], ref), "threshold %g replay: buffer %d differs" % (t, k)
+        assert st["gate_accepted"] == accepted == st2["gate_accepted"]
+        assert all(d > 0 for d in done) and all(b >= a for a, b in zip(done, done[1:]))
+
+
+def test_gated_batch_needs_parameters(tlr, batch):
+    sbufs, _, jobs, _, _ = batch
+    ctx = tlr.Ctx(0)
+    ctx.batch_set_samples(0, sbufs[0])
+    with pytest.raises(tlr.TlrError):
+        ctx.reconstruct_batch([jobs[0]], gate=True)

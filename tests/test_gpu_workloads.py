@@ -1,0 +1,87 @@
+"""GPU: the two extension workloads at their full size.
+
+Learned gate, 1920x1080 x 16 spp in, 16 out-spp: on held-out scenes of the
+training scene type (the gate was trained on seeds 100000 and up and
+calibrated on 200000; these are 300002 and 300003), the gated image stays
+within the 1e-3 mean-abs budget of the plain one with the committed
+parameters and threshold, and the gate sees exactly the plain method's
+hand-offs.
+
+Batch, 64 uneven buffers (resolutions 128^2 ... 1920x1080, 1-32 input spp,
+the bench's seeded mix): one batched call, plain and with the gate on every
+buffer, gives each buffer's single-buffer image bit for bit."""
+import concurrent.futures as cf
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BATCH_RES = [(128, 128), (256, 256), (512, 512), (1024, 1024), (960, 540), (1920, 1080)]
+BATCH_SPP = [1, 4, 8, 16, 32]
+
+
+def _committed_gate(tlr):
+    d = os.path.dirname(tlr.__file__)
+    params = np.fromfile(os.path.join(d, "gate_params.f32"), dtype="<f4")
+    with open(os.path.join(d, "gate_params.json")) as f:
+        return params, json.load(f)["threshold"]
+
+
+@pytest.mark.parametrize("seed", [300002, 300003])
+def test_gate_full_size_within_budget(tlr, seed):
+    params, thr = _committed_gate(tlr)
+    print("1920x1080 x 16, seed %d" % seed, flush=True)
+    sb = tlr.make_scene(2, 1920, 1080, 16, seed=seed)
+    ctx = tlr.Ctx(0)
+    ctx.gate_set_params(params)
+    ctx.gate_set_threshold(thr)
+    ctx.set_samples(sb)
+    del sb
+    ctx.build()
+    plain, st = ctx.reconstruct(out_spp=16)
+    gated, sg = ctx.reconstruct(out_spp=16, gate=True)
+    err = float(np.abs(gated[..., :3].astype(np.float64) - plain[..., :3]).mean())
+    acc = sg["gate_candidates"] - sg["gate_rejected"]
+    print("hand-offs %d, accepted %d, mean-abs %.3e" % (sg["gate_candidates"], acc, err), flush=True)
+    assert sg["gate_candidates"] == st["general_queries"] > 0
+    assert acc > 0
+    assert err <= 1e-3
+    # accepted hand-offs only change what they replace: far fewer pixels
+    # differ than the gate accepted output samples
+    assert np.count_nonzero(np.any(gated != plain, axis=2)) <= acc
+
+
+def test_batch_of_64_uneven_buffers(tlr):
+    seed = 2011
+    rng = np.random.default_rng(seed)
+    shapes = [(BATCH_RES[rng.integers(len(BATCH_RES))], BATCH_SPP[rng.integers(len(BATCH_SPP))]) for _ in range(64)]
+    print("64 buffers, %d records" % sum(W * H * s for (W, H), s in shapes), flush=True)
+    ctx = tlr.Ctx(0)
+
+    def gen(k):
+        (W, H), spp = shapes[k]
+        return tlr.make_scene(2, W, H, spp, seed=seed * 1000 + k)
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        futs = {ex.submit(gen, k): k for k in range(64)}
+        for fu in cf.as_completed(futs):
+            ctx.batch_set_samples(futs[fu], fu.result())
+    params, thr = _committed_gate(tlr)
+    ctx.gate_set_params(params)
+    ctx.gate_set_threshold(thr)
+    jobs = [(k, 16) for k in range(64)]
+    plain, _, st = ctx.reconstruct_batch(jobs)
+    print("plain batch: %s" % st, flush=True)
+    gated, _, sg = ctx.reconstruct_batch(jobs, gate=True)
+    print("gated batch: %s" % sg, flush=True)
+    assert sg["chained"] == 1 and sg["gate_accepted"] > 0
+    for k in range(64):
+        ctx.use_slot(k)
+        ctx.build()
+        ref, _ = ctx.reconstruct(out_spp=16)
+        assert np.array_equal(plain[k], ref), "plain: buffer %d (%s) differs" % (k, shapes[k])
+        ref, _ = ctx.reconstruct(out_spp=16, gate=True)
+        assert np.array_equal(gated[k], ref), "gated: buffer %d (%s) differs" % (k, shapes[k])
