@@ -12,7 +12,8 @@
 // common/CameraParams.hpp:92-107).  Differences: fatal conditions throw
 // FW::ReconstructionError instead of FW::fail's print-and-exit
 // (src/framework/base/Defs.cpp:304-331); the GPU is always used
-// (params.enableCuda is ignored); overrideRefocusDistance is not supported.
+// (params.enableCuda is ignored); overrideRefocusDistance needs the camera's
+// projection, windowWidth and apertureRadius set (CameraParams below).
 #pragma once
 #include <cfloat>
 #include <cstdint>
