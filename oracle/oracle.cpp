@@ -49,7 +49,15 @@
  *       primitives, bit for bit                          [test_oracle_pins]
  *   RefGpu 2R surface split             a 3-point front surface takes the
  *       next leaf in                                     [test_oracle_pins]
- *   Each of the last five rules is broken on purpose in a mutant build
+ *   shadow filter                       constructed light-space planes:
+ *       a query on / 0.02 behind / 0.009 behind the plane, a w-gradient
+ *       lighting one side, density 4 doubling the radius, the nearest
+ *       fallback lit and shadowed, the extended result   [test_oracle_shadow]
+ *   FW inverse, lightSpaceDot,          float64 inverse; the geometric
+ *   reconstructDofMotionShadows           normal and cosine to a point
+ *       light on a tilted plane; a pinhole scene lit from the camera, with
+ *       and without an occluder, in closed form          [test_oracle_shadow]
+ *   Each of the last rules is broken on purpose in a mutant build
  *   (ORC_MUTANT below) and the pins are checked to fail [test_oracle_mutants].
  * The reference itself cannot be built here (Win32-only headers), so no
  * function is pinned against reference outputs.
@@ -65,6 +73,9 @@
 //   7 emergency passes keep the dispersion    8 no nearest-sample fallback
 //   9 output times without the jitter         10 GPU-variant 2R surfaces
 //                                                split at 3 points
+//  11 shadow queries keep the dispersion      12 shadow bias added, not
+//     (no sqrt(density) scaling)                 subtracted
+//  13 shadow nearest fallback skips the light test
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
 #endif
@@ -562,6 +573,8 @@ inline float len2(float dx, float dy) { return std::sqrt(dx * dx + dy * dy); }
 
 struct RS {            // reprojected sample (TreeGather::Sample subset)
     float x, y, w, t, mvx, mvy, mvw, r, g, b;
+    float wgx, wgy;    // dw/dx, dw/dy of the input sample (Reconstruction.hpp:80)
+    float density;     // LIGHTSPACE_DENSITY channel, 1 without it (Reconstruction.cpp:547)
     int src;
 };
 
@@ -594,6 +607,7 @@ struct orc_ctx {
     bool built = false;
     int spp_pow2 = 1;
     float cc[2] = {0, 0};
+    std::vector<float> density;       // per input sample (empty: 1)
     bool coc_override = false;        // overrideRefocusDistance's coefficients
     float coc_ov[2] = {0, 0};
     std::vector<RS> repro;            // kept, input order
@@ -642,6 +656,8 @@ bool reproject_center(const float* r, const float cc0[2], RS& s) {
     s.t = nt;
     s.mvx = mvx; s.mvy = mvy; s.mvw = mvw;
     s.r = r[7]; s.g = r[8]; s.b = r[9];
+    s.wgx = r[14]; s.wgy = r[15];
+    s.density = 1.0f;
     return true;
 }
 
@@ -1003,7 +1019,10 @@ inline bool node_hit(const TLB& b, float ox, float oy, float ou, float ov, float
     return (ox >= mnx && oy >= mny && ox <= mxx && oy <= mxy);
 }
 
-struct OSample { float x, y, u, v, t; };
+struct OSample {
+    float x, y, u, v, t;
+    float w = 0.0f;    // shadow queries: the query's own w (ReconstructionOur.cpp:289)
+};
 
 long long g_nodes_tested = 0;  // instrumentation: hierarchy nodes tested
 
@@ -1164,11 +1183,34 @@ struct Filt {
         float disp;
     };
 
-    // Filterer::reconstruct: ReconstructionOur.cpp:50-359 (non-shadow)
-    Result reconstruct(const OSample& o, const orc_recon_params* p, StatsAcc& st) {
+    // Does a gathered sample (w = key at the output's t, w-gradient wg, at
+    // offset (dx, dy) = o.xy - its xy) light the shadow query?
+    // ReconstructionOur.cpp:289 with SHADOW_BIAS 0.01 (:46); FW's dot sums
+    // from zero (Math.hpp:182).
+    static bool lit(const OSample& o, float dx, float dy, float key, float wgx, float wgy) {
+        float d = 0.0f;
+        d += dx * wgx;
+        d += dy * wgy;
+        const float lhs = ORC_M(12) ? o.w + 0.01f : o.w - 0.01f;
+        return lhs <= key + d;
+    }
+
+    // Filterer::Result's xy, w, wg and density (Reconstruction.hpp:324-334):
+    // of the nearest gathered sample, at the output's (u,v,t)
+    struct Ext {
+        float x, y, w, density, wgx, wgy;
+    };
+
+    // Filterer::reconstruct: ReconstructionOur.cpp:50-359.  shadow: the query
+    // is a light-space point with its own w and density (reconstructShadow).
+    Result reconstruct(const OSample& o, const orc_recon_params* p, StatsAcc& st, bool shadow = false,
+                       float density = 1.0f, Ext* ext = nullptr) {
         Result res;
         std::memset(&res, 0, sizeof(res));
+        if (ext) std::memset(ext, 0, sizeof(*ext));   // Result::clear, xy zero-constructed
         float disp = dispersion_for(c->spp_pow2);
+        // non-uniform view sample density in light space: ReconstructionOur.cpp:78-80
+        if (shadow && !ORC_M(11)) disp *= std::sqrt(density);
         int n1 = collect(o, disp, st);
         res.nsurf1 = nsurf;
         res.n1 = n1;
@@ -1250,10 +1292,13 @@ struct Filt {
                     float dx = o.x - r.x, dy = o.y - r.y;
                     float wgt = 1 - std::sqrt(dx * dx + dy * dy) / disp;
                     if (wgt <= 0) continue;
-                    col[0] += wgt * s.r;
-                    col[1] += wgt * s.g;
-                    col[2] += wgt * s.b;
-                    col[3] += wgt * 1.0f;
+                    float sc[4] = {s.r, s.g, s.b, 1.0f};
+                    // shadows: per-input-sample thresholding, ReconstructionOur.cpp:283-291
+                    if (shadow && !lit(o, dx, dy, r.key, s.wgx, s.wgy)) sc[0] = sc[1] = sc[2] = 0.0f;
+                    col[0] += wgt * sc[0];
+                    col[1] += wgt * sc[1];
+                    col[2] += wgt * sc[2];
+                    col[3] += wgt * sc[3];
                 }
                 float oow = rcpf(col[3]);
                 for (int k = 0; k < 4; k++) col[k] *= oow;
@@ -1268,9 +1313,21 @@ struct Filt {
         }
         const RSamp& na = surf[nI0].s[nI1];
         const RS& ns = c->samples[na.index];
+        // w, wg and density from the nearest sample: ReconstructionOur.cpp:325-331
+        if (ext) {
+            ext->x = na.x; ext->y = na.y; ext->w = na.key;
+            ext->density = ns.density; ext->wgx = ns.wgx; ext->wgy = ns.wgy;
+        }
         if (col[3] == 0 && !ORC_M(8)) {
             col[0] = ns.r; col[1] = ns.g; col[2] = ns.b; col[3] = 1.0f;
+            // ReconstructionOur.cpp:341-347
+            if (shadow && !ORC_M(13) && !lit(o, o.x - na.x, o.y - na.y, na.key, ns.wgx, ns.wgy))
+                col[0] = col[1] = col[2] = 0.0f;
             res.nearest = 1;
+        }
+        if (shadow) {
+            for (int k = 0; k < 4; k++) res.color[k] = col[k];
+            return res;
         }
         if (p->lens_gaussian) {
             // Gaussian: Reconstruction.hpp:378
@@ -1656,6 +1713,147 @@ struct RefGpu {
     }
 };
 
+// ---------------------------------------------------------------------------
+// FW matrix arithmetic (src/framework/base/Math.hpp), row-major m[4*i+j] =
+// get(i, j), every sum started at zero as FW does.
+// ---------------------------------------------------------------------------
+struct M4 {
+    float m[16];
+    float& operator()(int i, int j) { return m[4 * i + j]; }
+    float operator()(int i, int j) const { return m[4 * i + j]; }
+    static M4 identity() {
+        M4 r;
+        for (int k = 0; k < 16; k++) r.m[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+        return r;
+    }
+};
+
+// Matrix * Matrix: Math.hpp:1024-1037
+M4 mmul(const M4& a, const M4& b) {
+    M4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float rr = 0.0f;
+            for (int k = 0; k < 4; k++) rr += a(i, k) * b(k, j);
+            r(i, j) = rr;
+        }
+    return r;
+}
+
+// Matrix * Vector: Math.hpp:989-1000
+void mvec(const M4& a, const float v[4], float out[4]) {
+    float r[4];
+    for (int i = 0; i < 4; i++) {
+        float rr = 0.0f;
+        for (int j = 0; j < 4; j++) rr += a(i, j) * v[j];
+        r[i] = rr;
+    }
+    for (int i = 0; i < 4; i++) out[i] = r[i];
+}
+
+// 3x3 determinant: Math.hpp:928-933
+float det3(const float v[3][3]) {
+    return v[0][0] * v[1][1] * v[2][2] - v[0][0] * v[1][2] * v[2][1] + v[1][0] * v[2][1] * v[0][2] -
+           v[1][0] * v[2][2] * v[0][1] + v[2][0] * v[0][1] * v[1][2] - v[2][0] * v[0][2] * v[1][1];
+}
+
+// inverted(): Math.hpp:955-976 (cofactors; d accumulates every i, j, so it
+// is 4 det, undone by the final * L)
+M4 minv(const M4& a) {
+    M4 r;
+    float d = 0.0f, si = 1.0f;
+    for (int i = 0; i < 4; i++) {
+        float sj = si;
+        for (int j = 0; j < 4; j++) {
+            float sub[3][3];
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) sub[k][l] = a((k < j) ? k : k + 1, (l < i) ? l : l + 1);
+            const float dd = det3(sub) * sj;
+            r(i, j) = dd;
+            d += dd * a(j, i);
+            sj = -sj;
+        }
+        si = -si;
+    }
+    const float rd = rcpf(d);
+    for (int k = 0; k < 16; k++) r.m[k] = r.m[k] * rd * 4.0f;
+    return r;
+}
+
+M4 mtrans(const M4& a) {
+    M4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) r(i, j) = a(j, i);
+    return r;
+}
+
+M4 mscale(float x, float y, float z) {
+    M4 r = M4::identity();
+    r(0, 0) = x; r(1, 1) = y; r(2, 2) = z;
+    return r;
+}
+
+M4 mtranslate(float x, float y, float z) {
+    M4 r = M4::identity();
+    r(0, 3) = x; r(1, 3) = y; r(2, 3) = z;
+    return r;
+}
+
+M4 m4_in(const float* p) {
+    M4 r;
+    for (int k = 0; k < 16; k++) r.m[k] = p[k];
+    return r;
+}
+
+// normalized(): v * (1 * rcp(length())), length = sqrt of the squares summed
+// from zero (Math.hpp:139-141)
+void normalized3(const float v[3], float out[3]) {
+    float l2 = 0.0f;
+    for (int k = 0; k < 3; k++) l2 += v[k] * v[k];
+    const float f = 1.0f * rcpf(std::sqrt(l2));
+    for (int k = 0; k < 3; k++) out[k] = v[k] * f;
+}
+
+// viewSpaceNormalFromWGradients: common/Util.cpp:172-198
+void view_normal(const float p[3], float dwdu, float dwdv, int vw, int vh, const M4& inv_proj, float n[3]) {
+    const float x = p[0], y = p[1], w = p[2];
+    const float u = x / w, v = y / w;
+    dwdu *= (float)vw / 2.0f;
+    dwdv *= (float)vh / 2.0f;
+    float b1[4] = {w + u * dwdu, v * dwdu, dwdu, 0.0f};
+    float b2[4] = {u * dwdv, w + v * dwdv, dwdv, 0.0f};
+    M4 ip = inv_proj;
+    ip(2, 0) = 0; ip(2, 1) = 0; ip(2, 2) = -1; ip(2, 3) = 0;
+    ip(3, 0) = 0; ip(3, 1) = 0; ip(3, 2) = 0; ip(3, 3) = 1;
+    mvec(ip, b1, b1);
+    mvec(ip, b2, b2);
+    const float c[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2], b1[0] * b2[1] - b1[1] * b2[0]};
+    normalized3(c, n);
+}
+
+// lightSpaceDot: common/Util.cpp:202-224, with the two matrix products it
+// forms first (invws * cameraProjectedZfromW, cameraToShadow2 *
+// invCameraProjection) taken from the setup
+float light_space_dot(const float xyw[4], float wgx, float wgy, const orc_shadow_params* sp, float n[3]) {
+    float xyzw[4];
+    mvec(m4_in(sp->zfromw_ndc), xyw, xyzw);
+    const float p3[3] = {xyzw[0], xyzw[1], xyzw[3]};
+    view_normal(p3, wgx, wgy, sp->width, sp->height, m4_in(sp->inv_projection), n);
+    float n4[4] = {n[0], n[1], n[2], 0.0f}, nl4[4];
+    mvec(m4_in(sp->shadow2_inv_t), n4, nl4);
+    float pl[4];
+    mvec(m4_in(sp->shadow2_inv_proj), xyzw, pl);
+    const float c = rcpf(pl[3]);
+    const float plight[3] = {pl[0] * c, pl[1] * c, pl[2] * c};
+    const float nl[3] = {nl4[0], nl4[1], nl4[2]};
+    float a[3], b[3];
+    normalized3(plight, a);
+    normalized3(nl, b);
+    float d = 0.0f;
+    for (int k = 0; k < 3; k++) d += a[k] * b[k];
+    return -d;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1715,6 +1913,7 @@ int orc_build(orc_ctx* c, float ap_adj, float foc_adj, int threads) {
                 continue;
             }
             s.src = k;
+            if (!c->density.empty()) s.density = c->density[k];
             c->repro.push_back(s);
         }
     }
@@ -2191,6 +2390,194 @@ int orc_same_surface(const float* n0, const float* n1, const float* o5) {
     float umin[3] = {o5[2] - RANGE, o5[3] - RANGE, o5[4] - 0.5f * RANGE};
     float umax[3] = {o5[2] + RANGE, o5[3] + RANGE, o5[4] + 0.5f * RANGE};
     return same_surface(tlb_in(n0), tlb_in(n1), umin, umax) ? 1 : 0;
+}
+
+void orc_set_density(orc_ctx* c, const float* d) {
+    if (d)
+        c->density.assign(d, d + c->N);
+    else
+        c->density.clear();
+    c->built = false;
+}
+
+// TreeGather::reconstructShadows / FilterTask::process with a query buffer
+// (Reconstruction.cpp:302-347, :648-719): every query is an output sample of
+// its own; shadow selects Filterer::reconstruct's reconstructShadow.
+int orc_reconstruct_queries(orc_ctx* c, const orc_recon_params* p, const float* q8, int n, int shadow, float* rgba,
+                            float* ext8, orc_stats* stats) {
+    if (!c->built) return 1;
+    const int threads = hw_threads(p->threads);
+    std::vector<StatsAcc> st(threads);
+    std::vector<Filt> fl(threads);
+    for (auto& f : fl) f.c = c;
+    const int chunk = 256;
+    parallel_for((n + chunk - 1) / chunk, threads, [&](int w, int ci) {
+        for (int qi = ci * chunk; qi < std::min(n, (ci + 1) * chunk); qi++) {
+            const float* q = q8 + 8 * (size_t)qi;
+            OSample o;
+            o.x = q[0]; o.y = q[1]; o.u = q[2]; o.v = q[3]; o.t = q[4]; o.w = q[5];
+            Filt::Ext e;
+            st[w].count += 1;
+            Filt::Result r = fl[w].reconstruct(o, p, st[w], shadow != 0, q[6], &e);
+            if (rgba)
+                for (int k = 0; k < 4; k++) rgba[4 * (size_t)qi + k] = r.color[k];
+            if (ext8) {
+                float* x = ext8 + 8 * (size_t)qi;
+                x[0] = e.x; x[1] = e.y; x[2] = e.w; x[3] = e.density; x[4] = e.wgx; x[5] = e.wgy;
+                x[6] = x[7] = 0.0f;
+            }
+        }
+    });
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        for (auto& a : st) {
+            stats->leaf_nodes += a.v[0];
+            stats->samples_in_leaf_nodes += a.v[1];
+            stats->samples_within_1r += a.v[2];
+            stats->samples_fetched += a.v[3];
+            stats->surfaces += a.v[4];
+            stats->num_2r += a.v[5];
+            stats->surfaces_merged += a.v[6];
+            stats->at_least_one += a.v[7];
+            stats->count += a.count;
+        }
+    }
+    return 0;
+}
+
+// The matrices of TreeGather::reconstructDofMotionShadows
+// (Reconstruction.cpp:365-388) from the two cameras.
+void orc_shadow_setup(const float* cam_proj, const float* cam_to_world, const float* light_proj,
+                      const float* light_world_to_cam, int w, int h, const float* cam_coc, const float* light_coc,
+                      float cam_far, orc_shadow_params* out) {
+    const M4 P = m4_in(cam_proj), C2W = m4_in(cam_to_world), PL = m4_in(light_proj), W2L = m4_in(light_world_to_cam);
+    const M4 invP = minv(P);
+    const M4 c2s = mmul(mmul(mmul(PL, W2L), C2W), invP);
+    const M4 c2s2 = mmul(W2L, C2W);
+    const M4 c2s2_it = mtrans(minv(c2s2));
+    const M4 c2w_it = mtrans(minv(C2W));
+    const M4 ws = mmul(mscale(0.5f * w, 0.5f * h, 0.5f), mtranslate(1.0f, 1.0f, 1.0f));
+    const M4 invws = mmul(mtranslate(-1.0f, -1.0f, -1.0f), mscale(2.f / w, 2.f / h, 2.f));
+    const M4 wsP = mmul(ws, P);
+    M4 zw = M4::identity();
+    zw(2, 2) = wsP(2, 3);
+    zw(2, 3) = -wsP(2, 2);
+    const M4 c2l = mmul(mmul(mmul(ws, c2s), invws), zw);
+    const M4 zndc = mmul(invws, zw);
+    const M4 s2ip = mmul(c2s2, invP);
+    out->width = w;
+    out->height = h;
+    out->camera_coc[0] = cam_coc[0]; out->camera_coc[1] = cam_coc[1];
+    out->light_coc[0] = light_coc[0]; out->light_coc[1] = light_coc[1];
+    out->camera_far = cam_far;
+    std::memcpy(out->c2l, c2l.m, 64);
+    std::memcpy(out->zfromw_ndc, zndc.m, 64);
+    std::memcpy(out->inv_projection, invP.m, 64);
+    std::memcpy(out->shadow2_inv_proj, s2ip.m, 64);
+    std::memcpy(out->shadow2_inv_t, c2s2_it.m, 64);
+    std::memcpy(out->world_inv_t, c2w_it.m, 64);
+}
+
+void orc_mat4_inverse(const float* m, float* out) {
+    const M4 r = minv(m4_in(m));
+    std::memcpy(out, r.m, 64);
+}
+
+float orc_light_space_dot(const float* xyw, const float* wg, const orc_shadow_params* sp, float* n3) {
+    return light_space_dot(xyw, wg[0], wg[1], sp, n3);
+}
+
+// FilterTask::process2 (Reconstruction.cpp:722-833): the camera's output
+// samples, each one's shadow "ray" into the light's buffer, the shadow query
+// there, the Lambert term (light-space dot with the normal from the
+// w-gradients) and the ambient term; per pixel, red where a sample finds no
+// surface.  uvoffset takes the generator's first and second draws as x, y.
+int orc_reconstruct_dof_motion_shadows(orc_ctx* cam, orc_ctx* light, const orc_recon_params* p,
+                                       const orc_shadow_params* sp, float* image) {
+    if (!cam->built || !light->built) return 1;
+    if (cam->W != light->W || cam->H != light->H || cam->W != sp->width || cam->H != sp->height) return 2;
+    std::vector<Tmpl> pat;
+    const int M = make_pattern(p->out_spp, p->override_uvt, p->uvt, pat);
+    const int W = cam->W, H = cam->H;
+    const int lw = light->W, lh = light->H;
+    const int threads = hw_threads(p->threads);
+    std::vector<StatsAcc> st(threads);
+    std::vector<Filt> fc(threads), fli(threads);
+    for (auto& f : fc) f.c = cam;
+    for (auto& f : fli) f.c = light;
+    const M4 c2l = m4_in(sp->c2l), w_it = m4_in(sp->world_inv_t);
+    const float LIGHTING_SCALE = 1.5f, AMBIENT_SCALE = 0.5f;   // Reconstruction.cpp:34-35
+    parallel_for(H, threads, [&](int w, int y) {
+        for (int x = 0; x < W; x++) {
+            const uint32_t h = hash_bits((uint32_t)x, (uint32_t)y);
+            const int pat_i = (int)(h % 64u);
+            Ranrot rnd(h);
+            const float uo0 = rnd.f32();
+            const float uo1 = rnd.f32();
+            const float camFarW = sp->camera_far - 1e-2f;
+            float pc[4] = {0, 0, 0, 0};
+            for (int i = 0; i < M; i++) {
+                st[w].count += 1;
+                const Tmpl& tp = pat[(size_t)pat_i * M + i];
+                OSample cs;
+                cs.x = tp.x + (float)x;
+                cs.y = tp.y + (float)y;
+                cs.u = tp.u; cs.v = tp.v; cs.t = tp.t;
+                Filt::Ext e;
+                Filt::Result r = fc[w].reconstruct(cs, p, st[w], false, 1.0f, &e);
+                float cx = e.x, cy = e.y;
+                const float cw_ = e.w;
+                const float cc[4] = {r.color[0], r.color[1], r.color[2], r.color[3]};
+                if (cc[0] == 0 && cc[1] == 0 && cc[2] == 0 && cc[3] == 0) {
+                    pc[0] = 1; pc[1] = 0; pc[2] = 0; pc[3] = 1;
+                    break;
+                }
+                float col[4] = {cc[0], cc[1], cc[2], cc[3]};
+                if (cw_ <= camFarW) {
+                    // reprojectToUVCenter: Reconstruction.hpp:87-91
+                    const float coc = sp->camera_coc[0] / cw_ + sp->camera_coc[1];
+                    cx = cx - coc * cs.u;
+                    cy = cy - coc * cs.v;
+                    const float v4[4] = {cx * cw_, cy * cw_, 1.0f, cw_};
+                    float lp[4];
+                    mvec(c2l, v4, lp);
+                    const float lc = rcpf(lp[3]);
+                    const float lx = lp[0] * lc, ly = lp[1] * lc, lpw = lp[3];
+                    if (lx >= 0 && ly >= 0 && lx <= (float)lw && ly <= (float)lh && lpw > 0) {
+                        OSample ls;
+                        ls.x = lx; ls.y = ly; ls.w = lpw; ls.t = cs.t;
+                        float uu = sobol(3, i + 1) + uo0, vv = sobol(4, i + 1) + uo1;
+                        if (uu >= 1.f) uu -= 1.f;
+                        if (vv >= 1.f) vv -= 1.f;
+                        ls.u = 2.f * uu - 1.f;
+                        ls.v = 2.f * vv - 1.f;
+                        const float lcoc = sp->light_coc[0] / ls.w + sp->light_coc[1];
+                        ls.x += ls.u * lcoc;
+                        ls.y += ls.v * lcoc;
+                        Filt::Result sr = fli[w].reconstruct(ls, p, st[w], true, e.density, nullptr);
+                        for (int k = 0; k < 4; k++) col[k] = sr.color[k];
+                        float n[3];
+                        const float ldotn = light_space_dot(v4, e.wgx, e.wgy, sp, n);
+                        float n4[4] = {n[0], n[1], n[2], 0.0f}, wn[4];
+                        mvec(w_it, n4, wn);
+                        const float ambient = wn[1] * 0.5f + 0.5f;
+                        const float cosine = std::fmax(ldotn, 0.0f);
+                        const float sc[4] = {LIGHTING_SCALE * cosine, LIGHTING_SCALE * cosine,
+                                             LIGHTING_SCALE * cosine, LIGHTING_SCALE * 1.0f};
+                        for (int k = 0; k < 4; k++) col[k] *= sc[k];
+                        const float a = ambient * AMBIENT_SCALE;
+                        for (int k = 0; k < 4; k++) col[k] += a * cc[k];
+                        col[3] = 1;
+                    }
+                }
+                for (int k = 0; k < 4; k++) pc[k] += col[k];
+            }
+            if (pc[3] == 0) { pc[0] = 1; pc[1] = 0; pc[2] = 0; pc[3] = 1; }
+            const float rw = rcpf(pc[3]);
+            for (int k = 0; k < 4; k++) image[4 * ((size_t)y * W + x) + k] = pc[k] * rw;
+        }
+    });
+    return 0;
 }
 
 }  // extern "C"

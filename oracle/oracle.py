@@ -46,6 +46,13 @@ class Stats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class ShadowParams(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("camera_coc", C.c_float * 2),
+                ("light_coc", C.c_float * 2), ("camera_far", C.c_float)] + [
+        (n, C.c_float * 16) for n in ("c2l", "zfromw_ndc", "inv_projection", "shadow2_inv_proj",
+                                      "shadow2_inv_t", "world_inv_t")]
+
+
 class QueryInfo(C.Structure):
     _fields_ = [
         ("fetch2r", C.c_int), ("num_surfaces_1r", C.c_int), ("num_samples_1r", C.c_int),
@@ -124,6 +131,16 @@ def lib():
         L.orc_triangle_covers.argtypes = [fp, fp, fp, fp]
         L.orc_same_surface.argtypes = [fp, fp, fp]
         L.orc_surface_scan.argtypes = [C.c_int, fp, fp, fp, fp, fp, ip]
+        L.orc_set_density.argtypes = [P, C.c_void_p]
+        L.orc_reconstruct_queries.argtypes = [P, C.POINTER(ReconParams), fp, C.c_int, C.c_int, C.c_void_p,
+                                              C.c_void_p, C.POINTER(Stats)]
+        L.orc_shadow_setup.argtypes = [fp, fp, fp, fp, C.c_int, C.c_int, fp, fp, C.c_float,
+                                       C.POINTER(ShadowParams)]
+        L.orc_mat4_inverse.argtypes = [fp, fp]
+        L.orc_light_space_dot.restype = C.c_float
+        L.orc_light_space_dot.argtypes = [fp, fp, C.POINTER(ShadowParams), fp]
+        L.orc_reconstruct_dof_motion_shadows.argtypes = [P, P, C.POINTER(ReconParams), C.POINTER(ShadowParams),
+                                                         C.c_void_p]
         _lib = L
     return _lib
 
@@ -234,6 +251,36 @@ class Oracle:
             return img, q, st.as_dict(), dbg
         return img, q, st.as_dict()
 
+    def set_density(self, density):
+        """Per-sample LIGHTSPACE_DENSITY (input order) or None for 1; rebuild after."""
+        d = None if density is None else np.ascontiguousarray(density, np.float32).reshape(-1)
+        assert d is None or d.size == self.W * self.H * self.spp
+        lib().orc_set_density(self.h, _ptr(d))
+
+    def reconstruct_queries(self, queries, shadow=False, **kw):
+        """Explicit output samples (n x 8: x, y, u, v, t, w, density, 0):
+        (colours n x 4, extended results n x 8, stats)."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 8)
+        p = self.params(**kw)
+        rgba = np.zeros((q.shape[0], 4), np.float32)
+        ext = np.zeros((q.shape[0], 8), np.float32)
+        st = Stats()
+        rc = lib().orc_reconstruct_queries(self.h, C.byref(p), q, q.shape[0], 1 if shadow else 0, _ptr(rgba),
+                                           _ptr(ext), C.byref(st))
+        if rc != 0:
+            raise RuntimeError("oracle queries failed (%d)" % rc)
+        return rgba, ext, st.as_dict()
+
+    def reconstruct_dof_motion_shadows(self, light, sp, out_spp=16, **kw):
+        """FilterTask::process2 over the image with `light` (another built
+        Oracle of the same size) as the shadow TreeGather."""
+        p = self.params(out_spp=out_spp, **kw)
+        img = np.zeros((self.H, self.W, 4), np.float32)
+        rc = lib().orc_reconstruct_dof_motion_shadows(self.h, light.h, C.byref(p), C.byref(sp), _ptr(img))
+        if rc != 0:
+            raise RuntimeError("oracle shadows failed (%d)" % rc)
+        return img
+
     def reconstruct_refgpu(self, out_spp=16, want_queries=False, **kw):
         """The reference's GPU-variant method (recKernel / failKernel /
         nearestKernel): image, per-output-sample colours and stages (0: 1R,
@@ -328,3 +375,28 @@ def output_pattern(out_spp, override_uvt=None):
     out = np.zeros((64 * M, 6), np.float32)
     lib().orc_output_pattern(out_spp, 1 if override_uvt is not None else 0, uvt, _ptr(out))
     return out
+
+
+def _m16(m):
+    return np.ascontiguousarray(m, np.float32).reshape(16)
+
+
+def shadow_setup(cam_proj, cam_to_world, light_proj, light_world_to_cam, w, h, cam_coc, light_coc, cam_far):
+    """ShadowParams from the cameras' row-major 4x4 matrices."""
+    sp = ShadowParams()
+    lib().orc_shadow_setup(_m16(cam_proj), _m16(cam_to_world), _m16(light_proj), _m16(light_world_to_cam), w, h,
+                           np.asarray(cam_coc, np.float32), np.asarray(light_coc, np.float32), cam_far,
+                           C.byref(sp))
+    return sp
+
+
+def mat4_inverse(m):
+    out = np.zeros(16, np.float32)
+    lib().orc_mat4_inverse(_m16(m), out)
+    return out.reshape(4, 4)
+
+
+def light_space_dot(xyw, wg, sp):
+    n = np.zeros(3, np.float32)
+    d = lib().orc_light_space_dot(np.asarray(xyw, np.float32), np.asarray(wg, np.float32), C.byref(sp), n)
+    return d, n

@@ -188,6 +188,49 @@ int      orc_same_surface(const float* n0, const float* n1, const float* o5);
 int orc_surface_scan(int n, const float* t, const float* ba, const float* bb, const float* ba1, const float* bb1,
                      int32_t* surf);
 
+/* ---- Shadows (TreeGather::reconstructShadows / reconstructDofMotionShadows,
+ * Reconstruction.cpp:302-433; the filter's shadow test
+ * ReconstructionOur.cpp:78-80, 283-291, 325-347) ---- */
+
+/* Matrices and camera data of reconstructDofMotionShadows, row-major
+ * (m[4*i+j] = row i, column j). */
+typedef struct orc_shadow_params {
+    int   width, height;          /* camera buffer size (= the light's)            */
+    float camera_coc[2];          /* the camera's CameraParams::getCocCoeffs()     */
+    float light_coc[2];           /* the light's                                   */
+    float camera_far;             /* the camera's far plane                        */
+    float c2l[16];                /* ws * cameraToShadow * invws * cameraProjectedZfromW */
+    float zfromw_ndc[16];         /* invws * cameraProjectedZfromW                 */
+    float inv_projection[16];     /* invCameraProjection                           */
+    float shadow2_inv_proj[16];   /* cameraToShadow2 * invCameraProjection         */
+    float shadow2_inv_t[16];      /* cameraToShadow2.inverted().transposed()       */
+    float world_inv_t[16];        /* cameraToWorld.inverted().transposed()         */
+} orc_shadow_params;
+
+/* Per-sample LIGHTSPACE_DENSITY channel, input order (N floats; NULL: 1).
+ * The w-gradients are the records' dwdx, dwdy (floats 14, 15).  Rebuild after. */
+void  orc_set_density(orc_ctx* c, const float* density);
+/* Explicit output samples: q8 = n x (x, y, u, v, t, w, density, 0).
+ * shadow != 0: Filterer::reconstruct with reconstructShadow (w and density
+ * are the query's own).  rgba: n x 4; ext8 (optional): n x (xy of the
+ * nearest gathered sample at the output's (u,v,t), its w there, its density,
+ * its w-gradient, 0, 0) -- the Result's xy, w, density, wg. */
+int   orc_reconstruct_queries(orc_ctx* c, const orc_recon_params* p, const float* q8, int n, int shadow,
+                              float* rgba, float* ext8, orc_stats* stats);
+/* The matrices from the cameras (Reconstruction.cpp:365-388): projections,
+ * the camera's camera-to-world and the light's world-to-camera, all
+ * row-major, with FW's float arithmetic. */
+void  orc_shadow_setup(const float* cam_proj, const float* cam_to_world, const float* light_proj,
+                       const float* light_world_to_cam, int w, int h, const float* cam_coc,
+                       const float* light_coc, float cam_far, orc_shadow_params* out);
+/* FW's Mat4f::inverted (Math.hpp:955-976). */
+void  orc_mat4_inverse(const float* m16, float* out16);
+/* lightSpaceDot (common/Util.cpp:202-224); n3 receives the view-space normal. */
+float orc_light_space_dot(const float* xyw4, const float* wg2, const orc_shadow_params* sp, float* n3);
+/* FilterTask::process2 over the whole image (cam and light built, same size). */
+int   orc_reconstruct_dof_motion_shadows(orc_ctx* cam, orc_ctx* light, const orc_recon_params* p,
+                                         const orc_shadow_params* sp, float* image);
+
 #ifdef __cplusplus
 }
 #endif

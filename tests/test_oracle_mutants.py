@@ -1,11 +1,13 @@
-"""CPU: every pinned oracle rule is load-bearing.  The oracle is rebuilt ten
-times with one rule broken each (oracle.cpp's ORC_MUTANT switches: sameSurface
-without its time test, ternaryEqual needing equal signs, the leaf scan
+"""CPU: every pinned oracle rule is load-bearing.  The oracle is rebuilt
+thirteen times with one rule broken each (oracle.cpp's ORC_MUTANT switches:
+sameSurface without its time test, ternaryEqual needing equal signs, the leaf scan
 without the cocboundst0[0] typo or ignoring time changes, fetch2R ignoring the
 quadrant mask, no tiny-surface merge, emergency passes keeping the
-dispersion, no nearest-sample fallback, output times without the jitter, and
-the GPU-variant 2R surface split at 3 points), and the pin tests must fail on
-each, in the named test; on the unbroken oracle they pass."""
+dispersion, no nearest-sample fallback, output times without the jitter,
+the GPU-variant 2R surface split at 3 points, shadow queries without the
+density scaling, the shadow bias added instead of subtracted, and the shadow
+fallback without its light test), and the pin tests must fail on each, in the
+named test; on the unbroken oracle they pass."""
 import os
 import subprocess
 import sys
@@ -15,6 +17,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
 PINS = os.path.join(ROOT, "tests", "test_oracle_pins.py")
+SHADOW_PINS = os.path.join(ROOT, "tests", "test_oracle_shadow.py")
 
 # mutant -> a pin test that must catch it
 CATCHES = {
@@ -28,6 +31,9 @@ CATCHES = {
     8: "test_emergency_pass_gives_nearest_sample",
     9: "test_output_pattern_composition",
     10: "test_refgpu_2r_surface_needs_four_points",
+    11: "test_density_scales_the_dispersion",
+    12: "test_bias_keeps_a_query_just_behind_lit",
+    13: "test_nearest_fallback_takes_the_light_test",
 }
 
 
@@ -39,7 +45,7 @@ def mutants():
 
 def _run_pins(lib, select=None):
     env = dict(os.environ, TLR_ORACLE_LIB=lib)
-    cmd = [sys.executable, "-m", "pytest", PINS, "-q", "-p", "no:cacheprovider"]
+    cmd = [sys.executable, "-m", "pytest", PINS, SHADOW_PINS, "-q", "-p", "no:cacheprovider"]
     if select:
         cmd += ["-k", select]
     return subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=300)
