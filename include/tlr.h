@@ -162,6 +162,71 @@ tlr_status  tlr_reconstruct(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba
  * overlap in time on one GPU, from any context or thread, race. */
 tlr_status  tlr_reconstruct_noalloc(tlr_ctx* ctx, const tlr_recon_params* p, float* rgba_out, tlr_stats* stats);
 
+/* ---- Shadows --------------------------------------------------------------
+ * TreeGather::reconstructShadows and reconstructDofMotionShadows
+ * (src/reconstruction_lib/reconstruction/Reconstruction.cpp:302-433, declared
+ * Reconstruction.hpp:47-48), and the filter's shadow mode
+ * (ReconstructionOur.cpp:78-80, 283-291, 325-347). */
+
+/* Per-sample inputs of shadow reconstruction, input order, n = the buffer's
+ * sample count: wg = n x (dw/dx, dw/dy) (UVTSampleBuffer::getSampleWG,
+ * SampleBuffer.hpp:156; the records' floats 14, 15), density = n floats (the
+ * LIGHTSPACE_DENSITY channel, Reconstruction.cpp:547).  NULL: 0 and 1, the
+ * reference's defaults.  Set after tlr_set_samples*; new samples reset them. */
+tlr_status  tlr_set_shadow_inputs(tlr_ctx* ctx, const float* wg, const float* density, int64_t n);
+
+/* Explicit output samples through the filter (FilterTask::process with a
+ * query buffer, Reconstruction.cpp:648-719, as reconstructShadows drives it):
+ * queries = n x (x, y, u, v, t, w, density, 0), host memory.  shadow != 0:
+ * Filterer::reconstruct with reconstructShadow -- the dispersion scaled by
+ * sqrt(density), a gathered sample's colour counted only when it lights the
+ * query (o.w - 0.01 <= its w at t + dot(o.xy - its xy, its w-gradient)),
+ * black with its alpha otherwise, no lens/time weighting.  shadow == 0: the
+ * plain filter (p's lens/time filters; w and density unused).  rgba_out: n x
+ * 4 floats; ext_out (optional): n x 8 floats (xy of the nearest gathered
+ * sample at the output's (u,v,t), its w there, its density, its w-gradient,
+ * 0, 0): Filterer::Result's xy, w, density, wg.  p may be NULL. */
+tlr_status  tlr_reconstruct_queries(tlr_ctx* ctx, const tlr_recon_params* p, const float* queries, int64_t n,
+                                    int shadow, float* rgba_out, float* ext_out, tlr_stats* stats);
+
+/* Matrices and camera data of reconstructDofMotionShadows, row-major
+ * (m[4*i+j] = row i, column j). */
+typedef struct tlr_shadow_params {
+    int   width, height;          /* camera buffer size (= the light's)                 */
+    float camera_coc[2];          /* the camera's CameraParams::getCocCoeffs()          */
+    float light_coc[2];           /* the light's                                        */
+    float camera_far;             /* the camera's far plane (camera.getFar())           */
+    float c2l[16];                /* ws * cameraToShadow * invws * cameraProjectedZfromW */
+    float zfromw_ndc[16];         /* invws * cameraProjectedZfromW                      */
+    float inv_projection[16];     /* invCameraProjection                                */
+    float shadow2_inv_proj[16];   /* cameraToShadow2 * invCameraProjection              */
+    float shadow2_inv_t[16];      /* cameraToShadow2.inverted().transposed()            */
+    float world_inv_t[16];        /* cameraToWorld.inverted().transposed()              */
+} tlr_shadow_params;
+
+/* The matrices of Reconstruction.cpp:365-388 from the two cameras, with FW's
+ * float arithmetic (Mat4f products, inverted(), transposed()): the camera's
+ * projection and camera-to-world, the light's projection and world-to-camera,
+ * all row-major.  Host only (no GPU needed). */
+tlr_status  tlr_shadow_setup(const float* cam_proj, const float* cam_to_world, const float* light_proj,
+                             const float* light_world_to_cam, int width, int height, const float* cam_coc,
+                             const float* light_coc, float cam_far, tlr_shadow_params* out);
+
+/* TreeGather::reconstructDofMotionShadows (FilterTask::process2,
+ * Reconstruction.cpp:722-833): camera and light are built contexts of the
+ * same size on one GPU; p: out_spp, override_uvt and the lens/time filters of
+ * the camera pass (gate, path and debug_image are ignored).  Each output
+ * sample's result is taken back to the (u,v) centre, mapped into the light's
+ * window (c2l), queried there in shadow mode at a position on the square
+ * light (Sobol dimensions 3 and 4 plus a per-pixel offset: the first and
+ * second draws of Random(hashBits(x, y)) as x and y), and shaded: 1.5 x the
+ * light-space cosine of the normal from its w-gradients, plus 0.5 x the
+ * ambient term from the world-space normal; a sample with no surface makes
+ * its pixel red. */
+tlr_status  tlr_reconstruct_dof_motion_shadows(tlr_ctx* camera, tlr_ctx* light, const tlr_recon_params* p,
+                                               const tlr_shadow_params* sp, float* rgba_out, int out_on_device,
+                                               tlr_stats* stats);
+
 /* Blocks until the context's stream is idle. */
 tlr_status  tlr_sync(tlr_ctx* ctx);
 

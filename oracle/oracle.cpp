@@ -225,21 +225,16 @@ struct Ranrot {
     float f32() { return (float)next() * (1.0f / 4294967296.0f); }
 };
 
-// Sobol direction numbers of the 4th dimension (index 3), generated from the
-// primitive polynomial x^3+x+1 with initial m = 1,3,1 (the standard
-// Joe-Kuo entry the reference tabulates in common/Sobol5.inl:97-128).
-// Dimension 0 is the van der Corput sequence (v_c = 2^(31-c)).
+// Sobol direction numbers generated from Joe-Kuo entries (primitive
+// polynomial degree s, coefficients a, initial m): index 3 = x^3+x+1 with
+// m = 1,3,1 and index 4 = x^3+x^2+1 with m = 1,1,1, the rows the reference
+// tabulates in common/Sobol5.inl:97-128 and :129-160 (the output pattern uses
+// index 3, the shadow path's light positions indices 3 and 4).  Dimension 0
+// is the van der Corput sequence (v_c = 2^(31-c)); indices 1, 2 are unused
+// and left zero.
 struct SobolTable {
     uint32_t v[5][32];
-    SobolTable() {
-        for (int c = 0; c < 32; c++) v[0][c] = 1u << (31 - c);
-        // Only dimension 3 is used by the output pattern; dims 1,2,4 are not
-        // needed and left zero.
-        for (int d = 1; d < 5; d++) for (int c = 0; c < 32; c++) v[d][c] = 0;
-        const int s = 3;
-        const uint32_t a = 1;
-        const uint32_t m[3] = {1, 3, 1};
-        uint32_t* V = v[3];
+    static void gen(uint32_t* V, int s, uint32_t a, const uint32_t* m) {
         for (int c = 0; c < s; c++) V[c] = m[c] << (31 - c);
         for (int c = s; c < 32; c++) {
             uint32_t x = V[c - s] ^ (V[c - s] >> s);
@@ -247,6 +242,13 @@ struct SobolTable {
                 if ((a >> (s - 1 - k)) & 1) x ^= V[c - k];
             V[c] = x;
         }
+    }
+    SobolTable() {
+        for (int c = 0; c < 32; c++) v[0][c] = 1u << (31 - c);
+        for (int d = 1; d < 5; d++) for (int c = 0; c < 32; c++) v[d][c] = 0;
+        const uint32_t m3[3] = {1, 3, 1}, m4[3] = {1, 1, 1};
+        gen(v[3], 3, 1, m3);
+        gen(v[4], 3, 2, m4);
     }
 };
 const SobolTable& sobol_table() { static SobolTable t; return t; }
@@ -2561,7 +2563,8 @@ int orc_reconstruct_dof_motion_shadows(orc_ctx* cam, orc_ctx* light, const orc_r
                         float n4[4] = {n[0], n[1], n[2], 0.0f}, wn[4];
                         mvec(w_it, n4, wn);
                         const float ambient = wn[1] * 0.5f + 0.5f;
-                        const float cosine = std::fmax(ldotn, 0.0f);
+                        // max(ldotn, 0) (fmaxf) with +0 for -0 and NaN
+                        const float cosine = ldotn > 0.0f ? ldotn : 0.0f;
                         const float sc[4] = {LIGHTING_SCALE * cosine, LIGHTING_SCALE * cosine,
                                              LIGHTING_SCALE * cosine, LIGHTING_SCALE * 1.0f};
                         for (int k = 0; k < 4; k++) col[k] *= sc[k];

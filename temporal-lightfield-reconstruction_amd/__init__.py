@@ -36,6 +36,7 @@ EXPORTED = [
     "tlr_gate_grad", "tlr_gate_collect", "tlr_gate_append_training", "tlr_gate_clear_training",
     "tlr_gate_training_size", "tlr_gate_train",
     "tlr_batch_set_samples", "tlr_reconstruct_batch", "tlr_batch_retune", "tlr_batch_set_mode", "tlr_use_slot",
+    "tlr_set_shadow_inputs", "tlr_reconstruct_queries", "tlr_shadow_setup", "tlr_reconstruct_dof_motion_shadows",
 ]
 GATE_FEATURES = 24
 GATE_PARAMS = 5825
@@ -91,6 +92,15 @@ class BatchStats(C.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ShadowParams(C.Structure):
+    """tlr_shadow_params: matrices (row-major 4x4) and camera data of
+    reconstructDofMotionShadows."""
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("camera_coc", C.c_float * 2),
+                ("light_coc", C.c_float * 2), ("camera_far", C.c_float)] + [
+        (n, C.c_float * 16) for n in ("c2l", "zfromw_ndc", "inv_projection", "shadow2_inv_proj",
+                                      "shadow2_inv_t", "world_inv_t")]
 
 
 class BuildInfo(C.Structure):
@@ -174,6 +184,12 @@ def lib():
         L.tlr_gate_training_size.restype = C.c_int64
         L.tlr_gate_training_size.argtypes = [V]
         L.tlr_gate_train.argtypes = [V, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, V]
+        L.tlr_set_shadow_inputs.argtypes = [V, V, V, C.c_int64]
+        L.tlr_reconstruct_queries.argtypes = [V, C.POINTER(ReconParams), V, C.c_int64, C.c_int, V, V,
+                                              C.POINTER(Stats)]
+        L.tlr_shadow_setup.argtypes = [V, V, V, V, C.c_int, C.c_int, V, V, C.c_float, C.POINTER(ShadowParams)]
+        L.tlr_reconstruct_dof_motion_shadows.argtypes = [V, V, C.POINTER(ReconParams), C.POINTER(ShadowParams), V,
+                                                         C.c_int, C.POINTER(Stats)]
         _tlr = L
     return _tlr
 
@@ -207,6 +223,21 @@ def make_scene(preset, width, height, spp, seed, threads=0, **overrides):
     if scene_lib().tlr_scene_generate(C.byref(d), _p(rec)) != 0:
         raise RuntimeError("scene generation failed")
     return UVTSampleBuffer(width, height, spp, d.coc_c0, d.coc_c1, rec)
+
+
+def shadow_setup(cam_proj, cam_to_world, light_proj, light_world_to_cam, width, height, cam_coc, light_coc,
+                 cam_far):
+    """tlr_shadow_setup: ShadowParams from the cameras' row-major 4x4 matrices."""
+    m = [np.ascontiguousarray(a, np.float32).reshape(16) for a in (cam_proj, cam_to_world, light_proj,
+                                                                   light_world_to_cam)]
+    cc = np.asarray(cam_coc, np.float32)
+    lc = np.asarray(light_coc, np.float32)
+    sp = ShadowParams()
+    rc = lib().tlr_shadow_setup(_p(m[0]), _p(m[1]), _p(m[2]), _p(m[3]), width, height, _p(cc), _p(lc),
+                                float(cam_far), C.byref(sp))
+    if rc != 0:
+        raise TlrError(rc, "tlr_shadow_setup")
+    return sp
 
 
 def output_pattern(out_spp, override_uvt=None):
@@ -392,6 +423,46 @@ class Ctx:
             img = out if out is not None else np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
         st = Stats()
         self._chk(lib().tlr_reconstruct(self.h, C.byref(p), _p(img) if img is not None else None, 0, C.byref(st)))
+        return img, st.as_dict()
+
+    def set_shadow_inputs(self, wg=None, density=None):
+        """Per input sample w-gradients ((N, 2), the records' floats 14-15)
+        and light-space densities (N) for shadow reconstruction; None: 0, 1."""
+        n = self.sbuf.records.shape[0]
+        wg = None if wg is None else np.ascontiguousarray(wg, np.float32).reshape(n, 2)
+        density = None if density is None else np.ascontiguousarray(density, np.float32).reshape(n)
+        self._chk(lib().tlr_set_shadow_inputs(self.h, _p(wg), _p(density), n))
+
+    def reconstruct_queries(self, queries, shadow=False, lens_gaussian=False, time_gaussian=False):
+        """Explicit output samples (n x 8: x, y, u, v, t, w, density, 0):
+        (colours n x 4, extended results n x 8, stats)."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 8)
+        p = ReconParams()
+        p.lens_gaussian = int(lens_gaussian)
+        p.time_gaussian = int(time_gaussian)
+        rgba = np.empty((q.shape[0], 4), np.float32)
+        ext = np.empty((q.shape[0], 8), np.float32)
+        st = Stats()
+        self._chk(lib().tlr_reconstruct_queries(self.h, C.byref(p), _p(q), q.shape[0], 1 if shadow else 0,
+                                                _p(rgba), _p(ext), C.byref(st)))
+        return rgba, ext, st.as_dict()
+
+    def reconstruct_dof_motion_shadows(self, light, sp, out_spp=16, override_uvt=None, lens_gaussian=False,
+                                       time_gaussian=False):
+        """reconstructDofMotionShadows with `light` (a built Ctx of the same
+        size) as the shadow TreeGather: (image, stats)."""
+        p = ReconParams()
+        p.out_spp = out_spp
+        p.override_uvt = 1 if override_uvt is not None else 0
+        if override_uvt is not None:
+            for k in range(3):
+                p.uvt[k] = override_uvt[k]
+        p.lens_gaussian = int(lens_gaussian)
+        p.time_gaussian = int(time_gaussian)
+        img = np.empty((self.sbuf.height, self.sbuf.width, 4), np.float32)
+        st = Stats()
+        self._chk(lib().tlr_reconstruct_dof_motion_shadows(self.h, light.h, C.byref(p), C.byref(sp), _p(img), 0,
+                                                           C.byref(st)))
         return img, st.as_dict()
 
     def reconstruct_noalloc(self, device_out, out_spp=16, override_uvt=None, lens_gaussian=False,

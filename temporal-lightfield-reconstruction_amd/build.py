@@ -17,8 +17,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib")
 
-HIP_SOURCES = ["prims.hip", "build.hip", "query.hip", "refgpu.hip", "gate.hip", "noalloc.hip", "tlr.cpp", "pattern.cpp",
-               "sample_io.cpp"]
+HIP_SOURCES = ["prims.hip", "build.hip", "query.hip", "refgpu.hip", "gate.hip", "noalloc.hip", "shadow.hip", "tlr.cpp",
+               "pattern.cpp", "sample_io.cpp", "shadow_setup.cpp"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math",

@@ -3,6 +3,7 @@
 // (Reconstruction.cpp:452-512) with the output spp a parameter instead of the
 // hard-wired 128.  Tiny (64*M entries) and computed once per reconstruction
 // setting, so it stays on the host and is uploaded.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -59,6 +60,26 @@ struct SobolDim3 {
 
 float sobol_dim3(int i) {
     static const SobolDim3 tab;
+    uint32_t r = 0;
+    for (int c = 0; i; i >>= 1, c++)
+        if (i & 1) r ^= tab.v[c];
+    return (float)r * (1.f / 4294967296.0f);
+}
+
+// Direction numbers of Sobol dimension 4 (primitive polynomial x^3+x^2+1,
+// m = 1,1,1), the table row common/Sobol5.inl:129-160 holds.
+struct SobolDim4 {
+    uint32_t v[32];
+    SobolDim4() {
+        v[0] = 1u << 31;
+        v[1] = 1u << 30;
+        v[2] = 1u << 29;
+        for (int c = 3; c < 32; c++) v[c] = v[c - 1] ^ v[c - 3] ^ (v[c - 3] >> 3);
+    }
+};
+
+float sobol_dim4(int i) {
+    static const SobolDim4 tab;
     uint32_t r = 0;
     for (int c = 0; i; i >>= 1, c++)
         if (i & 1) r ^= tab.v[c];
@@ -239,6 +260,14 @@ int make_output_pattern(int out_spp, bool override_uvt, const float* uvt, std::v
         out5[5 * k + 4] = all[k].t;
     }
     return M;
+}
+
+void light_sample_positions(int M, std::vector<float>& out2) {
+    out2.resize(2 * (size_t)std::max(M, 1));
+    for (int i = 0; i < M; i++) {
+        out2[2 * i] = sobol_dim3(i + 1);
+        out2[2 * i + 1] = sobol_dim4(i + 1);
+    }
 }
 
 }  // namespace tlr

@@ -481,26 +481,58 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
 // (Filterer::reconstruct, ReconstructionOur.cpp:50-358), wave-uniform.
 // DEBUG also writes the debug-image values (a separate instantiation: the
 // extra live values cost the plain one registers).
-template <bool DEBUG>
+// w of tree sample k at output time t: the key collectInputSamples2 gives a
+// gathered sample (ReconstructionOur.cpp:438-451)
+__device__ __forceinline__ float sample_w_at(const Points& P, int k, float t) {
+    const float4 a = pt_a(P, k), b = pt_b(P, k);
+    if (b.x != 0.0f || b.y != 0.0f || b.z != 0.0f) return 1.0f * a.z + (t - a.w) * b.z;
+    return a.z;
+}
+
+// Does a gathered sample at offset (dx, dy) = o.xy - its xy, with w = key at
+// the output's t and w-gradient (wgx, wgy), light a shadow query at w = ow?
+// ReconstructionOur.cpp:289 (SHADOW_BIAS 0.01, :46); FW's dot sums from zero.
+__device__ __forceinline__ bool shadow_lit(float ow, float dx, float dy, float key, float wgx, float wgy) {
+    float d = 0.0f;
+    d += dx * wgx;
+    d += dy * wgy;
+    return ow - 0.01f <= key + d;
+}
+
+template <bool DEBUG, int MODE = 0>
 __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMem& m, unsigned qi, bool skip1r,
                                               unsigned long long* st) {
     const int lane = threadIdx.x & 63;
-    const int M = Q.M;
-    const unsigned pix = qi / (unsigned)M;
-    const int i = (int)(qi - pix * (unsigned)M);
-    const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
-    const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
-    const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
     OSample o;
-    o.x = tp[0] + (float)px;
-    o.y = tp[1] + (float)py;
-    o.u = tp[2];
-    o.v = tp[3];
-    o.t = tp[4];
+    float ow = 0.0f, density = 1.0f;
+    if (MODE != 0 && Q.qin) {
+        const float4 a = Q.qin[2 * (size_t)qi], b = Q.qin[2 * (size_t)qi + 1];
+        o.x = unif(a.x);
+        o.y = unif(a.y);
+        o.u = unif(a.z);
+        o.v = unif(a.w);
+        o.t = unif(b.x);
+        ow = unif(b.y);
+        density = unif(b.z);
+    } else {
+        const int M = Q.M;
+        const unsigned pix = qi / (unsigned)M;
+        const int i = (int)(qi - pix * (unsigned)M);
+        const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
+        const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
+        const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
+        o.x = tp[0] + (float)px;
+        o.y = tp[1] + (float)py;
+        o.u = tp[2];
+        o.v = tp[3];
+        o.t = tp[4];
+    }
     st[ST_COUNT] += 1;
 
     // Filterer::reconstruct: ReconstructionOur.cpp:50-159
     float disp = dispersion_for_spp(Q.spp_pow2);
+    // non-uniform view sample density in light space: ReconstructionOur.cpp:78-80
+    if (MODE == 2) disp *= sqrtf(density);
     Gather g{0, 0, 0, 0, 0};
     int overflow = 0;
     int n1 = 0;
@@ -608,7 +640,16 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
                         const float rx = m.sx[fs + k], ry = m.sy[fs + k];
                         const float dx = o.x - rx, dy = o.y - ry;
                         wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
-                        if (wgt > 0) c = Q.pts.pc[m.sidx[fs + k]];
+                        if (wgt > 0) {
+                            const int sk = m.sidx[fs + k];
+                            c = Q.pts.pc[sk];
+                            if (MODE == 2) {
+                                // per-input-sample thresholding: ReconstructionOur.cpp:283-291
+                                const float4 e = Q.pext[sk];
+                                if (!shadow_lit(ow, dx, dy, sample_w_at(Q.pts, sk, o.t), e.x, e.y))
+                                    c.x = c.y = c.z = 0.0f;
+                            }
+                        }
                     }
                     const int cnt = min(64, n - c0);
                     for (int j = 0; j < cnt; j++) {
@@ -626,16 +667,38 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
                 nI1 = fmini;
             }
         }
+        if (MODE != 0 && lane == 0 && Q.qext) {
+            // Filterer::Result's xy, w, wg and density from the nearest
+            // sample: ReconstructionOur.cpp:325-331 (zero without one)
+            float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f), e1 = e0;
+            if (!overflow && nI0 != -1) {
+                const int fs = m.f_start[nI0];
+                const int sk = m.sidx[fs + nI1];
+                const float4 pe = Q.pext[sk];
+                e0 = make_float4(m.sx[fs + nI1], m.sy[fs + nI1], sample_w_at(Q.pts, sk, o.t), pe.z);
+                e1 = make_float4(pe.x, pe.y, 0.f, 0.f);
+            }
+            Q.qext[2 * (size_t)qi] = e0;
+            Q.qext[2 * (size_t)qi + 1] = e1;
+        }
         if (!overflow && nI0 != -1) {
             if (ca == 0) {
                 const int fs = uni(m.f_start[nI0]);
-                const float4 c = Q.pts.pc[uni(m.sidx[fs + nI1])];
+                const int sk = uni(m.sidx[fs + nI1]);
+                const float4 c = Q.pts.pc[sk];
                 cr = c.x;
                 cg = c.y;
                 cb = c.z;
                 ca = 1.0f;
+                if (MODE == 2) {
+                    // ReconstructionOur.cpp:341-347
+                    const float4 pe = Q.pext[sk];
+                    const float nx = unif(m.sx[fs + nI1]), ny = unif(m.sy[fs + nI1]);
+                    if (!shadow_lit(ow, o.x - nx, o.y - ny, sample_w_at(Q.pts, sk, o.t), pe.x, pe.y))
+                        cr = cg = cb = 0.0f;
+                }
             }
-            if (Q.lens_gauss) {
+            if (MODE != 2 && Q.lens_gauss) {
                 // Gaussian(x, 0.5, 0): Reconstruction.hpp:378
                 const float e = 2.718281828f;
                 const float gu = powf(e, -((o.u - 0) * (o.u - 0)) / (2 * (0.5f * 0.5f)));
@@ -643,7 +706,7 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
                 const float gg = gu * gv;
                 cr *= gg; cg *= gg; cb *= gg; ca *= gg;
             }
-            if (Q.time_gauss) {
+            if (MODE != 2 && Q.time_gauss) {
                 const float e = 2.718281828f;
                 const float gg = powf(e, -((o.t - 0.5f) * (o.t - 0.5f)) / (2 * (0.25f * 0.25f)));
                 cr *= gg; cg *= gg; cb *= gg; ca *= gg;
@@ -679,7 +742,7 @@ __device__ __forceinline__ void flush_stats(unsigned long long* dst, const unsig
 // 7 waves per SIMD (the register limit this sets spills a little; the
 // kernel waits on memory and gains more from resident waves).  Persistent:
 // waves take queries from an atomic counter until the list is done.
-template <bool DEBUG>
+template <bool DEBUG, int MODE>
 __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -701,7 +764,7 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
             skip1r = (qi & kNeeds2R) != 0;
             qi &= ~kNeeds2R;
         }
-        general_query<DEBUG>(Q, m, qi, skip1r, st);
+        general_query<DEBUG, MODE>(Q, m, qi, skip1r, st);
     }
     flush_stats(Q.stats, st, Q.collect_stats != 0);
 }
@@ -1426,7 +1489,7 @@ __global__ void k_take_rows(const uint32_t* __restrict__ src, uint32_t* __restri
 
 int general_waves_per_cu() {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general<false>, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general<false, 0>, kBlock, 0) != hipSuccess ||
         blocks < 1)
         blocks = 1;
     return blocks * kWavesPerBlock;
@@ -1435,10 +1498,14 @@ int general_waves_per_cu() {
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     const int blocks = p.num_waves / kWavesPerBlock;
-    if (p.qdebug)
-        k_query_general<true><<<blocks, kBlock, 0, s>>>(p);
+    if (p.mode == 1)
+        k_query_general<false, 1><<<blocks, kBlock, 0, s>>>(p);
+    else if (p.mode == 2)
+        k_query_general<false, 2><<<blocks, kBlock, 0, s>>>(p);
+    else if (p.qdebug)
+        k_query_general<true, 0><<<blocks, kBlock, 0, s>>>(p);
     else
-        k_query_general<false><<<blocks, kBlock, 0, s>>>(p);
+        k_query_general<false, 0><<<blocks, kBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 

@@ -86,6 +86,18 @@ struct QueryParams {
     // optional, with stats: per query (#surfaces of the last gather, #samples
     // within 1R), the values of the reference's debug image
     float2* qdebug;
+    // general kernel: 0 = the plain method; 1 = also the extended result of
+    // each query (qext); 2 = shadow queries (ReconstructionOur.cpp:78-80,
+    // 283-291, 336-347) with extended results.  qin (modes 1, 2; optional in
+    // 1): explicit queries, 2 float4 each: (x, y, u, v), (t, w, density, 0)
+    // in place of the output pattern.  pext: per tree sample (wgx, wgy,
+    // density, 0).  qext: 2 float4 per query: (x, y, w, density), (wgx, wgy,
+    // 0, 0) of the nearest gathered sample (Filterer::Result's xy, w,
+    // density, wg).
+    int mode;
+    const float4* qin;
+    const float4* pext;
+    float4* qext;
 };
 
 // Lanes the fast kernel is launched with over all queries of Q (not a list):

@@ -23,6 +23,7 @@
 #include "pattern.hpp"
 #include "prims.hpp"
 #include "query.hpp"
+#include "shadow.hpp"
 
 using namespace tlr;
 
@@ -142,6 +143,14 @@ struct tlr_ctx {
     float* gate_g = nullptr;
     float* gate_loss = nullptr;
     bool has_gate = false;
+    // shadow inputs: per input sample (wgx, wgy, density, 0) (null: 0, 0, 1),
+    // and the same per tree sample, gathered after each build on demand
+    float4* d_wgd = nullptr;
+    int wgd_n = 0;
+    float4* pext = nullptr;
+    size_t pext_cap = 0;
+    bool pext_valid = false;
+    DeviceArena sarena;             // explicit queries and shadow passes
     unsigned long long* d_stamps = nullptr;  // batch completion stamps (n + 1)
     int stamps_cap = 0;
     float gate_thr = 0.0f;
@@ -242,6 +251,8 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->d_gi) (void)hipFree(c->d_gi);
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_counter) (void)hipFree(c->d_counter);
+    if (c->d_wgd) (void)hipFree(c->d_wgd);
+    if (c->pext) (void)hipFree(c->pext);
     if (c->d_stamps) (void)hipFree(c->d_stamps);
     if (c->d_fail_count) (void)hipFree(c->d_fail_count);
     if (c->d_accept_count) (void)hipFree(c->d_accept_count);
@@ -305,6 +316,7 @@ tlr_status tlr_set_samples(tlr_ctx* c, const tlr_header* h, const float* rec, in
     c->N = N;
     c->has_samples = true;
     c->built = false;
+    c->wgd_n = 0;
     return TLR_OK;
 }
 
@@ -329,6 +341,7 @@ tlr_status tlr_set_samples_soa(tlr_ctx* c, const tlr_header* h, const tlr_soa* s
     c->N = N;
     c->has_samples = true;
     c->built = false;
+    c->wgd_n = 0;
     return TLR_OK;
 }
 
@@ -562,6 +575,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     HIPCHK(c, hipStreamSynchronize(s));
     HIPCHK(c, hipMemcpy(&c->gi, c->d_gi, sizeof(GridInfo), hipMemcpyDeviceToHost));
     c->built = true;
+    c->pext_valid = false;
     return TLR_OK;
 }
 
@@ -1104,6 +1118,260 @@ tlr_status tlr_copy_leaves(tlr_ctx* c, int32_t* s01, float* b14) {
 tlr_status tlr_copy_query_colors(tlr_ctx* c, float* out4) {
     if (!c || !c->qcol || !out4) return TLR_E_STATE;
     HIPCHK(c, hipMemcpy(out4, c->qcol, c->qcol_n * sizeof(float4), hipMemcpyDeviceToHost));
+    return TLR_OK;
+}
+
+// ---- Shadows ----------------------------------------------------------------
+
+tlr_status tlr_set_shadow_inputs(tlr_ctx* c, const float* wg, const float* density, int64_t n) {
+    if (!c) return TLR_E_INVALID;
+    if (!c->has_samples) return c->fail(TLR_E_STATE, "no samples");
+    if (n != c->N) return c->fail(TLR_E_INVALID, "shadow inputs for %lld samples, the buffer has %d", (long long)n, c->N);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->d_wgd) HIPCHK(c, hipFree(c->d_wgd));
+    c->d_wgd = nullptr;
+    c->wgd_n = 0;
+    std::vector<float4> tmp((size_t)std::max(n, (int64_t)1));
+    for (int64_t k = 0; k < n; k++)
+        tmp[k] = make_float4(wg ? wg[2 * k] : 0.0f, wg ? wg[2 * k + 1] : 0.0f, density ? density[k] : 1.0f, 0.0f);
+    HIPCHK(c, hipMalloc(&c->d_wgd, sizeof(float4) * tmp.size()));
+    HIPCHK(c, hipMemcpy(c->d_wgd, tmp.data(), sizeof(float4) * tmp.size(), hipMemcpyHostToDevice));
+    c->wgd_n = (int)n;
+    c->pext_valid = false;
+    return TLR_OK;
+}
+
+// Tree-order (wgx, wgy, density, 0) of the current build.
+static tlr_status ensure_pext(tlr_ctx* c) {
+    if (c->pext_valid) return TLR_OK;
+    const size_t need = (size_t)std::max(c->nb, 1);
+    if (c->pext_cap < need) {
+        if (c->pext) HIPCHK(c, hipFree(c->pext));
+        c->pext = nullptr;
+        c->pext_cap = 0;
+        HIPCHK(c, hipMalloc(&c->pext, sizeof(float4) * need));
+        c->pext_cap = need;
+    }
+    const float4* in = (c->wgd_n == c->N && c->d_wgd) ? c->d_wgd : nullptr;
+    HIPCHK(c, launch_gather_ext(reinterpret_cast<const int*>(c->cand), c->nb, in, c->pext, c->stream));
+    c->pext_valid = true;
+    return TLR_OK;
+}
+
+static void stats_from(const unsigned long long* hs, tlr_stats* stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->leaf_nodes = (double)hs[ST_LEAF_NODES];
+    stats->samples_in_leaf_nodes = (double)hs[ST_SAMPLES_IN_LEAVES];
+    stats->samples_within_1r = (double)hs[ST_WITHIN_1R];
+    stats->samples_fetched = (double)hs[ST_FETCHED];
+    stats->surfaces = (double)hs[ST_SURFACES];
+    stats->num_2r = (double)hs[ST_2R];
+    stats->surfaces_merged = (double)hs[ST_MERGED];
+    stats->at_least_one = (double)hs[ST_AT_LEAST_ONE];
+    stats->count = (double)hs[ST_COUNT];
+    stats->overflow = (double)hs[ST_OVERFLOW];
+}
+
+// QueryParams of a built context's tree for the general kernel over an
+// explicit list (or the output pattern), with the context's wave scratch.
+static void tree_params(const tlr_ctx* c, QueryParams& Q) {
+    Q.spp_pow2 = c->spp_pow2;
+    Q.cc0 = c->cc0; Q.cc1 = c->cc1;
+    Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
+    Q.root = c->nv > 1 ? c->nv / 2 - 1 : 0;
+    Q.trav = c->trav;
+    Q.leaves = c->leaves;
+    Q.leaf_b = c->leaf_b;
+    Q.pts = c->pts;
+    Q.pext = c->pext;
+}
+
+tlr_status tlr_reconstruct_queries(tlr_ctx* c, const tlr_recon_params* p, const float* queries, int64_t n, int shadow,
+                                   float* rgba_out, float* ext_out, tlr_stats* stats) {
+    if (!c) return TLR_E_INVALID;
+    if (!c->built) return c->fail(TLR_E_STATE, "not built");
+    if (n < 0 || n >= ((int64_t)1 << 31)) return c->fail(TLR_E_INVALID, "query count %lld", (long long)n);
+    if (n > 0 && (!queries || !rgba_out)) return c->fail(TLR_E_INVALID, "null queries or output");
+    if (n == 0) return TLR_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    tlr_status st = ensure_pext(c);
+    if (st != TLR_OK) return st;
+    if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
+    const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
+    const size_t nq = (size_t)n;
+    HIPCHK(c, c->sarena.reserve(nq * 5 * sizeof(float4) + (size_t)waves * kWaveScratchWords * 4 + 8 * 256));
+    c->sarena.reset();
+    float4* qin = c->sarena.alloc<float4>(2 * nq);
+    float4* qcol = c->sarena.alloc<float4>(nq);
+    float4* qext = c->sarena.alloc<float4>(2 * nq);
+    float* scratch = c->sarena.alloc<float>((size_t)waves * kWaveScratchWords);
+    if (c->sarena.failed) return c->fail(TLR_E_HIP, "query arena exhausted");
+    HIPCHK(c, hipMemcpyAsync(qin, queries, nq * 2 * sizeof(float4), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
+    HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
+    QueryParams Q;
+    memset(&Q, 0, sizeof(Q));
+    tree_params(c, Q);
+    Q.W = c->h.width; Q.H = c->h.height; Q.M = 1;
+    Q.lens_gauss = p ? p->lens_gaussian : 0;
+    Q.time_gauss = p ? p->time_gaussian : 0;
+    Q.collect_stats = 1;
+    Q.num_queries = (unsigned)nq;
+    Q.qcol = qcol;
+    Q.stats = c->d_stats;
+    Q.next_query = c->d_counter;
+    Q.scratch = scratch;
+    Q.num_waves = waves;
+    Q.mode = shadow ? 2 : 1;
+    Q.qin = qin;
+    Q.qext = qext;
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    HIPCHK(c, launch_query_general(Q, s));
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    HIPCHK(c, hipMemcpyAsync(rgba_out, qcol, nq * sizeof(float4), hipMemcpyDeviceToHost, s));
+    if (ext_out) HIPCHK(c, hipMemcpyAsync(ext_out, qext, nq * 2 * sizeof(float4), hipMemcpyDeviceToHost, s));
+    unsigned long long hs[kNumStats];
+    HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (stats) {
+        stats_from(hs, stats);
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = stats->general_ms = ms;
+        stats->general_queries = (double)nq;
+        stats->total_ms = stats->query_ms;
+    }
+    if (hs[ST_OVERFLOW])
+        return c->fail(TLR_E_CAPACITY, "%llu queries exceeded the per-query capacity", (unsigned long long)hs[ST_OVERFLOW]);
+    return TLR_OK;
+}
+
+tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const tlr_recon_params* p,
+                                              const tlr_shadow_params* sp, float* out, int out_on_device,
+                                              tlr_stats* stats) {
+    if (!c || !light || !p || !sp) return TLR_E_INVALID;
+    if (out_on_device && !out) return c->fail(TLR_E_INVALID, "out_on_device set with a NULL rgba_out");
+    if (!c->built || !light->built) return c->fail(TLR_E_STATE, "camera or light not built");
+    if (c->device != light->device) return c->fail(TLR_E_INVALID, "camera and light contexts on different GPUs");
+    if (c->h.width != light->h.width || c->h.height != light->h.height || sp->width != c->h.width ||
+        sp->height != c->h.height)
+        return c->fail(TLR_E_INVALID, "camera, light and shadow parameters differ in size");
+    HIPCHK(c, hipSetDevice(c->device));
+    // the light's tree (and its tree-order extras) come from its own stream
+    tlr_status st = ensure_pext(light);
+    if (st != TLR_OK) return st;
+    HIPCHK(c, hipStreamSynchronize(light->stream));
+    st = ensure_pext(c);
+    if (st != TLR_OK) return st;
+    tlr_recon_params pp = *p;
+    pp.gate = 0;
+    pp.path = 1;
+    pp.debug_image = 0;
+    QueryParams Q;
+    st = prepare_queries(c, &pp, false, Q);
+    if (st != TLR_OK) return st;
+    hipStream_t s = c->stream;
+    const int W = c->h.width, H = c->h.height, M = c->last_M;
+    const size_t nq = c->qcol_n;
+    std::vector<float> luv;
+    light_sample_positions(M, luv);
+    HIPCHK(c, c->sarena.reserve(nq * (6 * sizeof(float4) + 8) + luv.size() * 4 + 8 * 256));
+    c->sarena.reset();
+    float4* qext = c->sarena.alloc<float4>(2 * nq);
+    float4* lq = c->sarena.alloc<float4>(2 * nq);
+    float4* xyw = c->sarena.alloc<float4>(nq);
+    float4* scol = c->sarena.alloc<float4>(nq);
+    int* flag = c->sarena.alloc<int>(nq);
+    unsigned* list = c->sarena.alloc<unsigned>(nq);
+    float* d_luv = c->sarena.alloc<float>(luv.size());
+    unsigned* cnt = c->sarena.alloc<unsigned>(64);
+    if (c->sarena.failed) return c->fail(TLR_E_HIP, "shadow arena exhausted");
+    HIPCHK(c, hipMemcpyAsync(d_luv, luv.data(), luv.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(cnt, 0, 64 * sizeof(unsigned), s));
+    ShadowMats S;
+    memcpy(S.c2l, sp->c2l, 64);
+    memcpy(S.zfromw_ndc, sp->zfromw_ndc, 64);
+    memcpy(S.inv_projection, sp->inv_projection, 64);
+    memcpy(S.shadow2_inv_proj, sp->shadow2_inv_proj, 64);
+    memcpy(S.shadow2_inv_t, sp->shadow2_inv_t, 64);
+    memcpy(S.world_inv_t, sp->world_inv_t, 64);
+    S.cam_coc[0] = sp->camera_coc[0]; S.cam_coc[1] = sp->camera_coc[1];
+    S.light_coc[0] = sp->light_coc[0]; S.light_coc[1] = sp->light_coc[1];
+    S.cam_far = sp->camera_far;
+    S.cw = W; S.ch = H;
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    // 1. the camera's output samples, with the nearest sample's xy, w, wg
+    //    and density (the general kernel over the whole pattern)
+    Q.mode = 1;
+    Q.pext = c->pext;
+    Q.qext = qext;
+    HIPCHK(c, launch_query_general(Q, s));
+    HIPCHK(c, hipEventRecord(c->ev[5], s));
+    // 2. shadow rays into the light's window
+    ShadowRays R;
+    R.W = W; R.H = H; R.M = M;
+    R.nq = (unsigned)nq;
+    R.pattern = c->d_pattern;
+    R.light_uv = d_luv;
+    R.qcol = c->qcol;
+    R.qext = qext;
+    R.S = S;
+    R.lq = lq;
+    R.xyw = xyw;
+    R.flag = flag;
+    R.list = list;
+    R.count = cnt;
+    HIPCHK(c, launch_shadow_rays(R, s));
+    // 3. the light queries on the light's tree (shadow mode)
+    QueryParams L;
+    memset(&L, 0, sizeof(L));
+    tree_params(light, L);
+    L.W = W; L.H = H; L.M = M;
+    L.collect_stats = Q.collect_stats;
+    L.num_queries = (unsigned)nq;
+    L.qcol = scol;
+    L.stats = c->d_stats;
+    L.next_query = cnt + 32;
+    L.scratch = Q.scratch;
+    L.num_waves = Q.num_waves;
+    L.mode = 2;
+    L.qin = lq;
+    L.qlist = list;
+    L.qcount = cnt;
+    HIPCHK(c, launch_query_general(L, s));
+    HIPCHK(c, hipEventRecord(c->ev[6], s));
+    // 4. shading, per pixel in output-sample order
+    ShadowShade D;
+    D.W = W; D.H = H; D.M = M;
+    D.qcol = c->qcol;
+    D.qext = qext;
+    D.scol = scol;
+    D.xyw = xyw;
+    D.flag = flag;
+    D.S = S;
+    D.out = out_on_device ? out : c->d_img;
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    HIPCHK(c, launch_shadow_shade(D, s));
+    HIPCHK(c, hipEventRecord(c->ev[4], s));
+    if (!out_on_device && out) HIPCHK(c, hipMemcpyAsync(out, c->d_img, (size_t)W * H * 16, hipMemcpyDeviceToHost, s));
+    unsigned long long hs[kNumStats];
+    unsigned nl = 0;
+    HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&nl, cnt, sizeof(nl), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (stats) {
+        stats_from(hs, stats);
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) stats->general_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) stats->fast_ms = ms;
+        stats->general_queries = (double)nl;   // light queries
+        stats->total_ms = stats->query_ms + stats->combine_ms;
+    }
+    if (hs[ST_OVERFLOW])
+        return c->fail(TLR_E_CAPACITY, "%llu output samples exceeded the per-query capacity",
+                       (unsigned long long)hs[ST_OVERFLOW]);
     return TLR_OK;
 }
 
@@ -1879,6 +2147,7 @@ tlr_status tlr_use_slot(tlr_ctx* c, int slot) {
     c->N = b->N;
     c->has_samples = true;
     c->built = false;
+    c->wgd_n = 0;
     return TLR_OK;
 }
 

@@ -19,7 +19,21 @@ def test_sobol_dim3_direction_numbers(orc):
         assert L.orc_sobol(3, 1 << c) == np.float32(v * 2.0 ** -32)
 
 
-@pytest.mark.parametrize("dim", [0, 3])
+def test_sobol_dim4_direction_numbers(orc):
+    # Sobol dimension 5 (index 4; Joe-Kuo: s=3, a=2, m=1,1,1, polynomial
+    # x^3+x^2+1), used for the light positions of the shadow path: the
+    # recurrence v_c = v_{c-3} ^ (v_{c-3} >> 3) ^ v_{c-1} from the published
+    # initial numbers.
+    L = orc.lib()
+    v = [0x80000000, 0x40000000, 0x20000000]
+    for c in range(3, 12):
+        v.append(v[c - 3] ^ (v[c - 3] >> 3) ^ v[c - 1])
+    assert v[3:8] == [0xB0000000, 0xF8000000, 0xDC000000, 0x7A000000, 0x9D000000]
+    for c, x in enumerate(v):
+        assert L.orc_sobol(4, 1 << c) == np.float32(x * 2.0 ** -32)
+
+
+@pytest.mark.parametrize("dim", [0, 3, 4])
 def test_sobol_is_a_net(orc, dim):
     # The first 2^m points of a 1-D Sobol dimension are a (0,m,1)-net:
     # exactly one point per interval [k/2^m, (k+1)/2^m).
