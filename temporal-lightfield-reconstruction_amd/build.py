@@ -68,17 +68,18 @@ def build_tlr_tiny_cap():
     return build_tlr("libtlr_tiny_cap.so", ("TLR_LEAF_CAP=8",))
 
 
-def build_facade_example():
-    """C++ caller of the facade (tests/cpp/facade_example.cpp), host compiler only."""
-    out = os.path.join(LIB, "facade_example")
+def build_facade_example(name="facade_example"):
+    """C++ caller of the facade (tests/cpp/<name>.cpp), host compiler only."""
+    out = os.path.join(LIB, name)
     _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-          "-o", out, os.path.join(ROOT, "tests", "cpp", "facade_example.cpp"),
+          "-o", out, os.path.join(ROOT, "tests", "cpp", name + ".cpp"),
           "-L", LIB, "-ltlr", "-Wl,-rpath,$ORIGIN"])
     return out
 
 
 def build_all():
-    return [build_scene(), build_tlr(), build_tlr_small_lds(), build_tlr_tiny_cap(), build_facade_example()]
+    return [build_scene(), build_tlr(), build_tlr_small_lds(), build_tlr_tiny_cap(), build_facade_example(),
+            build_facade_example("facade_shadow_example")]
 
 
 if __name__ == "__main__":

@@ -103,6 +103,38 @@ class UVTSampleBuffer {
     float getSampleW(int x, int y, int i) const { return rec(x, y, i)[3]; }
     Vec4f getSampleColor(int x, int y, int i) const { const float* r = rec(x, y, i); return Vec4f(r[7], r[8], r[9], 1); }
     Vec3f getSampleMV(int x, int y, int i) const { const float* r = rec(x, y, i); return Vec3f(r[11], r[12], r[13]); }
+    Vec2f getSampleWG(int x, int y, int i) const { const float* r = rec(x, y, i); return Vec2f(r[14], r[15]); }
+    void setSampleColor(int x, int y, int i, const Vec4f& c) {
+        float* r = &rec_[16 * (size_t)getIndex(x, y, i)];
+        r[7] = c.x; r[8] = c.y; r[9] = c.z; r[10] = c.w;
+    }
+    // Extra per-sample float channels by name (SampleBuffer.hpp:104-112; the
+    // shadow path reads "LIGHTSPACE_DENSITY").  getSampleExtra of channel -1
+    // is 0, as in the reference.
+    int reserveChannel(const std::string& name) {
+        int id = getChannelID(name);
+        if (id >= 0) return id;
+        names_.push_back(name);
+        channels_.emplace_back(rec_.size() / 16, 0.0f);
+        return (int)names_.size() - 1;
+    }
+    int getChannelID(const std::string& name) const {
+        for (size_t k = 0; k < names_.size(); k++)
+            if (names_[k] == name) return (int)k;
+        return -1;
+    }
+    template <class T>
+    T getSampleExtra(int cid, int x, int y, int i) const {
+        return cid < 0 ? T(0) : T(channels_[cid][getIndex(x, y, i)]);
+    }
+    template <class T>
+    void setSampleExtra(int cid, int x, int y, int i, T v) {
+        channels_[cid][getIndex(x, y, i)] = (float)v;
+    }
+    const std::vector<float>* channel(const std::string& name) const {
+        const int id = getChannelID(name);
+        return id < 0 ? nullptr : &channels_[id];
+    }
     void serialize(const char* filename, bool separateHeader = false, bool binary = false) const {
         tlr_status s = tlr_save_sample_file(filename, &h_, rec_.data(), binary, separateHeader);
         if (s != TLR_OK) throw ReconstructionError(s, std::string("cannot write ") + filename);
@@ -112,6 +144,8 @@ class UVTSampleBuffer {
 
   private:
     const float* rec(int x, int y, int i) const { return &rec_[16 * (size_t)getIndex(x, y, i)]; }
+    std::vector<std::string> names_;
+    std::vector<std::vector<float>> channels_;
     tlr_header h_{};
     std::vector<float> rec_;
 };
@@ -129,6 +163,15 @@ class CameraParams {
     float projection[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     int windowWidth = 0;
     float apertureRadius = 0;
+    float focusDistance = 1;
+    // The camera data the shadow path reads from the reference's
+    // CameraControls (getCameraToWorld, getWorldToCamera, getFar), row-major.
+    float cameraToWorld[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float worldToCamera[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float farPlane = 100.0f;
+
+    // getCocCoeffs (CameraParams.hpp:97-118) at the focus distance
+    void getCocCoeffs(float out[2]) const { cocCoeffsAt(focusDistance, out); }
 
     // getCocCoeffs at focus distance f (CameraParams.hpp:97-118): m =
     // projection * getDofMatrix((0.5, 0)) - projection, products summed over k
@@ -212,9 +255,93 @@ class TreeGather {
             }
     }
 
+    // Shadow-only reconstruction (Reconstruction.cpp:302-347): every sample
+    // of qbuf is a light-space query (its xy, uv, t, w and LIGHTSPACE_DENSITY
+    // channel -- 0 without the channel, as the reference reads it); its
+    // colour is replaced by the result.  debugImage is not filled.
+    void reconstructShadows(UVTSampleBuffer* qbuf, Image* debugImage = nullptr) {
+        (void)debugImage;
+        uploadShadowInputs();
+        const int W = qbuf->getWidth(), H = qbuf->getHeight(), S = qbuf->getNumSamples();
+        const int cid = qbuf->getChannelID("LIGHTSPACE_DENSITY");
+        std::vector<float> q((size_t)W * H * S * 8);
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++)
+                for (int i = 0; i < S; i++) {
+                    float* o = &q[8 * (size_t)qbuf->getIndex(x, y, i)];
+                    const Vec2f xy = qbuf->getSampleXY(x, y, i), uv = qbuf->getSampleUV(x, y, i);
+                    o[0] = xy.x; o[1] = xy.y; o[2] = uv.x; o[3] = uv.y;
+                    o[4] = qbuf->getSampleT(x, y, i);
+                    o[5] = qbuf->getSampleW(x, y, i);
+                    o[6] = qbuf->getSampleExtra<float>(cid, x, y, i);
+                    o[7] = 0.0f;
+                }
+        std::vector<float> rgba((size_t)W * H * S * 4);
+        check(tlr_reconstruct_queries(ctx_, nullptr, q.data(), (int64_t)W * H * S, 1, rgba.data(), nullptr, &stats_),
+              "shadow queries");
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++)
+                for (int i = 0; i < S; i++) {
+                    const float* c = &rgba[4 * (size_t)qbuf->getIndex(x, y, i)];
+                    qbuf->setSampleColor(x, y, i, Vec4f(c[0], c[1], c[2], c[3]));
+                }
+    }
+
+    // Depth of field and motion with shadows from shadowTG, the light's
+    // TreeGather of the same size (Reconstruction.cpp:355-433); the matrices
+    // come from both CameraParams.  The reference also writes a screenshot
+    // file; this does not.
+    void reconstructDofMotionShadows(Image& image, const TreeGather& shadowTG) {
+        const int W = sbuf_->getWidth(), H = sbuf_->getHeight();
+        if (image.getSize().x < W || image.getSize().y < H)
+            throw ReconstructionError(TLR_E_IMAGE_TOO_SMALL,
+                                      "TreeGather::reconstructDofMotionShadows image smaller than < sample buffer");
+        uploadShadowInputs();
+        const_cast<TreeGather&>(shadowTG).uploadShadowInputs();
+        const CameraParams& c = *params_;
+        const CameraParams& l = *shadowTG.params_;
+        float cc[2], lc[2];
+        c.getCocCoeffs(cc);
+        l.getCocCoeffs(lc);
+        tlr_shadow_params sp;
+        check(tlr_shadow_setup(c.projection, c.cameraToWorld, l.projection, l.worldToCamera, W, H, cc, lc,
+                               c.farPlane, &sp),
+              "shadow setup");
+        tlr_recon_params p{};
+        p.out_spp = out_spp_;
+        p.override_uvt = c.overrideUVT != Vec3f(FLT_MAX);
+        p.uvt[0] = c.overrideUVT.x; p.uvt[1] = c.overrideUVT.y; p.uvt[2] = c.overrideUVT.z;
+        p.lens_gaussian = c.lensFilter == LENS_GAUSSIAN;
+        p.time_gaussian = c.timeFilter == TIME_GAUSSIAN;
+        std::vector<float> rgba((size_t)W * H * 4);
+        check(tlr_reconstruct_dof_motion_shadows(ctx_, shadowTG.ctx_, &p, &sp, rgba.data(), 0, &stats_),
+              "dof motion shadows");
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                const float* q = &rgba[4 * ((size_t)y * W + x)];
+                image.setVec4f(Vec2i(x, y), Vec4f(q[0], q[1], q[2], q[3]));
+            }
+    }
+
     const tlr_stats& lastStats() const { return stats_; }
 
   private:
+    // w-gradients (records' floats 14, 15) and the LIGHTSPACE_DENSITY
+    // channel (1 without it) of the input buffer, once
+    void uploadShadowInputs() {
+        if (shadow_inputs_) return;
+        const std::vector<float>& r = sbuf_->records();
+        const size_t n = r.size() / 16;
+        std::vector<float> wg(2 * n);
+        for (size_t k = 0; k < n; k++) {
+            wg[2 * k] = r[16 * k + 14];
+            wg[2 * k + 1] = r[16 * k + 15];
+        }
+        const std::vector<float>* d = sbuf_->channel("LIGHTSPACE_DENSITY");
+        check(tlr_set_shadow_inputs(ctx_, wg.data(), d ? d->data() : nullptr, (int64_t)n), "shadow inputs");
+        shadow_inputs_ = true;
+    }
+
     void check(tlr_status s, const char* what) {
         if (s != TLR_OK)
             throw ReconstructionError(s, std::string(what) + ": " + (ctx_ ? tlr_last_error(ctx_) : "no device"));
@@ -223,6 +350,7 @@ class TreeGather {
     const CameraParams* params_;
     tlr_ctx* ctx_ = nullptr;
     int out_spp_ = NUM_OUTPUT_SAMPLES;
+    bool shadow_inputs_ = false;
     tlr_stats stats_{};
 };
 

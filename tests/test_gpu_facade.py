@@ -28,3 +28,62 @@ def test_facade_example_matches_oracle(tlr, orc, tmp_path):
     ref, _, _ = o.reconstruct(out_spp=8)
     err = np.abs(img - ref)
     assert err.max() <= 2e-3 and err.mean() <= 1e-4
+
+
+SHADOW_EXE = os.path.join(ROOT, "temporal-lightfield-reconstruction_amd", "lib", "facade_shadow_example")
+
+
+def _persp(fovy, aspect, near, far):
+    f = 1.0 / np.tan(fovy / 2)
+    return np.array([[f / aspect, 0, 0, 0], [0, f, 0, 0],
+                     [0, 0, (far + near) / (near - far), 2 * far * near / (near - far)], [0, 0, -1, 0]])
+
+
+def test_facade_shadows_match_oracle(tlr, orc, tmp_path):
+    """TreeGather::reconstructDofMotionShadows and reconstructShadows called
+    through the facade from C++ (tests/cpp/facade_shadow_example.cpp) give
+    the oracle's image and query colours bit for bit.  The cameras have no
+    aperture, so the CameraParams' CoC coefficients are zero."""
+    W, H, spp = 48, 32, 8
+    rng = np.random.default_rng(12)
+    paths = []
+    bufs = []
+    for k, seed in enumerate((61, 62)):
+        sb = tlr.make_scene(2, W, H, spp, seed=seed)
+        sb.records[:, 14:16] = rng.normal(0, 0.05, (sb.records.shape[0], 2)).astype(np.float32)
+        path = str(tmp_path / ("buf%d.bin" % k))
+        sb.save(path, binary=True, separate_header=True)
+        paths.append(path)
+        bufs.append(tlr.UVTSampleBuffer.load(path))
+    P = _persp(np.radians(60), W / H, 0.1, 100.0)
+    a = np.radians(3.0)
+    W2L = np.eye(4)
+    W2L[:3, :3] = [[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]]
+    W2L[:3, 3] = [0.2, 0.0, 0.1]
+    lift = 0.05
+    mats = np.concatenate([P.reshape(-1), np.eye(4).reshape(-1), P.reshape(-1), W2L.reshape(-1),
+                           [100.0, lift]]).astype(np.float32)
+    mpath = str(tmp_path / "m.f32")
+    mats.tofile(mpath)
+    img_path, q_path = str(tmp_path / "img.rgba"), str(tmp_path / "q.rgba")
+    print("facade shadow example", flush=True)
+    r = subprocess.run([SHADOW_EXE, paths[0], paths[1], mpath, img_path, q_path, "8"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    img = np.fromfile(img_path, np.float32).reshape(H, W, 4)
+    qcol = np.fromfile(q_path, np.float32).reshape(-1, 4)
+    cam = orc.Oracle(W, H, spp, bufs[0].coc[0], bufs[0].coc[1], bufs[0].records)
+    light = orc.Oracle(W, H, spp, bufs[1].coc[0], bufs[1].coc[1], bufs[1].records)
+    cam.build()
+    light.build()
+    sp = orc.shadow_setup(P, np.eye(4), P, W2L, W, H, (0.0, 0.0), (0.0, 0.0), 100.0)
+    ref = cam.reconstruct_dof_motion_shadows(light, sp, out_spp=8)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    rec = bufs[1].records
+    q = np.zeros((rec.shape[0], 8), np.float32)
+    q[:, 0:2], q[:, 2:4], q[:, 4] = rec[:, 0:2], rec[:, 4:6], rec[:, 6]
+    q[:, 5] = rec[:, 3] + np.float32(lift)
+    q[:, 6] = 1.5
+    want, _, _ = light.reconstruct_queries(q, shadow=True)
+    want[:, 3] = 1.0    # the facade's buffers read alpha as 1
+    assert np.array_equal(qcol.view(np.uint32), want.view(np.uint32))
