@@ -38,7 +38,7 @@ typedef struct tlr_ctx tlr_ctx;
 /* Sample-buffer header: UVTSampleBuffer width/height/spp/CoC coefficients
  * (SampleBuffer.hpp:138-183, v1.3 header SampleBuffer.cpp:244-268). */
 typedef struct tlr_header {
-    int   width, height, spp;
+    int   width, height, spp;      /* spp 0: an irregular buffer (tlr_set_samples only) */
     float coc_c0, coc_c1;      /* CoC radius = C0/w + C1 (CameraParams.hpp:72) */
 } tlr_header;
 
@@ -124,7 +124,11 @@ void        tlr_free(void* p);
 
 /* Input upload.  AoS records (16 floats, v1.3 order) from host memory, or an
  * SoA view of host (on_device=0) or device (on_device=1) arrays.  The buffers
- * are borrowed for the duration of the call. */
+ * are borrowed for the duration of the call.  With h->spp 0 (an irregular
+ * buffer, TreeGather::init's hook Reconstruction.cpp:65-71), tlr_set_samples
+ * takes n records in any order and the build's dispersion uses spp =
+ * n / (width * height), rounded up to a power of two as for regular buffers;
+ * tlr_set_samples_soa and the batch slots take regular buffers only. */
 tlr_status  tlr_set_samples(tlr_ctx* ctx, const tlr_header* h, const float* records, int64_t n);
 tlr_status  tlr_set_samples_soa(tlr_ctx* ctx, const tlr_header* h, const tlr_soa* soa, int on_device);
 

@@ -1872,13 +1872,24 @@ orc_ctx* orc_create(int W, int H, int spp, float C0, float C1, const float* reco
     return c;
 }
 
+// An irregular buffer: n samples in any pixel order (the header's spp 0).
+orc_ctx* orc_create_irregular(int W, int H, float C0, float C1, const float* records, int n) {
+    orc_ctx* c = new orc_ctx;
+    c->W = W; c->H = H; c->spp = 0; c->C0 = C0; c->C1 = C1;
+    c->N = n;
+    c->rec.assign(records, records + (size_t)n * 16);
+    return c;
+}
+
 void orc_destroy(orc_ctx* c) { delete c; }
 const char* orc_error(const orc_ctx* c) { return c->err.c_str(); }
 
 int orc_build(orc_ctx* c, float ap_adj, float foc_adj, int threads) {
     threads = hw_threads(threads);
-    // TreeGather::init preamble: Reconstruction.cpp:63-87
+    // TreeGather::init preamble: Reconstruction.cpp:63-87; an irregular
+    // buffer (spp 0) averages its per-pixel counts, integer division (:65-71)
     int spp = c->spp;
+    if (spp == 0) spp = c->N / (c->W * c->H);
     c->spp_pow2 = std::max(1, (int)round_pow2((uint32_t)spp));
     if (!dispersion_index_ok(c->spp_pow2)) {
         c->err = "unsupported spp";

@@ -132,6 +132,8 @@ def lib():
         L.orc_same_surface.argtypes = [fp, fp, fp]
         L.orc_surface_scan.argtypes = [C.c_int, fp, fp, fp, fp, fp, ip]
         L.orc_set_density.argtypes = [P, C.c_void_p]
+        L.orc_create_irregular.restype = P
+        L.orc_create_irregular.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, fp, C.c_int]
         L.orc_reconstruct_queries.argtypes = [P, C.POINTER(ReconParams), fp, C.c_int, C.c_int, C.c_void_p,
                                               C.c_void_p, C.POINTER(Stats)]
         L.orc_shadow_setup.argtypes = [fp, fp, fp, fp, C.c_int, C.c_int, fp, fp, C.c_float,
@@ -158,6 +160,16 @@ class Oracle:
         assert rec.size == W * H * spp * 16
         self.W, self.H, self.spp = W, H, spp
         self.h = L.orc_create(W, H, spp, C0, C1, rec)
+
+    @classmethod
+    def irregular(cls, W, H, C0, C1, records):
+        """An irregular buffer (spp 0): n records in any order."""
+        self = cls.__new__(cls)
+        rec = np.ascontiguousarray(records, dtype=np.float32).reshape(-1)
+        self.W, self.H, self.spp = W, H, 0
+        self.n = rec.size // 16
+        self.h = lib().orc_create_irregular(W, H, C0, C1, rec, self.n)
+        return self
 
     def close(self):
         if self.h:
@@ -254,7 +266,7 @@ class Oracle:
     def set_density(self, density):
         """Per-sample LIGHTSPACE_DENSITY (input order) or None for 1; rebuild after."""
         d = None if density is None else np.ascontiguousarray(density, np.float32).reshape(-1)
-        assert d is None or d.size == self.W * self.H * self.spp
+        assert d is None or d.size == (self.W * self.H * self.spp if self.spp else self.n)
         lib().orc_set_density(self.h, _ptr(d))
 
     def reconstruct_queries(self, queries, shadow=False, **kw):

@@ -188,6 +188,10 @@ int      orc_same_surface(const float* n0, const float* n1, const float* o5);
 int orc_surface_scan(int n, const float* t, const float* ba, const float* bb, const float* ba1, const float* bb1,
                      int32_t* surf);
 
+/* An irregular buffer (Reconstruction.cpp:65-71): n records in any order,
+ * spp 0; the dispersion takes spp = n / (W*H) (integer division). */
+orc_ctx* orc_create_irregular(int W, int H, float C0, float C1, const float* records, int n);
+
 /* ---- Shadows (TreeGather::reconstructShadows / reconstructDofMotionShadows,
  * Reconstruction.cpp:302-433; the filter's shadow test
  * ReconstructionOur.cpp:78-80, 283-291, 325-347) ---- */

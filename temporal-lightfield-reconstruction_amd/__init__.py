@@ -258,7 +258,8 @@ class UVTSampleBuffer:
         self.width, self.height, self.spp = int(width), int(height), int(spp)
         self.coc = (float(coc_c0), float(coc_c1))
         self.records = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
-        if self.records.shape[0] != self.width * self.height * self.spp:
+        # spp 0: an irregular buffer, any number of records
+        if self.spp and self.records.shape[0] != self.width * self.height * self.spp:
             raise ValueError("record count does not match the header")
 
     @classmethod

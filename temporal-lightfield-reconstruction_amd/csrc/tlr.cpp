@@ -274,8 +274,10 @@ void tlr_destroy(tlr_ctx* c) {
 
 const char* tlr_last_error(const tlr_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-static tlr_status check_header(tlr_ctx* c, const tlr_header* h) {
-    if (!h || h->width <= 0 || h->height <= 0 || h->spp <= 0)
+// spp 0 (an irregular buffer, Reconstruction.cpp:65-71) only where the
+// caller gives the sample count.
+static tlr_status check_header(tlr_ctx* c, const tlr_header* h, bool irregular_ok = false) {
+    if (!h || h->width <= 0 || h->height <= 0 || h->spp < 0 || (h->spp == 0 && !irregular_ok))
         return c->fail(TLR_E_INVALID, "invalid header");
     const int64_t n = (int64_t)h->width * h->height * h->spp;
     if (n >= (int64_t)1 << 31) return c->fail(TLR_E_INVALID, "sample count %lld exceeds 2^31", (long long)n);
@@ -297,9 +299,9 @@ static tlr_status reserve_input(tlr_ctx* c, int N) {
 // AoS records -> device SoA: the transpose runs on the host once per upload.
 tlr_status tlr_set_samples(tlr_ctx* c, const tlr_header* h, const float* rec, int64_t n) {
     if (!c) return TLR_E_INVALID;
-    tlr_status s = check_header(c, h);
+    tlr_status s = check_header(c, h, true);
     if (s != TLR_OK) return s;
-    if (!rec || n != (int64_t)h->width * h->height * h->spp)
+    if (h->spp == 0 ? (!rec || n < 1 || n >= ((int64_t)1 << 31)) : (!rec || n != (int64_t)h->width * h->height * h->spp))
         return c->fail(TLR_E_INVALID, "record count does not match the header");
     HIPCHK(c, hipSetDevice(c->device));
     const int N = (int)n;
@@ -382,9 +384,10 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     hipStream_t s = c->stream;
     const int N = c->N, W = c->h.width, H = c->h.height;
 
-    // TreeGather::init preamble: Reconstruction.cpp:63-87
+    // TreeGather::init preamble: Reconstruction.cpp:63-87; an irregular
+    // buffer (spp 0) averages its per-pixel counts, integer division (:65-71)
     {
-        uint32_t v = (uint32_t)c->h.spp;
+        uint32_t v = (uint32_t)(c->h.spp ? c->h.spp : N / (W * H));
         v--;
         v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
         v++;

@@ -150,3 +150,28 @@ def test_shadow_errors(tlr):
         c.reconstruct_dof_motion_shadows(other, sp)   # sizes differ
     col, ext, _ = c.reconstruct_queries(np.zeros((0, 8), np.float32))
     assert col.shape == (0, 4)
+
+
+def test_irregular_buffer_matches_oracle(tlr, orc):
+    """An irregular buffer (spp 0: 70 % of a scene's samples in shuffled
+    order) built and reconstructed on the GPU, and queried in shadow mode,
+    equals the oracle bit for bit."""
+    sb = tlr.make_scene(2, 64, 48, 8, seed=71)
+    rng = np.random.default_rng(4)
+    rec = sb.records[rng.permutation(sb.records.shape[0])[: int(0.7 * sb.records.shape[0])]].copy()
+    rec[:, 14:16] = rng.normal(0, 0.05, (rec.shape[0], 2)).astype(np.float32)
+    irr = tlr.UVTSampleBuffer(64, 48, 0, sb.coc[0], sb.coc[1], rec)
+    ctx = tlr.Ctx(0)
+    ctx.set_samples(irr)
+    ctx.set_shadow_inputs(rec[:, 14:16], None)
+    ctx.build()
+    o = orc.Oracle.irregular(64, 48, sb.coc[0], sb.coc[1], rec)
+    info = o.build()
+    assert info.spp_pow2 == 8    # 5.6 per pixel -> 5 -> 8
+    img, _ = ctx.reconstruct(out_spp=8)
+    ref, _, _ = o.reconstruct(out_spp=8)
+    assert np.array_equal(_bits(img), _bits(ref))
+    q = _queries(np.random.default_rng(9), irr, rec, 5000)
+    col, ext, _ = ctx.reconstruct_queries(q, shadow=True)
+    ocol, oext, _ = o.reconstruct_queries(q, shadow=True)
+    assert np.array_equal(_bits(col), _bits(ocol)) and np.array_equal(_bits(ext[:, :6]), _bits(oext[:, :6]))

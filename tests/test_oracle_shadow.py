@@ -234,3 +234,18 @@ def test_dof_motion_shadows_lit_from_the_camera(orc):
             f = 0.25 if (shadowed and px < 16) else 1.5 * cosv + 0.25
             assert np.allclose(img[py, px, :3], c * f, atol=2e-3), (shadowed, px, py, img[py, px], c * f)
             assert abs(img[py, px, 3] - 1.0) < 1e-6
+
+
+# -------------------------------------------------------- irregular buffers
+@pytest.mark.parametrize("keep,spp_pow2", [(1.0, 16), (0.55, 8), (0.3, 4), (0.05, 1)])
+def test_irregular_buffer_spp_is_the_average(orc, keep, spp_pow2):
+    """An irregular buffer (spp 0, Reconstruction.cpp:65-71) takes the
+    average samples per pixel, integer division, rounded up to a power of
+    two: 16 -> 16, 8.8 -> 8, 4.8 -> 4, 0.8 -> 0 -> 1."""
+    rec = ps.records(plane).reshape(-1, 16)
+    n = int(round(keep * rec.shape[0]))
+    sub = rec[np.random.default_rng(0).permutation(rec.shape[0])[:n]]
+    o = orc.Oracle.irregular(ps.W, ps.H, ps.C0, ps.C1, sub)
+    info = o.build()
+    assert info.spp_pow2 == spp_pow2
+    assert info.num_input == n
