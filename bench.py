@@ -541,6 +541,61 @@ def gate_leg(args, ctx, sbuf, step, timed, dt_plain, log):
             "params": os.path.relpath(GATE_PARAMS, ROOT), "trained_by": meta.get("command")}
 
 
+def bench_shadows(args):
+    """reconstructDofMotionShadows (camera pass with extended results,
+    shadow rays, light pass in shadow mode, shading) on the workload's scene
+    with a second seeded scene as the light's buffer; the light is the camera
+    turned 3 degrees about the y axis.  Both builds are outside the timed
+    region (the reference builds both TreeGathers before the call)."""
+    tlr = ge.load_package()
+    cfg = CONFIGS[args.config]
+    W, H, spp, M = cfg["W"], cfg["H"], cfg["spp"], cfg["out_spp"]
+    cam_sb = tlr.make_scene(cfg["preset"], W, H, spp, seed=args.seed)
+    light_sb = tlr.make_scene(cfg["preset"], W, H, spp, seed=args.seed + 1)
+    cam, light = tlr.Ctx(0), tlr.Ctx(0)
+    for c, sb in ((cam, cam_sb), (light, light_sb)):
+        c.set_samples(sb)
+        c.set_shadow_inputs(sb.records[:, 14:16], None)
+        c.build()
+    f = 1.0 / np.tan(np.radians(60) / 2)
+    P = np.array([[f * H / W, 0, 0, 0], [0, f, 0, 0], [0, 0, -100.1 / 99.9, -20.0 / 99.9], [0, 0, -1, 0]])
+    a = np.radians(3.0)
+    W2L = np.eye(4)
+    W2L[:3, :3] = [[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]]
+    sp = tlr.shadow_setup(P, np.eye(4), P, W2L, W, H, cam_sb.coc, light_sb.coc, 100.0)
+    for _ in range(args.warmup):
+        cam.reconstruct_dof_motion_shadows(light, sp, out_spp=M)
+    walls, sts = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, st = cam.reconstruct_dof_motion_shadows(light, sp, out_spp=M)
+        walls.append(time.perf_counter() - t0)
+        sts.append(st)
+    dt = float(np.mean(walls))
+    dev = {k: float(np.mean([s[k] for s in sts])) for k in ("general_ms", "fast_ms", "combine_ms", "query_ms")}
+    res = {
+        "metric": "reconstructed Mpix/s @%d out-spp with shadows, 1 GPU" % M,
+        "value": W * H / dt / 1e6,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (two seeded scenes: camera and light buffers)",
+        "config": {"workload": cfg["name"] + "; light buffer seed + 1, light turned 3 degrees", "width": W,
+                   "height": H, "input_spp": spp, "out_spp": M, "seed": args.seed},
+        "device_ms": {"camera_pass": dev["general_ms"], "light_pass": dev["fast_ms"], "shading": dev["combine_ms"],
+                      "total": dev["query_ms"] + dev["combine_ms"]},
+        "light_queries": float(np.mean([s["general_queries"] for s in sts])),
+        "output_samples": W * H * M,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -560,6 +615,9 @@ def main():
     ap.add_argument("--gate-epochs", type=int, default=4, help="--gate-train: epochs")
     ap.add_argument("--path", type=int, default=0, choices=(0, 2),
                     help="2: the reference's GPU-variant method instead of the plain one")
+    ap.add_argument("--shadows", action="store_true",
+                    help="reconstructDofMotionShadows on the workload: a second seeded scene as the light's "
+                         "buffer, the light turned 3 degrees about the camera")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the counter-calibration kernels over MIB MiB")
     args = ap.parse_args()
@@ -582,6 +640,9 @@ def main():
         return
     if args.path == 2:
         bench_refgpu(args)
+        return
+    if args.shadows:
+        bench_shadows(args)
         return
     bench_single(args, CONFIGS[args.config])
 
