@@ -457,11 +457,15 @@ def bench_single(args, cfg):
     st = step(stats=True)
     mean = lambda k, ss=None: float(np.mean([x[k] for x in (ss or sts)]))
     q_ms, b_ms = mean("query_ms"), mean("build_ms")
-    fast_ms, gen_ms = mean("fast_ms"), mean("general_ms")
+    # fast_ms spans both 1R lane passes; wide_ms is the second's part
+    wide_ms = mean("wide_ms")
+    fast_ms, gen_ms = mean("fast_ms") - wide_ms, mean("general_ms")
     b_in, b_build, b_query, b_out = algorithmic_bytes(st, n_in, W, H)
     b_fast = (64.0 * st["fast_leaf_nodes"] + 28.0 * st["fast_samples_in_leaf_nodes"] +
               12.0 * st["fast_samples_within_1r"])
-    b_gen = b_query - b_fast
+    b_wide = (64.0 * st["wide_leaf_nodes"] + 28.0 * st["wide_samples_in_leaf_nodes"] +
+              12.0 * st["wide_samples_within_1r"])
+    b_gen = b_query - b_fast - b_wide
     wk = "cfg%d" % args.config
     achieved = b_fast / (fast_ms * 1e-3) / 1e9
     traffic = read_traffic("k_query_fast", wk)
@@ -485,8 +489,9 @@ def bench_single(args, cfg):
                    "note": "host AoS->SoA transpose + pageable copies, once per buffer, outside the timed steps",
                    "mpix_s_including_upload_each_step": W * H / (dt + upload_s) / 1e6},
         "with_image_to_host": {"ms_per_step": dt_host * 1e3, "mpix_s": W * H / dt_host / 1e6},
-        "device_ms": {"build": b_ms, "query": q_ms, "query_fast": fast_ms, "query_general": gen_ms,
-                      "combine": mean("combine_ms")},
+        "device_ms": {"build": b_ms, "query": q_ms, "query_fast": fast_ms, "query_wide": wide_ms,
+                      "query_general": gen_ms, "combine": mean("combine_ms")},
+        "wide_pass_entries": st["wide_queries"],
         "general_queries": st["general_queries"],
         "stats_per_output_sample": {k: st[k] / st["count"] for k in (
             "leaf_nodes", "samples_in_leaf_nodes", "samples_within_1r", "samples_fetched", "surfaces",
@@ -496,6 +501,8 @@ def bench_single(args, cfg):
                      "traffic": traffic,
                      "traffic_gbs": traffic / (fast_ms * 1e-3) / 1e9 if traffic else None,
                      "algorithmic_bytes_per_launch": b_fast, "avg_launch_ms": fast_ms},
+        "roofline_wide": {"kernel": "k_query_fast_wide", "algorithmic_bytes_per_launch": b_wide,
+                          "avg_launch_ms": wide_ms, "achieved": b_wide / max(wide_ms * 1e-3, 1e-9) / 1e9},
         "roofline_general": {"kernel": "k_query_general", "algorithmic_bytes_per_launch": b_gen,
                              "avg_launch_ms": gen_ms,
                              "achieved": b_gen / max(gen_ms * 1e-3, 1e-9) / 1e9},

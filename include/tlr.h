@@ -92,6 +92,12 @@ typedef struct tlr_stats {
     /* the 1R lane kernel's share of leaf_nodes, samples_in_leaf_nodes and
      * samples_within_1r (its algorithmic bytes) */
     double fast_leaf_nodes, fast_samples_in_leaf_nodes, fast_samples_within_1r;
+    /* the plain path's second 1R lane pass, over the first one's hand-offs
+     * that held more 1R leaves than its per-lane lists: its time (part of
+     * fast_ms), the output samples it ran, and its share of leaf_nodes,
+     * samples_in_leaf_nodes and samples_within_1r */
+    double wide_ms, wide_queries;
+    double wide_leaf_nodes, wide_samples_in_leaf_nodes, wide_samples_within_1r;
 } tlr_stats;
 
 typedef struct tlr_build_info {

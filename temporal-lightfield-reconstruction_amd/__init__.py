@@ -71,7 +71,8 @@ class Stats(C.Structure):
         "surfaces", "num_2r", "surfaces_merged", "at_least_one", "count", "overflow",
         "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
         "fast_ms", "general_ms", "general_queries", "gate_candidates", "gate_ms", "gate_rejected",
-        "fast_leaf_nodes", "fast_samples_in_leaf_nodes", "fast_samples_within_1r")]
+        "fast_leaf_nodes", "fast_samples_in_leaf_nodes", "fast_samples_within_1r",
+        "wide_ms", "wide_queries", "wide_leaf_nodes", "wide_samples_in_leaf_nodes", "wide_samples_within_1r")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -60,9 +60,12 @@ def build_tlr(name="libtlr.so", defines=()):
 
 def build_tlr_small_lds():
     """Test build: the general kernel's LDS leaf staging and frontier stack at
-    their smallest, so that the paths past them run on every query
+    their smallest, so that the paths past them run on every query, and the
+    1R lane passes' per-lane leaf lists small (4, then 8), so that ordinary
+    scenes take both lane passes and the general kernel
     (tests/test_gpu_small_lds.py)."""
-    return build_tlr("libtlr_small_lds.so", ("TLR_GATHER_STAGE=4", "TLR_GATHER_LDS_STACK=64"))
+    return build_tlr("libtlr_small_lds.so", ("TLR_GATHER_STAGE=4", "TLR_GATHER_LDS_STACK=64",
+                                             "TLR_FAST_LEAF_CAP=4", "TLR_WIDE_LEAF_CAP=8"))
 
 
 def build_tlr_tiny_cap():

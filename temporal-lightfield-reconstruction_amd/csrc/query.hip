@@ -901,12 +901,17 @@ __device__ __forceinline__ void write_gate_features(float* out, bool prov, const
 // GATE: hand-offs that have a first surface at 1R still get its filtered
 // colour in qcol (a provisional result the learned gate may keep), and every
 // hand-off entry gets its gate features in Q.features (row = list position).
-template <bool STATS, int CAP, int BLOCK, bool GATE = false>
+// RELAY (list mode over a hand-off list): entries already marked kNeeds2R
+// are appended to fail_list unchanged, the others are run.
+template <bool STATS, int CAP, int BLOCK, bool GATE = false, bool RELAY = false>
 __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, unsigned item, int (*lid)[BLOCK]) {
     const int tid = threadIdx.x;
     const int M = Q.M;
-    const bool active = slot < (Q.fast_list ? *Q.qcount : Q.num_queries);
-    const unsigned q = active ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
+    const bool listed = slot < (Q.fast_list ? *Q.qcount : Q.num_queries);
+    unsigned q = listed ? (Q.fast_list ? Q.qlist[slot] : slot) : 0u;
+    const bool relay = RELAY && listed && (q & kNeeds2R) != 0u;
+    if (RELAY) q &= ~kNeeds2R;
+    const bool active = listed && !relay;
     const unsigned pix = q / (unsigned)M;
     const int i = (int)(q - pix * (unsigned)M);
     const int px = (int)(pix % (unsigned)Q.W), py = (int)(pix / (unsigned)Q.W) + Q.y0;
@@ -1034,16 +1039,29 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             //     (ReconstructionOur.cpp:466-484);
             //  3. surface 0's samples in order: quadrant mask, nearest sample
             //     and tent-weighted sum (:486-499, :268-297).
+            // One loop per lane over (leaf, group of 4 samples) steps rather
+            // than a loop over leaves around a loop over a leaf's groups: a
+            // wave then runs as many steps as its busiest lane, not the sum
+            // over leaf positions of the lanes' largest leaf.  The next
+            // leaf's sample range is loaded a leaf ahead.
             uint64_t kmask = 0;
-            for (int li = 0; li < n; li++) {
-                const int id = lid[li][tid];
-                const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
-                s_in_leaves += (unsigned long long)(k1 - k0);
-                bool done_leaf = false;
+            {
+                int li = 0, kg = 0, k1 = 0, nk0 = 0, nk1 = 0;
+                if (n > 0) {
+                    const LeafRec& r = Q.leaves[lid[0][tid]];
+                    kg = r.s0;
+                    k1 = r.s1;
+                }
+                if (n > 1) {
+                    const LeafRec& r = Q.leaves[lid[1][tid]];
+                    nk0 = r.s0;
+                    nk1 = r.s1;
+                }
+                s_in_leaves += (unsigned long long)(k1 - kg);
                 // GATE: nearest sample (hardware-reciprocal estimate) of the
                 // leaves in front of the first one with a sample within R
                 float leaf_min = kFltMax;
-                for (int kg = k0; kg < k1 && !done_leaf; kg += 4) {
+                while (li < n) {
                     const float4* g = pab + 2 * (size_t)kg;
                     float4 ga[4], gb[4];
 #pragma unroll
@@ -1051,6 +1069,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                         ga[e] = g[2 * e];
                         gb[e] = g[2 * e + 1];
                     }
+                    bool hit = false;
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
                         if (kg + e >= k1) break;
@@ -1061,16 +1080,29 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                         const float dx = x - o.x, dy = y - o.y;
                         if (!(dx * dx + dy * dy <= R2)) continue;
                         kmask |= 1ull << li;
+                        hit = true;
                         s_kept++;
-                        if (!STATS) {
-                            done_leaf = true;
-                            break;
+                        if (!STATS) break;
+                    }
+                    kg += 4;
+                    // a leaf's scan stops at its first sample within R
+                    // unless every sample is counted
+                    if ((!STATS && hit) || kg >= k1) {
+                        if (GATE && kmask == 0) {
+                            g_front++;
+                            g_front_d2 = fminf(g_front_d2, leaf_min);
+                        }
+                        leaf_min = kFltMax;
+                        li++;
+                        kg = nk0;
+                        k1 = nk1;
+                        if (li < n) s_in_leaves += (unsigned long long)(k1 - kg);
+                        if (li + 1 < n) {
+                            const LeafRec& r = Q.leaves[lid[li + 1][tid]];
+                            nk0 = r.s0;
+                            nk1 = r.s1;
                         }
                     }
-                }
-                if (GATE && kmask == 0) {
-                    g_front++;
-                    g_front_d2 = fminf(g_front_d2, leaf_min);
                 }
             }
             // 2. surfaces; surface 0 = kept leaves in [first0, end0)
@@ -1105,13 +1137,23 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             float ar = 0, ag = 0, ab = 0, aw = 0;
             float min0 = kFltMax;
             float4 near = make_float4(0, 0, 0, 0);
-            for (uint64_t km = nsurf > 0 ? (kmask & ((end0 < 64 ? (1ull << end0) : 0ull) - 1ull)) : 0ull; km;
-                 km &= km - 1) {
-                const int li = __builtin_ctzll(km);
-                const int id = lid[li][tid];
-                const int k0 = Q.leaves[id].s0, k1 = Q.leaves[id].s1;
-                if (GATE) gacc.tested0 += k1 - k0;
-                for (int kg = k0; kg < k1; kg += 4) {
+            {
+                // one loop over (leaf, group) steps, as in pass 1
+                uint64_t km = nsurf > 0 ? (kmask & ((end0 < 64 ? (1ull << end0) : 0ull) - 1ull)) : 0ull;
+                int kg = 0, k1 = 0, nk0 = 0, nk1 = 0;
+                if (km) {
+                    const LeafRec& r = Q.leaves[lid[__builtin_ctzll(km)][tid]];
+                    kg = r.s0;
+                    k1 = r.s1;
+                    km &= km - 1;
+                }
+                if (km) {
+                    const LeafRec& r = Q.leaves[lid[__builtin_ctzll(km)][tid]];
+                    nk0 = r.s0;
+                    nk1 = r.s1;
+                }
+                if (GATE) gacc.tested0 += k1 - kg;
+                while (kg < k1) {
                     const float4* g = pab + 2 * (size_t)kg;
                     float4 ga[4], gb[4];
 #pragma unroll
@@ -1147,6 +1189,22 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                             ag += wgt * col.y;
                             ab += wgt * col.z;
                             aw += wgt * 1.0f;
+                        }
+                    }
+                    kg += 4;
+                    if (kg >= k1) {
+                        // next kept leaf of surface 0, if any (leaves hold
+                        // at least one sample, so kg < k1 again)
+                        if (km) {
+                            km &= km - 1;
+                            kg = nk0;
+                            k1 = nk1;
+                            if (GATE) gacc.tested0 += k1 - kg;
+                            if (km) {
+                                const LeafRec& r = Q.leaves[lid[__builtin_ctzll(km)][tid]];
+                                nk0 = r.s0;
+                                nk1 = r.s1;
+                            }
                         }
                     }
                 }
@@ -1193,7 +1251,8 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
         }
     }
     // failures go to the general kernel (one atomic per wave)
-    const bool hand = active && fail;
+    if (RELAY && relay) needs2r = true;
+    const bool hand = (active && fail) || relay;
     const uint64_t fm = __ballot(hand);
     if (fm) {
         const int lane = tid & 63;
@@ -1233,9 +1292,9 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             atomicAdd(&Q.stats[ST_FETCHED], v[2]);
             atomicAdd(&Q.stats[ST_SURFACES], v[3]);
             atomicAdd(&Q.stats[ST_COUNT], v[4]);
-            atomicAdd(&Q.stats[ST_FAST_LEAVES], v[0]);
-            atomicAdd(&Q.stats[ST_FAST_IN_LEAVES], v[1]);
-            atomicAdd(&Q.stats[ST_FAST_KEPT], v[2]);
+            atomicAdd(&Q.stats[RELAY ? ST_WIDE_LEAVES : ST_FAST_LEAVES], v[0]);
+            atomicAdd(&Q.stats[RELAY ? ST_WIDE_IN_LEAVES : ST_FAST_IN_LEAVES], v[1]);
+            atomicAdd(&Q.stats[RELAY ? ST_WIDE_KEPT : ST_FAST_KEPT], v[2]);
         }
     }
 }
@@ -1288,6 +1347,25 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     // Queries in scan order (pixel-major): the whole device sweeps one band
     // of rows, whose samples stay resident in the last-level cache.
     fast_query<STATS, kFastLeafCap, kFastBlock>(Q, tile_slot<kFastBlock>(Q, blockIdx.x, threadIdx.x), 0u, lid);
+}
+
+// Second 1R pass over the first pass's hand-off list, for the output samples
+// whose 1R leaves did not fit the first pass's lists: the same lane kernel
+// with CAP leaves per lane in 64-lane workgroups, persistent (each wave takes
+// the next 64 entries until the list is done); entries the first pass marked
+// for 2R are passed on unchanged.  Its hand-offs (2R fetches and output
+// samples with more than CAP leaves) go to fail_list for the general kernel.
+template <bool STATS, int CAP>
+__global__ __launch_bounds__(64) void k_query_fast_wide(QueryParams Q) {
+    __shared__ int lid[CAP][64];
+    const unsigned n = *Q.qcount;
+    for (;;) {
+        unsigned base = 0;
+        if (threadIdx.x == 0) base = atomicAdd(Q.next_query, 64u);
+        base = (unsigned)uni((int)base);
+        if (base >= n) break;
+        fast_query<STATS, CAP, 64, false, true>(Q, base + threadIdx.x, 0u, lid);
+    }
 }
 
 // Gated fast path (single buffer): as k_query_fast<false>, plus provisional
@@ -1519,6 +1597,24 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
     else
         k_query_fast<false><<<blocks, kFastBlock, 0, s>>>(p);
     return hipGetLastError();
+}
+
+hipError_t launch_query_fast_wide(const QueryParams& p, bool stats, int blocks, hipStream_t s) {
+    if (p.num_queries == 0) return hipSuccess;
+    if (stats)
+        k_query_fast_wide<true, kWideLeafCap><<<blocks, 64, 0, s>>>(p);
+    else
+        k_query_fast_wide<false, kWideLeafCap><<<blocks, 64, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+int fast_wide_blocks_per_cu() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_query_fast_wide<false, kWideLeafCap>, 64, 0) !=
+            hipSuccess ||
+        b <= 0)
+        b = 1;
+    return b;
 }
 
 hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s) {

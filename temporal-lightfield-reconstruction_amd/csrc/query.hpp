@@ -25,12 +25,14 @@ constexpr unsigned kNeeds2R = 0x80000000u;
 // query ids stay below 2^31 - 1, so it never names a query.
 constexpr unsigned kSkipEntry = 0xFFFFFFFFu;
 
-constexpr int kNumStats = 13;
+constexpr int kNumStats = 16;
 enum StatIdx {
     ST_LEAF_NODES = 0, ST_SAMPLES_IN_LEAVES, ST_WITHIN_1R, ST_FETCHED, ST_SURFACES, ST_2R, ST_MERGED,
     ST_AT_LEAST_ONE, ST_COUNT, ST_OVERFLOW,
     // the fast kernel's share of the first three (its 1R passes)
-    ST_FAST_LEAVES, ST_FAST_IN_LEAVES, ST_FAST_KEPT
+    ST_FAST_LEAVES, ST_FAST_IN_LEAVES, ST_FAST_KEPT,
+    // the second (wide-list) 1R pass's share
+    ST_WIDE_LEAVES, ST_WIDE_IN_LEAVES, ST_WIDE_KEPT
 };
 
 // Per-wave scratch words (int/float) for the general query kernel.
@@ -132,10 +134,20 @@ struct BatchQuery {
 // more 1R leaves go to the general kernel.  22 ids x 128 lanes = 11 KB of LDS
 // per block, so the 7 waves per SIMD its registers allow (14 blocks) fit the
 // CU's 160 KB (24 would hold 13 blocks).
-constexpr int kFastLeafCap = 22;
+#ifndef TLR_FAST_LEAF_CAP
+#define TLR_FAST_LEAF_CAP 22
+#endif
+constexpr int kFastLeafCap = TLR_FAST_LEAF_CAP;
 constexpr int kFastBlock = 128;   // block size of the single-buffer fast kernel
 static_assert(kFastLeafCap <= 64, "the fast kernel keeps a 64-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
+// Per-lane leaf list of the second 1R pass (k_query_fast_wide) over the
+// first pass's hand-offs that had more than kFastLeafCap leaves.
+#ifndef TLR_WIDE_LEAF_CAP
+#define TLR_WIDE_LEAF_CAP 40
+#endif
+constexpr int kWideLeafCap = TLR_WIDE_LEAF_CAP;
+static_assert(kWideLeafCap > kFastLeafCap && kWideLeafCap <= 64, "the wide pass keeps a 64-bit leaf mask");
 
 // Batched launches (tlr_reconstruct_batch).  The fast kernel comes in
 // kNumFastVariants (leaf cap, block size) variants, chosen by the autotuner.
@@ -161,6 +173,12 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
 // with a first surface at 1R, its filtered colour in qcol (provisional) and,
 // for every hand-off entry, its gate features in p.features.
 hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s);
+// Second 1R pass (kWideLeafCap leaves per lane, persistent 64-lane blocks)
+// over p.qlist[0 .. *p.qcount), the first pass's hand-offs; p.next_query is
+// its work counter (zeroed before launch); its hand-offs, with the entries
+// already marked kNeeds2R, go to p.fail_list.
+hipError_t launch_query_fast_wide(const QueryParams& p, bool stats, int blocks, hipStream_t s);
+int fast_wide_blocks_per_cu();
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out_rgba, hipStream_t s);
 // Writes the device's constant-rate wall clock (hipDeviceAttributeWallClockRate
 // kHz) to *dst when the stream reaches it (also inside a captured graph).

@@ -119,6 +119,12 @@ struct tlr_ctx {
     unsigned* d_counter = nullptr;
     unsigned* d_fail_count = nullptr;
     unsigned* fail_list = nullptr;
+    // second 1R pass (wide leaf lists) over fail_list: its work counter, its
+    // hand-off list and count (what the general kernel then takes)
+    unsigned* d_wide_next = nullptr;
+    unsigned* d_fail_count2 = nullptr;
+    unsigned* fail_list2 = nullptr;
+    int wide_blocks = 0;
     float* scratch = nullptr;
     int num_waves = 0;
     int general_wpc = 0;  // resident general-kernel waves per CU
@@ -226,6 +232,8 @@ tlr_status tlr_create(tlr_ctx** out, int device) {
         hipMalloc(&c->d_stats, sizeof(unsigned long long) * kNumStats) != hipSuccess ||
         hipMalloc(&c->d_counter, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&c->d_fail_count, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->d_wide_next, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&c->d_fail_count2, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&c->d_accept_count, sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&c->gate_p, sizeof(float) * kGateParams) != hipSuccess ||
         hipMalloc(&c->gate_m, sizeof(float) * kGateParams) != hipSuccess ||
@@ -255,6 +263,8 @@ void tlr_destroy(tlr_ctx* c) {
     if (c->pext) (void)hipFree(c->pext);
     if (c->d_stamps) (void)hipFree(c->d_stamps);
     if (c->d_fail_count) (void)hipFree(c->d_fail_count);
+    if (c->d_wide_next) (void)hipFree(c->d_wide_next);
+    if (c->d_fail_count2) (void)hipFree(c->d_fail_count2);
     if (c->d_accept_count) (void)hipFree(c->d_accept_count);
     for (float* p : {c->gate_p, c->gate_m, c->gate_v, c->gate_g, c->gate_loss, c->train_X, c->train_y})
         if (p) (void)hipFree(p);
@@ -597,7 +607,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     // persistent general kernel: as many waves as can be resident
     if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
     const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
-    size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + nq * 4 +
+    size_t qbytes = pat.size() * 4 + nq * sizeof(float4) + (size_t)W * H * 16 + 2 * nq * 4 +
                     (size_t)waves * kWaveScratchWords * 4 + 16 * 256;
     if (gate_buffers) qbytes += nq * 4 * ((size_t)kGateFeatures + 2) + nq * sizeof(float4) + 5 * 256;
     if (debug) qbytes += nq * sizeof(float2) + (size_t)W * H * 16 + 2 * 256;
@@ -607,6 +617,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     c->qcol = c->qarena.alloc<float4>(nq);
     c->d_img = c->qarena.alloc<float>((size_t)W * H * 4);
     c->fail_list = c->qarena.alloc<unsigned>(nq);
+    c->fail_list2 = c->qarena.alloc<unsigned>(nq);
     c->scratch = c->qarena.alloc<float>((size_t)waves * kWaveScratchWords);
     c->features = c->logits = c->labels = nullptr;
     c->prov = nullptr;
@@ -632,6 +643,9 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
     HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
     HIPCHK(c, hipMemsetAsync(c->d_accept_count, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(c->d_wide_next, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(c->d_fail_count2, 0, sizeof(unsigned), s));
+    if (c->wide_blocks == 0) c->wide_blocks = c->prop.multiProcessorCount * fast_wide_blocks_per_cu();
 
     memset(&Q, 0, sizeof(Q));
     Q.W = W; Q.H = H; Q.M = M;
@@ -664,9 +678,11 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
 }
 
 // Plain method over all queries (or over Q.qlist when list != nullptr):
-// fast 1R lane kernel, then the general kernel for what it hands over.
+// fast 1R lane kernel, then (wide) the second 1R lane pass with the
+// context's wide lists over its hand-offs, then the general kernel for what
+// is left.
 static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* list, const unsigned* count,
-                            bool stats_mode, int path) {
+                            bool stats_mode, int path, bool wide = false) {
     hipStream_t s = c->stream;
     if (path == 1 && !list) {
         HIPCHK(c, launch_query_general(Q0, s));
@@ -680,12 +696,25 @@ static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* l
         F.qcount = count;
     }
     HIPCHK(c, launch_query_fast(F, stats_mode, s));
-    HIPCHK(c, hipEventRecord(c->ev[5], s));
     // the general kernel takes the hand-offs from the list the fast kernel
     // just filled (Q0's, which need not be the context's own)
     QueryParams G = Q0;
     G.qlist = Q0.fail_list;
     G.qcount = Q0.fail_count;
+    if (wide) {
+        QueryParams V = Q0;
+        V.fast_list = 1;
+        V.qlist = Q0.fail_list;
+        V.qcount = Q0.fail_count;
+        V.next_query = c->d_wide_next;
+        V.fail_list = c->fail_list2;
+        V.fail_count = c->d_fail_count2;
+        HIPCHK(c, hipEventRecord(c->ev[7], s));
+        HIPCHK(c, launch_query_fast_wide(V, stats_mode, c->wide_blocks, s));
+        G.qlist = c->fail_list2;
+        G.qcount = c->d_fail_count2;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[5], s));
     HIPCHK(c, launch_query_general(G, s));
     return TLR_OK;
 }
@@ -842,7 +871,7 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     } else if (p->path == 2) {
         st = run_refgpu(c, Q, p->list_cap, &n2r);
     } else {
-        st = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0 || p->debug_image != 0, p->path);
+        st = run_plain(c, Q, nullptr, nullptr, p->collect_stats != 0 || p->debug_image != 0, p->path, true);
     }
     if (st != TLR_OK) return st;
     if (p->debug_image) {
@@ -858,7 +887,11 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     unsigned long long hs[kNumStats];
     unsigned nfail = 0, naccept = 0;
     HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(&nfail, c->d_fail_count, sizeof(nfail), hipMemcpyDeviceToHost, s));
+    const bool wide = !p->gate && p->path == 0;
+    unsigned nwide = 0;
+    HIPCHK(c, hipMemcpyAsync(&nfail, wide ? c->d_fail_count2 : c->d_fail_count, sizeof(nfail),
+                             hipMemcpyDeviceToHost, s));
+    if (wide) HIPCHK(c, hipMemcpyAsync(&nwide, c->d_fail_count, sizeof(nwide), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(&naccept, c->d_accept_count, sizeof(naccept), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (stats) {
@@ -876,7 +909,15 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         stats->fast_leaf_nodes = (double)hs[ST_FAST_LEAVES];
         stats->fast_samples_in_leaf_nodes = (double)hs[ST_FAST_IN_LEAVES];
         stats->fast_samples_within_1r = (double)hs[ST_FAST_KEPT];
+        stats->wide_leaf_nodes = (double)hs[ST_WIDE_LEAVES];
+        stats->wide_samples_in_leaf_nodes = (double)hs[ST_WIDE_IN_LEAVES];
+        stats->wide_samples_within_1r = (double)hs[ST_WIDE_KEPT];
         float ms = 0;
+        if (wide) {
+            // entries the first pass had marked for 2R are relayed, not run
+            stats->wide_queries = (double)nwide;
+            if (hipEventElapsedTime(&ms, c->ev[7], c->ev[5]) == hipSuccess) stats->wide_ms = ms;
+        }
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;

@@ -63,7 +63,9 @@ def test_batch_forms_equal_single_buffer_path(tlr, batch, mode):
         single.set_samples(sb)
         single.build()
         ref, sst = single.reconstruct(out_spp=BUFS[k][4])
-        handed += sst["general_queries"]
+        # the first 1R lane pass's hand-offs (the entries the single-buffer
+        # path's second lane pass takes)
+        handed += sst["wide_queries"]
         assert np.array_equal(imgs[k], ref), "form %d: buffer %d differs" % (mode, k)
         assert np.array_equal(imgs2[k], ref), "form %d replay: buffer %d differs" % (mode, k)
     if mode == 1:

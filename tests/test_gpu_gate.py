@@ -63,7 +63,9 @@ def test_gate_training_reduces_loss(tlr, ctx):
     c.build()
     _, st = c.reconstruct(out_spp=8)
     n = c.gate_append_training(out_spp=8, stride=1)
-    assert n == st["general_queries"] > 0  # one row per hand-off of the 1R lane kernel
+    # one row per hand-off of the first 1R lane pass (the entries the plain
+    # path's second lane pass takes)
+    assert n == st["wide_queries"] > 0
     losses = c.gate_train(epochs=6, batch=512, lr=2e-3, seed=1)
     print("training losses", losses, flush=True)
     assert losses[-1] < losses[0]
@@ -87,7 +89,9 @@ def test_collected_rows_are_the_hand_offs(collected):
     method's colour of that output sample, bit for bit; the label says
     whether the provisional colour is within 0.001 of it in every channel."""
     sb, c, plain, st, qplain, (X, y, prov, pl, q) = collected
-    assert len(X) == st["general_queries"] > 0
+    # the hand-offs of the first 1R lane pass (the plain path's second lane
+    # pass takes the same entries)
+    assert len(X) == st["wide_queries"] > 0
     assert len(np.unique(q)) == len(q) and q.max() < qplain.shape[0]
     assert np.array_equal(pl.view(np.uint32), qplain[q].view(np.uint32))
     has = X[:, 0] >= 0
@@ -108,7 +112,7 @@ def test_gate_thresholds_bound_the_gated_path(tlr, collected):
     c.gate_set_threshold(float("inf"))
     g, sg = c.reconstruct(out_spp=8, gate=True)
     assert np.array_equal(g, plain)
-    assert sg["gate_candidates"] == st["general_queries"] == sg["gate_rejected"]
+    assert sg["gate_candidates"] == st["wide_queries"] == sg["gate_rejected"]
     c.gate_set_threshold(-float("inf"))
     g, sg = c.reconstruct(out_spp=8, gate=True)
     has = X[:, 0] >= 0

@@ -1,11 +1,15 @@
-"""GPU: the general kernel past its LDS capacities.  The general kernel keeps
-the first leaves of a gather (keys, ids, sample ranges) and the traversal
-frontier in LDS and continues in the wave's scratch memory past them.  The
-test build lib/libtlr_small_lds.so sets both capacities to their minimum (4
-staged leaves, a 64-entry frontier), so those continuations run on almost
-every output sample.  Its per-output-sample colours and Stats must equal the
-oracle's bit for bit, with every output sample sent through the general
-kernel (path 1) and through the plain split (path 0).
+"""GPU: the general kernel past its LDS capacities, and the plain split's
+three levels.  The general kernel keeps the first leaves of a gather (keys,
+ids, sample ranges) and the traversal frontier in LDS and continues in the
+wave's scratch memory past them.  The test build lib/libtlr_small_lds.so sets
+both capacities to their minimum (4 staged leaves, a 64-entry frontier), so
+those continuations run on almost every output sample; it also cuts the 1R
+lane passes' per-lane leaf lists to 4 and 8, so that the plain split (path 0)
+sends many output samples through the second (wide-list) lane pass, and the
+ones with more than 8 leaves on to the general kernel.  Its per-output-sample
+colours and Stats must equal the oracle's bit for bit, with every output
+sample sent through the general kernel (path 1) and through the plain split
+(path 0).
 
 The library is loaded through TLR_LIBRARY in a child process (one process,
 both cases), since a process holds one libtlr."""
@@ -48,9 +52,16 @@ def _child():
             q = ctx.query_colors(M)
             nq = int(np.count_nonzero(np.any(q.view(np.uint32) != qref.astype(np.float32).view(np.uint32), axis=-1)))
             ns = [k for k in STATS if st[k] != rst[k]]
-            print("%s path %d: output samples differing %d / %d, stats differing %s, overflow %d" % (
-                name, path, nq, q.size // 4, ns, st["overflow"]), flush=True)
+            print("%s path %d: output samples differing %d / %d, stats differing %s, overflow %d, "
+                  "wide-pass entries %d, general kernel %d" % (
+                      name, path, nq, q.size // 4, ns, st["overflow"], st["wide_queries"], st["general_queries"]),
+                  flush=True)
             bad += nq + len(ns) + int(st["overflow"])
+            if path == 0:
+                # all three levels ran: the wide pass over hand-offs, the
+                # general kernel over what it handed on
+                bad += int(st["wide_queries"] == 0) + int(st["general_queries"] == 0)
+                bad += int(st["general_queries"] >= st["wide_queries"])
     sys.exit(1 if bad else 0)
 
 
