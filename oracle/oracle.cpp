@@ -909,24 +909,37 @@ void build_leaf_cell(orc_ctx* c, int inode, int offset, LeafScratch& sc, std::ve
 }
 
 // Hierarchy over the valid leaves: a balanced binary tree of recursive
-// median splits.  Each node splits its leaves at the median of the feature
-// with the widest pixel extent among: centre x, centre y at (t=0.5, u=v=0),
-// 2x the CoC slope (u spans 2), and the t-slopes in x and y.  Grouping leaves
-// of similar slopes keeps the merged planes tight at u=+-1 and t=0,1.  The
-// order inside a node is a stable sort on the feature quantised to
-// kd_qbits(level) bits; internal node g is the one whose split falls between
-// sorted positions g and g+1.  (Replaces emitNodes, ReconstructionTreeBuilder
-// .cpp:409-451; which leaves a query finds does not depend on the hierarchy.)
+// median splits (this build's own rule; it replaces emitNodes,
+// ReconstructionTreeBuilder.cpp:409-451, and which leaves a query finds does
+// not depend on it).  Each node splits its leaves at the median of the
+// feature of widest extent among a leaf's centre in x at the (t, u) corners
+// (0,-1), (1,-1), (1,1), (0,1) and its centre in y at the same (t, v)
+// corners.  The order inside a node is a stable sort on that feature
+// quantised to kd_qbits(level) bits; internal node g is the one whose split
+// falls between sorted positions g and g+1.
+constexpr int kKdFeat = 8;
 struct KdFeat {
-    float f[5];
+    float f[kKdFeat];
 };
+// lower/upper x and y bounds of a leaf at the four corners
+void kd_corners(const TLB& b, float xl[4], float xh[4], float yl[4], float yh[4]) {
+    const float ct[4] = {0.0f, 1.0f, 1.0f, 0.0f};
+    const float cu[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
+    for (int c = 0; c < 4; c++) {
+        xl[c] = b.eval(0, ct[c], cu[c]);
+        xh[c] = b.eval(1, ct[c], cu[c]);
+        yl[c] = b.eval(2, ct[c], cu[c]);
+        yh[c] = b.eval(3, ct[c], cu[c]);
+    }
+}
 KdFeat kd_feature(const TLB& b) {
+    float xl[4], xh[4], yl[4], yh[4];
+    kd_corners(b, xl, xh, yl, yh);
     KdFeat k;
-    k.f[0] = 0.5f * (b.eval(0, 0.5f, 0.0f) + b.eval(1, 0.5f, 0.0f));
-    k.f[1] = 0.5f * (b.eval(2, 0.5f, 0.0f) + b.eval(3, 0.5f, 0.0f));
-    k.f[2] = b.p[0][1] + b.p[1][1];
-    k.f[3] = 0.5f * (b.p[0][0] + b.p[1][0]);
-    k.f[4] = 0.5f * (b.p[2][0] + b.p[3][0]);
+    for (int c = 0; c < 4; c++) {
+        k.f[c] = 0.5f * (xl[c] + xh[c]);
+        k.f[4 + c] = 0.5f * (yl[c] + yh[c]);
+    }
     return k;
 }
 inline int kd_qbits(int level) { return std::min(16, 32 - level); }
@@ -942,19 +955,19 @@ inline uint32_t kd_quant(float v, float lo, float hi, int bits) {
 
 void kd_order(std::vector<int>& ord, const std::vector<KdFeat>& F, int lo, int hi, int level) {
     if (hi - lo < 2) return;
-    float mn[5], mxv[5];
-    for (int d = 0; d < 5; d++) {
+    float mn[kKdFeat], mxv[kKdFeat];
+    for (int d = 0; d < kKdFeat; d++) {
         mn[d] = F[ord[lo]].f[d];
         mxv[d] = mn[d];
     }
     for (int k = lo + 1; k < hi; k++)
-        for (int d = 0; d < 5; d++) {
+        for (int d = 0; d < kKdFeat; d++) {
             mn[d] = std::min(mn[d], F[ord[k]].f[d]);
             mxv[d] = std::max(mxv[d], F[ord[k]].f[d]);
         }
     int best = 0;
     float bext = mxv[0] - mn[0];
-    for (int d = 1; d < 5; d++)
+    for (int d = 1; d < kKdFeat; d++)
         if (mxv[d] - mn[d] > bext) {
             bext = mxv[d] - mn[d];
             best = d;

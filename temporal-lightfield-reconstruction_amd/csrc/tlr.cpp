@@ -440,7 +440,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     words += 16 * (size_t)N;          // leaf bounds
     words += 2 * (size_t)N;           // valid flag / pos
     words += 4 * (size_t)N;           // hierarchy keys / values ping-pong
-    words += 21 * (size_t)N;          // leaf features (5), leaf of valid id, split info (4), extents (10), slot ids
+    words += 30 * (size_t)N;          // leaf features (8), leaf of valid id, split info (4), extents (16), slot ids
     words += 7 * (size_t)N;           // child0/1, parent(2N), range(2N), split owner
     words += 16 * (size_t)N + (size_t)N;  // node bounds, flags
     words += 32 * (size_t)N;          // traversal nodes (2N x 16 words)
@@ -558,15 +558,15 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
 
     // ---- median-split hierarchy over the valid leaves ----
     const int nv1 = std::max(nv, 1);
-    float* feat = A.alloc<float>(5 * (size_t)nv1);
+    float* feat = A.alloc<float>(8 * (size_t)nv1);
     uint32_t* vid_leaf = A.alloc<uint32_t>(nv1);
     uint32_t* tk0 = A.alloc<uint32_t>(nv1);
     uint32_t* tv0 = A.alloc<uint32_t>(nv1);
     uint32_t* tk1 = A.alloc<uint32_t>(nv1);
     uint32_t* tv1 = A.alloc<uint32_t>(nv1);
     float4* kinfo = A.alloc<float4>(nv1);
-    uint32_t* kmn = A.alloc<uint32_t>(5 * (size_t)nv1);
-    uint32_t* kmx = A.alloc<uint32_t>(5 * (size_t)nv1);
+    uint32_t* kmn = A.alloc<uint32_t>(8 * (size_t)nv1);
+    uint32_t* kmx = A.alloc<uint32_t>(8 * (size_t)nv1);
     c->tree_ids = A.alloc<uint32_t>(nv1);
     c->tree_c0 = A.alloc<int>(nv1);
     c->tree_c1 = A.alloc<int>(nv1);
