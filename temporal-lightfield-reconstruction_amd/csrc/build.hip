@@ -1045,6 +1045,40 @@ __global__ __launch_bounds__(kBlock) void k_kd_bounds_level(const uint32_t* __re
     node_b[g] = r;
 }
 
+// Child-pair records of the internal nodes, from the traversal records.
+__global__ __launch_bounds__(kBlock) void k_tree_pairs(const TravNode* __restrict__ trav, int n,
+                                                       const LeafRec* __restrict__ leaves,
+                                                       TravPair* __restrict__ pairs) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n - 1) return;
+    const TravNode& t = trav[g];
+    const TravNode& ta = trav[t.child];
+    const TravNode& tb = trav[t.child1];
+    TravPair r;
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+        r.a[q] = ta.p[q];
+        r.b[q] = tb.p[q];
+    }
+    // a leaf slot's record holds -(leaf id + 1); an internal node is its index
+    r.ra = ta.child < 0 ? ta.child : t.child;
+    r.rb = tb.child < 0 ? tb.child : t.child1;
+    r.ka[0] = r.ka[1] = r.ka[2] = r.kb[0] = r.kb[1] = r.kb[2] = 0.0f;
+    if (r.ra < 0) {
+        const LeafRec& l = leaves[-r.ra - 1];
+        r.ka[0] = l.w0;
+        r.ka[1] = l.t0;
+        r.ka[2] = l.mvw0;
+    }
+    if (r.rb < 0) {
+        const LeafRec& l = leaves[-r.rb - 1];
+        r.kb[0] = l.w0;
+        r.kb[1] = l.t0;
+        r.kb[2] = l.mvw0;
+    }
+    pairs[g] = r;
+}
+
 // Traversal records with escape links (next node after a subtree).
 __global__ __launch_bounds__(kBlock) void k_tree_trav(const uint32_t* __restrict__ ids, int n,
                                                       const Bounds* __restrict__ leaf_b,
@@ -1198,8 +1232,10 @@ hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, 
 }
 
 void launch_tree_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
-                      const int* c1, const int2* range, const int* split_owner, TravNode* trav, hipStream_t s) {
+                      const int* c1, const int2* range, const int* split_owner, const LeafRec* leaves,
+                      TravNode* trav, TravPair* pairs, hipStream_t s) {
     if (n > 0) k_tree_trav<<<TLR_GRID(2 * n - 1), 0, s>>>(ids, n, leaf_b, node_b, c0, c1, range, split_owner, trav);
+    if (n > 1) k_tree_pairs<<<TLR_GRID(n - 1), 0, s>>>(trav, n, leaves, pairs);
 }
 
 }  // namespace tlr

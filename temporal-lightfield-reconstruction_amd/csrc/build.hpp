@@ -39,7 +39,9 @@ hipError_t launch_kdtree(const uint32_t* valid_flag, const uint32_t* valid_pos, 
                          int nv, float* feat, uint32_t* vid_leaf, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                          uint32_t* v1, float4* info, uint32_t* emn, uint32_t* emx, uint32_t* hist, uint32_t* tmp, uint32_t* ids, int* c0, int* c1,
                          int* parent, int2* range, int* split_owner, Bounds* node_b, hipStream_t s);
+// Traversal records (2n - 1) and the internal nodes' child-pair records (n - 1).
 void launch_tree_trav(const uint32_t* ids, int n, const Bounds* leaf_b, const Bounds* node_b, const int* c0,
-                      const int* c1, const int2* range, const int* split_owner, TravNode* trav, hipStream_t s);
+                      const int* c1, const int2* range, const int* split_owner, const LeafRec* leaves,
+                      TravNode* trav, TravPair* pairs, hipStream_t s);
 
 }  // namespace tlr

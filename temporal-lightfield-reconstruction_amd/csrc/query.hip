@@ -942,10 +942,10 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
     // if the lane hits the node and the child; when no lane would enter one
     // of the two children, the wave moves to the other, keeping the lanes
     // that hit the node (`alive`: hit every ancestor of the current node).
-    // Node loads are uniform.  Each lane then walks the subtree of `start`
-    // exactly as from the root, stopping at the subtree's escape link.
+    // Node loads are uniform.  Each lane then walks the subtree of `start`,
+    // finding exactly the leaves a walk from the root would.
     bool alive = active;
-    int start = -1, stop = -1;
+    int start = -1;
     if (Q.n_nodes > 0) {
         int node = Q.root;
         for (;;) {
@@ -964,38 +964,117 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             alive = hn;
         }
         start = node;
-        stop = Q.trav[node].escape;
     }
     if (active) {
-        // --- stackless traversal; the LDS column holds (id, key) pairs while
-        // it has room for them (CAP / 2 leaves), ids only after that (keys
-        // then recomputed from the leaf records on compare) ---
+        // --- depth-first walk of start's subtree over child-pair records:
+        // a node's record holds both children's bounds, so the lane tests
+        // them there and loads a child's record only when it enters it;
+        // a second hit child waits on a short stack held in registers (a
+        // query that would overflow it is handed over).  Leaves found are
+        // appended to the LDS column, (id, key) pairs while it has room for
+        // them (CAP / 2 leaves), ids only after that, and sorted by
+        // (key, id) once the walk is done: sorting inside the walk would make
+        // every lane of the wave wait for the insertions of whichever lane
+        // just found a leaf, at almost every step ---
         int n = 0;
         bool pairs = true;
-        int node = alive ? start : -1;
-        while (node >= 0 && node != stop) {
-            const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + node);
+        int stk[kWalkStack];
+#pragma unroll
+        for (int k = 0; k < kWalkStack; k++) stk[k] = -1;
+        int sp = 0;
+        int node = -1;
+        // leaves to append after the current step, with their key terms
+        int fl[2] = {-1, -1};
+        float fk[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        if (alive) {
+            // the walk's first node, tested from its own record
+            const float4* tn4 = reinterpret_cast<const float4*>(Q.trav + start);
             const float4 a = tn4[0], b = tn4[1], c = tn4[2], d = tn4[3];
             const float p[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-            const int child = __float_as_int(d.x), escape = __float_as_int(d.y);
+            const int child = __float_as_int(d.x);
             if (node_hit(p, o, R)) {
                 if (child >= 0) {
-                    node = child;
-                    continue;
+                    node = start;
+                } else {
+                    fl[0] = -child - 1;
+                    const LeafRec& lr = Q.leaves[fl[0]];
+                    fk[0][0] = lr.w0;
+                    fk[0][1] = lr.t0;
+                    fk[0][2] = lr.mvw0;
                 }
+            }
+        }
+        for (;;) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int id = fl[c];
+                if (id < 0 || fail) continue;
                 if (n == CAP) {
                     fail = true;
-                    break;
+                    continue;
                 }
-                const int id = -child - 1;
-                const float key = leaf_key(Q.leaves, id, o.t);
                 if (pairs && n == CAP / 2) {
                     // the pairs fill the column: keep the ids only
                     for (int j = 1; j < n; j++) lid[j][tid] = lid[2 * j][tid];
                     pairs = false;
                 }
-                int j = n;
                 if (pairs) {
+                    lid[2 * n][tid] = id;
+                    // ReconstructionOur.cpp:384-385
+                    lid[2 * n + 1][tid] = __float_as_int(fk[c][0] + (o.t - fk[c][1]) * fk[c][2]);
+                } else {
+                    lid[n][tid] = id;
+                }
+                n++;
+            }
+            if (fail || node < 0) break;
+            const float4* pr4 = reinterpret_cast<const float4*>(Q.pairs + node);
+            const float4 a0 = pr4[0], a1 = pr4[1], a2 = pr4[2];
+            const float4 b0 = pr4[3], b1 = pr4[4], b2 = pr4[5];
+            const float4 r0 = pr4[6], r1 = pr4[7];
+            const float pa[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+            const float pb[12] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+            const int ra = __float_as_int(r0.x), rb = __float_as_int(r0.y);
+            const bool ha = node_hit(pa, o, R), hb = node_hit(pb, o, R);
+            fl[0] = (ha && ra < 0) ? -ra - 1 : -1;
+            fl[1] = (hb && rb < 0) ? -rb - 1 : -1;
+            fk[0][0] = r0.z;
+            fk[0][1] = r0.w;
+            fk[0][2] = r1.x;
+            fk[1][0] = r1.y;
+            fk[1][1] = r1.z;
+            fk[1][2] = r1.w;
+            const bool ea = ha && ra >= 0, eb = hb && rb >= 0;
+            if (ea && eb) {
+                if (sp == kWalkStack) {
+                    fail = true;
+                    break;
+                }
+#pragma unroll
+                for (int k = kWalkStack - 1; k > 0; k--) stk[k] = stk[k - 1];
+                stk[0] = rb;
+                sp++;
+                node = ra;
+            } else if (ea) {
+                node = ra;
+            } else if (eb) {
+                node = rb;
+            } else if (sp > 0) {
+                node = stk[0];
+#pragma unroll
+                for (int k = 0; k < kWalkStack - 1; k++) stk[k] = stk[k + 1];
+                sp--;
+            } else {
+                node = -1;
+            }
+        }
+        if (!fail) {
+            // sort by (key, id), leaving the ids in lid[0 .. n)
+            if (pairs) {
+                for (int i = 1; i < n; i++) {
+                    const int id = lid[2 * i][tid];
+                    const float key = __int_as_float(lid[2 * i + 1][tid]);
+                    int j = i;
                     while (j > 0) {
                         const int ii = lid[2 * j - 2][tid];
                         const float kk = __int_as_float(lid[2 * j - 1][tid]);
@@ -1006,7 +1085,13 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                     }
                     lid[2 * j][tid] = id;
                     lid[2 * j + 1][tid] = __float_as_int(key);
-                } else {
+                }
+                for (int j = 1; j < n; j++) lid[j][tid] = lid[2 * j][tid];
+            } else {
+                for (int i = 1; i < n; i++) {
+                    const int id = lid[i][tid];
+                    const float key = leaf_key(Q.leaves, id, o.t);
+                    int j = i;
                     while (j > 0) {
                         const int ii = lid[j - 1][tid];
                         const float kk = leaf_key(Q.leaves, ii, o.t);
@@ -1016,12 +1101,8 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                     }
                     lid[j][tid] = id;
                 }
-                n++;
             }
-            node = escape;
         }
-        if (pairs)
-            for (int j = 1; j < n; j++) lid[j][tid] = lid[2 * j][tid];
         if (!fail) {
             s_leaves = (unsigned long long)n;
             const float RANGE = 1.0f / sqrtf(128.0f);

@@ -55,6 +55,7 @@ struct QueryParams {
     unsigned num_queries;
     const float* pattern;     // 64*M templates: x,y,u,v,t
     const TravNode* trav;
+    const TravPair* pairs;    // child pairs of the internal nodes (n_nodes / 2 records)
     const LeafRec* leaves;
     const Bounds* leaf_b;
     Points pts;
@@ -141,6 +142,8 @@ constexpr int kFastLeafCap = TLR_FAST_LEAF_CAP;
 constexpr int kFastBlock = 128;   // block size of the single-buffer fast kernel
 static_assert(kFastLeafCap <= 64, "the fast kernel keeps a 64-bit mask over its leaves");
 constexpr int kGateFeatures = 24;
+// Pending hit children a 1R lane walk holds (a deeper one is handed over).
+constexpr int kWalkStack = 8;
 // Per-lane leaf list of the second 1R pass (k_query_fast_wide) over the
 // first pass's hand-offs that had more than kFastLeafCap leaves.
 #ifndef TLR_WIDE_LEAF_CAP

@@ -49,6 +49,7 @@ struct BatchSlot {
     LeafRec* leaves = nullptr;
     Bounds* leaf_b = nullptr;
     TravNode* trav = nullptr;
+    TravPair* pairs = nullptr;
     float* pattern = nullptr;
     float4* qcol = nullptr;
     float* img = nullptr;
@@ -108,6 +109,7 @@ struct tlr_ctx {
     int* tree_c1 = nullptr;
     Bounds* tree_b = nullptr;
     TravNode* trav = nullptr;
+    TravPair* pairs = nullptr;
 
     // reconstruct
     DeviceArena qarena;
@@ -166,7 +168,7 @@ struct tlr_ctx {
     int* train_idx = nullptr;
     int64_t train_n = 0, train_cap = 0;
 
-    hipEvent_t ev[8];
+    hipEvent_t ev[9];
     bool events = false;
 
     // batched reconstruction
@@ -444,6 +446,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     words += 7 * (size_t)N;           // child0/1, parent(2N), range(2N), split owner
     words += 16 * (size_t)N + (size_t)N;  // node bounds, flags
     words += 32 * (size_t)N;          // traversal nodes (2N x 16 words)
+    words += 32 * (size_t)N;          // child-pair records (N x 32 words)
     words += (size_t)W * H;           // hits
     words += 7 * cells_max + 2 * (size_t)kMaxGridSide + 16;  // counts, cell start, sat, owner(4), flags, y1
     words += (size_t)N;               // initial-leaf idx (<= cells)
@@ -575,13 +578,14 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     int* split_owner = A.alloc<int>(nv1);
     c->tree_b = A.alloc<Bounds>(nv1);
     c->trav = A.alloc<TravNode>(2 * (size_t)nv1);
+    c->pairs = A.alloc<TravPair>(nv1);
     if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (hierarchy)");
     if (nv > 0) {
         HIPCHK(c, hipMemsetAsync(parent, 0xFF, sizeof(int) * (2 * (size_t)nv), s));
         HIPCHK(c, launch_kdtree(vflag, vpos, L, c->leaf_b, nv, feat, vid_leaf, tk0, tv0, tk1, tv1, kinfo, kmn, kmx, hist, stmp,
                                 c->tree_ids, c->tree_c0, c->tree_c1, parent, range, split_owner, c->tree_b, s));
-        launch_tree_trav(c->tree_ids, nv, c->leaf_b, c->tree_b, c->tree_c0, c->tree_c1, range, split_owner, c->trav,
-                         s);
+        launch_tree_trav(c->tree_ids, nv, c->leaf_b, c->tree_b, c->tree_c0, c->tree_c1, range, split_owner,
+                         c->leaves, c->trav, c->pairs, s);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[1], s));
@@ -659,6 +663,7 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     Q.num_queries = (unsigned)nq;
     Q.pattern = c->d_pattern;
     Q.trav = c->trav;
+    Q.pairs = c->pairs;
     Q.leaves = c->leaves;
     Q.leaf_b = c->leaf_b;
     Q.pts = c->pts;
@@ -674,6 +679,25 @@ static tlr_status prepare_queries(tlr_ctx* c, const tlr_recon_params* p, bool ga
     // index is 32-bit, so very narrow images fall back to query order
     Q.tile_order = TLR_TILE_ORDER;
     if (fast_slots(Q) >= ((size_t)1 << 32)) Q.tile_order = 0;
+    return TLR_OK;
+}
+
+// The second 1R lane pass (wide leaf lists) over the context's hand-off
+// list, into its second list; event 7 marks its start.  Entries marked for
+// 2R, and entries the gate accepted (kSkipEntry has that bit set), pass
+// through unchanged.
+static tlr_status run_wide(tlr_ctx* c, const QueryParams& Q0, bool stats_mode) {
+    hipStream_t s = c->stream;
+    QueryParams V = Q0;
+    V.fast_list = 1;
+    V.qlist = c->fail_list;
+    V.qcount = c->d_fail_count;
+    V.next_query = c->d_wide_next;
+    V.fail_list = c->fail_list2;
+    V.fail_count = c->d_fail_count2;
+    V.features = nullptr;
+    HIPCHK(c, hipEventRecord(c->ev[7], s));
+    HIPCHK(c, launch_query_fast_wide(V, stats_mode, c->wide_blocks, s));
     return TLR_OK;
 }
 
@@ -702,15 +726,9 @@ static tlr_status run_plain(tlr_ctx* c, const QueryParams& Q0, const unsigned* l
     G.qlist = Q0.fail_list;
     G.qcount = Q0.fail_count;
     if (wide) {
-        QueryParams V = Q0;
-        V.fast_list = 1;
-        V.qlist = Q0.fail_list;
-        V.qcount = Q0.fail_count;
-        V.next_query = c->d_wide_next;
-        V.fail_list = c->fail_list2;
-        V.fail_count = c->d_fail_count2;
-        HIPCHK(c, hipEventRecord(c->ev[7], s));
-        HIPCHK(c, launch_query_fast_wide(V, stats_mode, c->wide_blocks, s));
+        // (the context's own lists: Q0's are those)
+        const tlr_status st = run_wide(c, Q0, stats_mode);
+        if (st != TLR_OK) return st;
         G.qlist = c->fail_list2;
         G.qcount = c->d_fail_count2;
     }
@@ -864,9 +882,13 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         HIPCHK(c, launch_gate_accept(c->logits, c->features, c->fail_list, c->d_fail_count, nq, c->gate_thr,
                                      c->d_accept_count, s));
         HIPCHK(c, hipEventRecord(c->ev[6], s));
+        // the second lane pass over what the gate left, then the general kernel
+        st = run_wide(c, Q, false);
+        if (st != TLR_OK) return st;
+        HIPCHK(c, hipEventRecord(c->ev[8], s));
         QueryParams G = Q;
-        G.qlist = c->fail_list;
-        G.qcount = c->d_fail_count;
+        G.qlist = c->fail_list2;
+        G.qcount = c->d_fail_count2;
         HIPCHK(c, launch_query_general(G, s));
     } else if (p->path == 2) {
         st = run_refgpu(c, Q, p->list_cap, &n2r);
@@ -887,7 +909,7 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
     unsigned long long hs[kNumStats];
     unsigned nfail = 0, naccept = 0;
     HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
-    const bool wide = !p->gate && p->path == 0;
+    const bool wide = p->path == 0;
     unsigned nwide = 0;
     HIPCHK(c, hipMemcpyAsync(&nfail, wide ? c->d_fail_count2 : c->d_fail_count, sizeof(nfail),
                              hipMemcpyDeviceToHost, s));
@@ -916,7 +938,7 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         if (wide) {
             // entries the first pass had marked for 2R are relayed, not run
             stats->wide_queries = (double)nwide;
-            if (hipEventElapsedTime(&ms, c->ev[7], c->ev[5]) == hipSuccess) stats->wide_ms = ms;
+            if (hipEventElapsedTime(&ms, c->ev[7], c->ev[p->gate ? 8 : 5]) == hipSuccess) stats->wide_ms = ms;
         }
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) stats->build_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) stats->query_ms = ms;
@@ -924,13 +946,16 @@ tlr_status tlr_reconstruct(tlr_ctx* c, const tlr_recon_params* p, float* out, in
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) stats->fast_ms = ms;
         if (p->gate) {
             if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) stats->gate_ms = ms;
-            if (hipEventElapsedTime(&ms, c->ev[6], c->ev[3]) == hipSuccess) stats->general_ms = ms;
-            stats->gate_candidates = (double)nfail;
-            stats->gate_rejected = (double)(nfail - naccept);
+            if (hipEventElapsedTime(&ms, c->ev[8], c->ev[3]) == hipSuccess) stats->general_ms = ms;
+            // the gate scored the first lane pass's hand-offs; accepted
+            // entries are relayed through the second pass and skipped
+            stats->gate_candidates = (double)nwide;
+            stats->gate_rejected = (double)(nwide - naccept);
         } else if (hipEventElapsedTime(&ms, c->ev[5], c->ev[3]) == hipSuccess) {
             stats->general_ms = ms;
         }
         stats->general_queries = (p->path == 1 && !p->gate) ? (double)nq : (double)(p->gate ? nfail - naccept : nfail);
+        if (p->gate) stats->fast_ms += stats->wide_ms;
         if (p->path == 2) {
             stats->num_2r = stats->general_queries = (double)n2r;
             stats->count = (double)nq;
@@ -999,6 +1024,7 @@ tlr_status tlr_reconstruct_noalloc(tlr_ctx* c, const tlr_recon_params* p, float*
         Q.num_queries = (unsigned)((size_t)W * bh * M);
         Q.pattern = pattern;
         Q.trav = c->trav;
+        Q.pairs = c->pairs;
         Q.leaves = c->leaves;
         Q.leaf_b = c->leaf_b;
         Q.pts = c->pts;
@@ -1224,6 +1250,7 @@ static void tree_params(const tlr_ctx* c, QueryParams& Q) {
     Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
     Q.root = c->nv > 1 ? c->nv / 2 - 1 : 0;
     Q.trav = c->trav;
+    Q.pairs = c->pairs;
     Q.leaves = c->leaves;
     Q.leaf_b = c->leaf_b;
     Q.pts = c->pts;
@@ -1462,6 +1489,7 @@ tlr_status tlr_copy_query_gather(tlr_ctx* c, const float* o5, int n, float radiu
     Q.n_nodes = c->nv > 0 ? 2 * c->nv - 1 : 0;
     Q.root = c->nv > 1 ? c->nv / 2 - 1 : 0;
     Q.trav = c->trav;
+    Q.pairs = c->pairs;
     Q.leaves = c->leaves;
     Q.leaf_b = c->leaf_b;
     Q.pts = c->pts;
@@ -1811,8 +1839,8 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
     const size_t nb = (size_t)c->nb, L = (size_t)c->L, nv = (size_t)c->nv;
     const size_t nt = nv > 0 ? 2 * nv - 1 : 0;
     const size_t bytes = sizeof(float4) * (2 * (nb + kPointPad) + nb) + sizeof(LeafRec) * L + sizeof(Bounds) * L +
-                         sizeof(TravNode) * nt + 4 * pat.size() + sizeof(float4) * nq + 16 * (size_t)W * H +
-                         16 * 256;
+                         sizeof(TravNode) * nt + sizeof(TravPair) * nv + 4 * pat.size() + sizeof(float4) * nq +
+                         16 * (size_t)W * H + 17 * 256;
     HIPCHK(c, b->prod.reserve(bytes));
     b->prod.reset();
     b->pts.pab = b->prod.alloc<float4>(2 * (nb + kPointPad));
@@ -1820,6 +1848,7 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
     b->leaves = b->prod.alloc<LeafRec>(std::max<size_t>(L, 1));
     b->leaf_b = b->prod.alloc<Bounds>(std::max<size_t>(L, 1));
     b->trav = b->prod.alloc<TravNode>(std::max<size_t>(nt, 1));
+    b->pairs = b->prod.alloc<TravPair>(std::max<size_t>(nv, 1));
     b->pattern = b->prod.alloc<float>(pat.size());
     b->qcol = b->prod.alloc<float4>(nq);
     b->img = b->prod.alloc<float>((size_t)W * H * 4);
@@ -1832,6 +1861,8 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
         HIPCHK(c, hipMemcpyAsync(b->leaf_b, c->leaf_b, sizeof(Bounds) * L, hipMemcpyDeviceToDevice, s));
     }
     if (nt) HIPCHK(c, hipMemcpyAsync(b->trav, c->trav, sizeof(TravNode) * nt, hipMemcpyDeviceToDevice, s));
+    if (nv > 1)
+        HIPCHK(c, hipMemcpyAsync(b->pairs, c->pairs, sizeof(TravPair) * (nv - 1), hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipMemcpyAsync(b->pattern, pat.data(), 4 * pat.size(), hipMemcpyHostToDevice, s));
     b->M = M;
     b->nq = nq;
@@ -1845,6 +1876,7 @@ static tlr_status batch_build(tlr_ctx* c, BatchSlot* b, const tlr_batch_item& it
     Q.num_queries = (unsigned)nq;
     Q.pattern = b->pattern;
     Q.trav = b->trav;
+    Q.pairs = b->pairs;
     Q.leaves = b->leaves;
     Q.leaf_b = b->leaf_b;
     Q.pts = b->pts;

@@ -66,6 +66,19 @@ struct __align__(16) TravNode {
     int pad;
 };
 
+// Both children of internal node g (g < n-1), for walks that test a node's
+// children from the node's own record and load a child's record only when
+// they enter it: the children's 12 plane coefficients, references (>= 0:
+// internal node index; < 0: -(leaf id + 1)) and, for a leaf child, the three
+// terms of its key (w0, t0, mv.w0 of LeafRec; zero for an internal child).
+// 128 bytes, one L2 line.
+struct __align__(128) TravPair {
+    float a[12];
+    float b[12];
+    int ra, rb;
+    float ka[3], kb[3];
+};
+
 TLR_HD int order_key(float f) {
     int i = __float_as_int(f);
     return i >= 0 ? i : (i ^ 0x7FFFFFFF);
