@@ -1126,11 +1126,15 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             // over leaf positions of the lanes' largest leaf.  The next
             // leaf's sample range is loaded a leaf ahead.
             uint64_t kmask = 0;
+            // offset from the leaf's first sample of the first sample within
+            // R, for the first four leaves that hold one, a byte each (0 when
+            // it does not fit); every sample before it is farther than R
+            uint32_t kst = 0;
             {
-                int li = 0, kg = 0, k1 = 0, nk0 = 0, nk1 = 0;
+                int li = 0, k0 = 0, kg = 0, k1 = 0, nk0 = 0, nk1 = 0;
                 if (n > 0) {
                     const LeafRec& r = Q.leaves[lid[0][tid]];
-                    kg = r.s0;
+                    kg = k0 = r.s0;
                     k1 = r.s1;
                 }
                 if (n > 1) {
@@ -1160,6 +1164,11 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                         reproject_point(ga[e], gb[e], o, Q.cc0, Q.cc1, x, y);
                         const float dx = x - o.x, dy = y - o.y;
                         if (!(dx * dx + dy * dy <= R2)) continue;
+                        if (!((kmask >> li) & 1ull)) {
+                            const int j = __popcll(kmask);
+                            const int off = kg + e - k0;
+                            if (j < 4 && off < 256) kst |= (uint32_t)off << (8 * j);
+                        }
                         kmask |= 1ull << li;
                         hit = true;
                         s_kept++;
@@ -1175,7 +1184,7 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                         }
                         leaf_min = kFltMax;
                         li++;
-                        kg = nk0;
+                        kg = k0 = nk0;
                         k1 = nk1;
                         if (li < n) s_in_leaves += (unsigned long long)(k1 - kg);
                         if (li + 1 < n) {
@@ -1219,12 +1228,15 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             float min0 = kFltMax;
             float4 near = make_float4(0, 0, 0, 0);
             {
-                // one loop over (leaf, group) steps, as in pass 1
+                // one loop over (leaf, group) steps, as in pass 1; surface 0's
+                // leaves are the first kept ones, so the first four start at
+                // their first sample within R
                 uint64_t km = nsurf > 0 ? (kmask & ((end0 < 64 ? (1ull << end0) : 0ull) - 1ull)) : 0ull;
                 int kg = 0, k1 = 0, nk0 = 0, nk1 = 0;
                 if (km) {
                     const LeafRec& r = Q.leaves[lid[__builtin_ctzll(km)][tid]];
-                    kg = r.s0;
+                    if (GATE) gacc.tested0 += r.s1 - r.s0;
+                    kg = r.s0 + (int)(kst & 255u);
                     k1 = r.s1;
                     km &= km - 1;
                 }
@@ -1233,7 +1245,6 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                     nk0 = r.s0;
                     nk1 = r.s1;
                 }
-                if (GATE) gacc.tested0 += k1 - kg;
                 while (kg < k1) {
                     const float4* g = pab + 2 * (size_t)kg;
                     float4 ga[4], gb[4];
@@ -1281,6 +1292,8 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                             kg = nk0;
                             k1 = nk1;
                             if (GATE) gacc.tested0 += k1 - kg;
+                            kst >>= 8;   // the next kept leaf's offset (0 past four)
+                            kg += (int)(kst & 255u);
                             if (km) {
                                 const LeafRec& r = Q.leaves[lid[__builtin_ctzll(km)][tid]];
                                 nk0 = r.s0;
