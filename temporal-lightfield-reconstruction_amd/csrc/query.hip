@@ -30,41 +30,48 @@ __device__ __forceinline__ float unif(float v) {
 // Orders global-memory scratch writes of the wave before later reads.
 __device__ __forceinline__ void wave_sync_mem() { __threadfence_block(); }
 
+// The wave's scratch area: one base pointer, each array at a constant
+// offset from it (computed where it is used, so that only the base stays
+// live in registers).
 struct WaveMem {
-    int* stack;
-    float* lkey;
-    int* lid;
-    int* sid;
-    float *sx, *sy, *sd;
-    int *sidx, *srpos;
-    int *f_start, *f_size, *f_quad, *f_mini;
-    float* f_min;
+    float* base;
+    template <class T>
+    __device__ __forceinline__ T* at(size_t words) const {
+        return reinterpret_cast<T*>(base + words);
+    }
+    static constexpr size_t kStack = 0;
+    static constexpr size_t kLkey = kStack + kStackCap;
+    static constexpr size_t kLid = kLkey + kLeafCap;
+    static constexpr size_t kSid = kLid + kLeafCap;
+    static constexpr size_t kSx = kSid + kLeafCap;
+    static constexpr size_t kSy = kSx + kSampleCap;
+    static constexpr size_t kSd = kSy + kSampleCap;
+    static constexpr size_t kSidx = kSd + kSampleCap;
+    static constexpr size_t kSrpos = kSidx + kSampleCap;
+    static constexpr size_t kFStart = kSrpos + kSampleCap;
+    static constexpr size_t kFSize = kFStart + kLeafCap;
+    static constexpr size_t kFQuad = kFSize + kLeafCap;
+    static constexpr size_t kFMini = kFQuad + kLeafCap;
+    static constexpr size_t kFMin = kFMini + kLeafCap;
+    static constexpr size_t kEnd = kFMin + kLeafCap;
+    __device__ __forceinline__ int* stack() const { return at<int>(kStack); }
+    __device__ __forceinline__ float* lkey() const { return at<float>(kLkey); }
+    __device__ __forceinline__ int* lid() const { return at<int>(kLid); }
+    __device__ __forceinline__ int* sid() const { return at<int>(kSid); }
+    __device__ __forceinline__ float* sx() const { return at<float>(kSx); }
+    __device__ __forceinline__ float* sy() const { return at<float>(kSy); }
+    __device__ __forceinline__ float* sd() const { return at<float>(kSd); }
+    __device__ __forceinline__ int* sidx() const { return at<int>(kSidx); }
+    __device__ __forceinline__ int* srpos() const { return at<int>(kSrpos); }
+    __device__ __forceinline__ int* f_start() const { return at<int>(kFStart); }
+    __device__ __forceinline__ int* f_size() const { return at<int>(kFSize); }
+    __device__ __forceinline__ int* f_quad() const { return at<int>(kFQuad); }
+    __device__ __forceinline__ int* f_mini() const { return at<int>(kFMini); }
+    __device__ __forceinline__ float* f_min() const { return at<float>(kFMin); }
 };
+static_assert(WaveMem::kEnd == kWaveScratchWords, "the wave's arrays fill its scratch words");
 
-__device__ __forceinline__ WaveMem wave_mem(float* base) {
-    WaveMem m;
-    char* p = reinterpret_cast<char*>(base);
-    auto take = [&](size_t words) {
-        char* r = p;
-        p += words * 4;
-        return r;
-    };
-    m.stack = reinterpret_cast<int*>(take(kStackCap));
-    m.lkey = reinterpret_cast<float*>(take(kLeafCap));
-    m.lid = reinterpret_cast<int*>(take(kLeafCap));
-    m.sid = reinterpret_cast<int*>(take(kLeafCap));
-    m.sx = reinterpret_cast<float*>(take(kSampleCap));
-    m.sy = reinterpret_cast<float*>(take(kSampleCap));
-    m.sd = reinterpret_cast<float*>(take(kSampleCap));
-    m.sidx = reinterpret_cast<int*>(take(kSampleCap));
-    m.srpos = reinterpret_cast<int*>(take(kSampleCap));
-    m.f_start = reinterpret_cast<int*>(take(kLeafCap));
-    m.f_size = reinterpret_cast<int*>(take(kLeafCap));
-    m.f_quad = reinterpret_cast<int*>(take(kLeafCap));
-    m.f_mini = reinterpret_cast<int*>(take(kLeafCap));
-    m.f_min = reinterpret_cast<float*>(take(kLeafCap));
-    return m;
-}
+__device__ __forceinline__ WaveMem wave_mem(float* base) { return WaveMem{base}; }
 
 // Reprojection of a tree sample (pa, pb records) to the output's (u,v,t):
 // ReconstructionOur.cpp:438-451
@@ -121,10 +128,12 @@ struct Gather {
     int nsurf;     // surfaces
     int used;      // sample slots in use (>= total after merges)
     int overflow;
-    int nl;        // leaves found (sorted ids: first kStage in LDS, the rest in m.sid)
+    int nl;        // leaves found (sorted ids: first kStage in LDS, the rest in m.sid())
 };
 
 // collectInputSamples2 for one output sample at radius R (wave-uniform).
+// STATS: count the reference's Stats (the overflow count is kept always).
+template <bool STATS>
 __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& o, float R,
                          unsigned long long* st) {
     const int lane = threadIdx.x & 63;
@@ -206,8 +215,8 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             }
             if (lds_stack && base + 2 * ni > kLdsStack) {
                 // the entries still pending move to scratch
-                for (int e = lane; e < base; e += 64) m.stack[e] = stk[e];
-                stk = m.stack;
+                for (int e = lane; e < base; e += 64) m.stack()[e] = stk[e];
+                stk = m.stack();
                 lds_stack = false;
             }
             if (hit && internal) {
@@ -221,8 +230,8 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                     Lkey[r] = key;
                     Lid[r] = lidv;
                 } else {
-                    m.lkey[r] = key;
-                    m.lid[r] = lidv;
+                    m.lkey()[r] = key;
+                    m.lid()[r] = lidv;
                 }
             }
             sp = base + 2 * ni;
@@ -231,17 +240,17 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
         }
     }
     if (g.overflow) return g;
-    st[ST_LEAF_NODES] += (unsigned long long)nl;
+    if (STATS) st[ST_LEAF_NODES] += (unsigned long long)nl;
     // --- front-to-back order: rank by (key, leaf id) ---
     for (int j = lane; j < nl; j += 64) {
-        const float kj = j < kStage ? Lkey[j] : m.lkey[j];
-        const int ij = j < kStage ? Lid[j] : m.lid[j];
+        const float kj = j < kStage ? Lkey[j] : m.lkey()[j];
+        const int ij = j < kStage ? Lid[j] : m.lid()[j];
         const LeafRec& lr = Q.leaves[ij];
         const int s0 = lr.s0, s1 = lr.s1;
         int rank = 0;
         for (int i = 0; i < nl; i++) {
-            const float ki = i < kStage ? Lkey[i] : m.lkey[i];
-            const int ii = i < kStage ? Lid[i] : m.lid[i];
+            const float ki = i < kStage ? Lkey[i] : m.lkey()[i];
+            const int ii = i < kStage ? Lid[i] : m.lid()[i];
             rank += (ki < kj || (!(kj < ki) && ii < ij)) ? 1 : 0;
         }
         if (rank < kStage) {
@@ -249,7 +258,7 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             S0[rank] = s0;
             S1[rank] = s1;
         } else {
-            m.sid[rank] = ij;
+            m.sid()[rank] = ij;
         }
     }
     wave_sync_mem();
@@ -269,7 +278,7 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             s0 = uni(S0[li]);
             s1 = uni(S1[li]);
         } else {
-            lid = uni(m.sid[li]);
+            lid = uni(m.sid()[li]);
             const LeafRec lr = Q.leaves[lid];
             s0 = uni(lr.s0);
             s1 = uni(lr.s1);
@@ -302,7 +311,7 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                     const int j = j0 + lane;
                     bool ok = true;
                     if (j < li) {
-                        const float4* b4 = reinterpret_cast<const float4*>(Q.leaf_b[j < kStage ? Sid[j] : m.sid[j]].p);
+                        const float4* b4 = reinterpret_cast<const float4*>(Q.leaf_b[j < kStage ? Sid[j] : m.sid()[j]].p);
                         const float4 ba = b4[0], bb = b4[1], bc = b4[2];
                         const float bp[12] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w, bc.x, bc.y, bc.z, bc.w};
                         ok = same_surface(np, bp, u0, v0, t0, u1, v1, t1);
@@ -311,11 +320,11 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
                 }
                 if (!same || nsurf == 0) {
                     if (nsurf > 0 && lane == 0) {
-                        m.f_start[nsurf - 1] = c_start;
-                        m.f_size[nsurf - 1] = c_size;
-                        m.f_quad[nsurf - 1] = c_quad;
-                        m.f_min[nsurf - 1] = c_min;
-                        m.f_mini[nsurf - 1] = c_mini;
+                        m.f_start()[nsurf - 1] = c_start;
+                        m.f_size()[nsurf - 1] = c_size;
+                        m.f_quad()[nsurf - 1] = c_quad;
+                        m.f_min()[nsurf - 1] = c_min;
+                        m.f_mini()[nsurf - 1] = c_mini;
                     }
                     sstart = li;
                     nsurf++;
@@ -335,11 +344,11 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
             const float dist = kept ? sqrtf(d2) : kFltMax;
             if (kept) {
                 const int pos = total + (int)lanes_below(km);
-                m.sx[pos] = x;
-                m.sy[pos] = y;
-                m.sd[pos] = dist;
-                m.sidx[pos] = k;
-                m.srpos[pos] = rpos;
+                m.sx()[pos] = x;
+                m.sy()[pos] = y;
+                m.sd()[pos] = dist;
+                m.sidx()[pos] = k;
+                m.srpos()[pos] = rpos;
             }
 #pragma unroll
             for (int q = 0; q < 4; q++)
@@ -363,15 +372,17 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
         }
     }
     if (nsurf > 0 && lane == 0) {
-        m.f_start[nsurf - 1] = c_start;
-        m.f_size[nsurf - 1] = c_size;
-        m.f_quad[nsurf - 1] = c_quad;
-        m.f_min[nsurf - 1] = c_min;
-        m.f_mini[nsurf - 1] = c_mini;
+        m.f_start()[nsurf - 1] = c_start;
+        m.f_size()[nsurf - 1] = c_size;
+        m.f_quad()[nsurf - 1] = c_quad;
+        m.f_min()[nsurf - 1] = c_min;
+        m.f_mini()[nsurf - 1] = c_mini;
     }
     wave_sync_mem();
-    st[ST_SAMPLES_IN_LEAVES] += in_leaves;
-    st[ST_FETCHED] += (unsigned long long)total;
+    if (STATS) {
+        st[ST_SAMPLES_IN_LEAVES] += in_leaves;
+        st[ST_FETCHED] += (unsigned long long)total;
+    }
     g.total = total;
     g.nsurf = nsurf;
     g.used = total;
@@ -411,9 +422,9 @@ __device__ bool triangle_search(const WaveMem& m, int start, int n, const OSampl
     constexpr int kAll = XPOS | XNEG | YPOS | YNEG;
     int cls = 0;  // bit q: some sample has quadrant q (q = rpos & (XPOS|YPOS))
     for (int j = lane; j < n; j += 64) {
-        const float x = m.sx[start + j], y = m.sy[start + j];
+        const float x = m.sx()[start + j], y = m.sy()[start + j];
         const float d = sqrtf((x - o.x) * (x - o.x) + (y - o.y) * (y - o.y));
-        const int rp = m.srpos[start + j];
+        const int rp = m.srpos()[start + j];
         tx[j] = x;
         ty[j] = y;
         tr[j] = rp | (!(d > disp) ? kNear : 0);
@@ -499,7 +510,7 @@ __device__ __forceinline__ bool shadow_lit(float ow, float dx, float dy, float k
     return ow - 0.01f <= key + d;
 }
 
-template <bool DEBUG, int MODE = 0>
+template <bool DEBUG, int MODE = 0, bool STATS = true>
 __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMem& m, unsigned qi, bool skip1r,
                                               unsigned long long* st) {
     const int lane = threadIdx.x & 63;
@@ -527,7 +538,7 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
         o.v = tp[3];
         o.t = tp[4];
     }
-    st[ST_COUNT] += 1;
+    if (STATS) st[ST_COUNT] += 1;
 
     // Filterer::reconstruct: ReconstructionOur.cpp:50-159
     float disp = dispersion_for_spp(Q.spp_pow2);
@@ -542,26 +553,28 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
     // (dispersion doubled before each, gather at 2x dispersion).
     int pass = skip1r ? 1 : 0;
     for (;;) {
-        g = gather(Q, m, o, pass == 0 ? disp : disp * 2, st);
+        g = gather<STATS>(Q, m, o, pass == 0 ? disp : disp * 2, st);
         overflow = g.overflow;
         if (overflow) break;
         if (pass == 0) {
             n1 = g.total;
-            const int quad0 = g.nsurf > 0 ? uni(m.f_quad[0]) : 0;
+            const int quad0 = g.nsurf > 0 ? uni(m.f_quad()[0]) : 0;
             f2 = g.nsurf == 0 || (quad0 != 0xF && g.nsurf > 1);
             if (!f2) break;
             pass = 1;
             continue;
         }
         if (g.total != 0 || pass == 4) break;
-        if (pass == 1) st[ST_AT_LEAST_ONE] += 1;
+        if (STATS && pass == 1) st[ST_AT_LEAST_ONE] += 1;
         disp *= 2.f;
         pass++;
     }
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!overflow) {
-        st[ST_2R] += f2 ? 1 : 0;
-        st[ST_WITHIN_1R] += (unsigned long long)n1;
+        if (STATS) {
+            st[ST_2R] += f2 ? 1 : 0;
+            st[ST_WITHIN_1R] += (unsigned long long)n1;
+        }
         const int nsurf = g.nsurf;
         int numSurfaces = nsurf;
         int used = g.used;
@@ -570,34 +583,34 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
         // minIndex is copied as is (it then indexes the merged list).
         if (f2) {
             for (int si = 0; si < nsurf - 1; si++) {
-                const int ssize = uni(m.f_size[si]);
+                const int ssize = uni(m.f_size()[si]);
                 if (ssize < 4) {
-                    const int sst = uni(m.f_start[si]);
-                    const int dst = uni(m.f_start[si + 1]);
-                    const int dsize = uni(m.f_size[si + 1]);
+                    const int sst = uni(m.f_start()[si]);
+                    const int dst = uni(m.f_start()[si + 1]);
+                    const int dsize = uni(m.f_size()[si + 1]);
                     if (used + dsize + ssize > kSampleCap) {
                         overflow = 1;
                         break;
                     }
                     for (int j = lane; j < dsize + ssize; j += 64) {
                         const int from = j < dsize ? dst + j : sst + (j - dsize);
-                        m.sx[used + j] = m.sx[from];
-                        m.sy[used + j] = m.sy[from];
-                        m.sd[used + j] = m.sd[from];
-                        m.sidx[used + j] = m.sidx[from];
-                        m.srpos[used + j] = m.srpos[from];
+                        m.sx()[used + j] = m.sx()[from];
+                        m.sy()[used + j] = m.sy()[from];
+                        m.sd()[used + j] = m.sd()[from];
+                        m.sidx()[used + j] = m.sidx()[from];
+                        m.srpos()[used + j] = m.srpos()[from];
                     }
-                    const float smin = unif(m.f_min[si]);
-                    const float dmin = unif(m.f_min[si + 1]);
-                    const int smini = uni(m.f_mini[si]);
+                    const float smin = unif(m.f_min()[si]);
+                    const float dmin = unif(m.f_min()[si + 1]);
+                    const int smini = uni(m.f_mini()[si]);
                     wave_sync_mem();
                     if (lane == 0) {
-                        m.f_start[si + 1] = used;
-                        m.f_size[si + 1] = dsize + ssize;
-                        m.f_size[si] = 0;
+                        m.f_start()[si + 1] = used;
+                        m.f_size()[si + 1] = dsize + ssize;
+                        m.f_size()[si] = 0;
                         if (smin < dmin) {
-                            m.f_min[si + 1] = smin;
-                            m.f_mini[si + 1] = smini;
+                            m.f_min()[si + 1] = smin;
+                            m.f_mini()[si + 1] = smini;
                         }
                     }
                     wave_sync_mem();
@@ -606,8 +619,10 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
                 }
             }
         }
-        st[ST_SURFACES] += (unsigned long long)numSurfaces;
-        st[ST_MERGED] += (numSurfaces != nsurf) ? 1 : 0;
+        if (STATS) {
+            st[ST_SURFACES] += (unsigned long long)numSurfaces;
+            st[ST_MERGED] += (numSurfaces != nsurf) ? 1 : 0;
+        }
 
         // Coverage and filtering, front to back: ReconstructionOur.cpp:165-302
         bool found = false;
@@ -615,11 +630,11 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
         float nearest = kFltMax;
         int nI0 = -1, nI1 = 1;
         for (int si = 0; si < nsurf && !found && !overflow; si++) {
-            const int fs = uni(m.f_start[si]);
-            const int n = uni(m.f_size[si]);
+            const int fs = uni(m.f_start()[si]);
+            const int n = uni(m.f_size()[si]);
             if (n == 0) continue;
-            const float fmin = unif(m.f_min[si]);
-            const int fmini = uni(m.f_mini[si]);
+            const float fmin = unif(m.f_min()[si]);
+            const int fmini = uni(m.f_mini()[si]);
             if (fmin < nearest) {
                 nearest = fmin;
                 nI0 = si;
@@ -637,11 +652,11 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
                     float wgt = 0.0f;
                     float4 c = make_float4(0, 0, 0, 0);
                     if (k < n) {
-                        const float rx = m.sx[fs + k], ry = m.sy[fs + k];
+                        const float rx = m.sx()[fs + k], ry = m.sy()[fs + k];
                         const float dx = o.x - rx, dy = o.y - ry;
                         wgt = 1 - sqrtf(dx * dx + dy * dy) / disp;
                         if (wgt > 0) {
-                            const int sk = m.sidx[fs + k];
+                            const int sk = m.sidx()[fs + k];
                             c = Q.pts.pc[sk];
                             if (MODE == 2) {
                                 // per-input-sample thresholding: ReconstructionOur.cpp:283-291
@@ -672,10 +687,10 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
             // sample: ReconstructionOur.cpp:325-331 (zero without one)
             float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f), e1 = e0;
             if (!overflow && nI0 != -1) {
-                const int fs = m.f_start[nI0];
-                const int sk = m.sidx[fs + nI1];
+                const int fs = m.f_start()[nI0];
+                const int sk = m.sidx()[fs + nI1];
                 const float4 pe = Q.pext[sk];
-                e0 = make_float4(m.sx[fs + nI1], m.sy[fs + nI1], sample_w_at(Q.pts, sk, o.t), pe.z);
+                e0 = make_float4(m.sx()[fs + nI1], m.sy()[fs + nI1], sample_w_at(Q.pts, sk, o.t), pe.z);
                 e1 = make_float4(pe.x, pe.y, 0.f, 0.f);
             }
             Q.qext[2 * (size_t)qi] = e0;
@@ -683,8 +698,8 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
         }
         if (!overflow && nI0 != -1) {
             if (ca == 0) {
-                const int fs = uni(m.f_start[nI0]);
-                const int sk = uni(m.sidx[fs + nI1]);
+                const int fs = uni(m.f_start()[nI0]);
+                const int sk = uni(m.sidx()[fs + nI1]);
                 const float4 c = Q.pts.pc[sk];
                 cr = c.x;
                 cg = c.y;
@@ -693,7 +708,7 @@ __device__ __forceinline__ void general_query(const QueryParams& Q, const WaveMe
                 if (MODE == 2) {
                     // ReconstructionOur.cpp:341-347
                     const float4 pe = Q.pext[sk];
-                    const float nx = unif(m.sx[fs + nI1]), ny = unif(m.sy[fs + nI1]);
+                    const float nx = unif(m.sx()[fs + nI1]), ny = unif(m.sy()[fs + nI1]);
                     if (!shadow_lit(ow, o.x - nx, o.y - ny, sample_w_at(Q.pts, sk, o.t), pe.x, pe.y))
                         cr = cg = cb = 0.0f;
                 }
@@ -742,10 +757,11 @@ __device__ __forceinline__ void flush_stats(unsigned long long* dst, const unsig
 // 7 waves per SIMD (the register limit this sets spills a little; the
 // kernel waits on memory and gains more from resident waves).  Persistent:
 // waves take queries from an atomic counter until the list is done.
-template <bool DEBUG, int MODE>
+template <bool DEBUG, int MODE, bool STATS>
 __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    // wave-uniform (so the scratch pointers derived from it live in scalar registers)
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
     const WaveMem m = wave_mem(Q.scratch + (size_t)wave * kWaveScratchWords);
     unsigned long long st[kNumStats];
 #pragma unroll
@@ -764,9 +780,9 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
             skip1r = (qi & kNeeds2R) != 0;
             qi &= ~kNeeds2R;
         }
-        general_query<DEBUG, MODE>(Q, m, qi, skip1r, st);
+        general_query<DEBUG, MODE, STATS>(Q, m, qi, skip1r, st);
     }
-    flush_stats(Q.stats, st, Q.collect_stats != 0);
+    flush_stats(Q.stats, st, STATS && Q.collect_stats != 0);
 }
 
 // Batched form: one list of hand-offs from every buffer of a batch, with the
@@ -778,7 +794,8 @@ constexpr unsigned kBatchChunk = 8;
 
 __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B) {
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    // wave-uniform (so the scratch pointers derived from it live in scalar registers)
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
     const WaveMem m = wave_mem(B.scratch + (size_t)wave * kWaveScratchWords);
     unsigned long long st[kNumStats];
 #pragma unroll
@@ -804,7 +821,8 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general_batch(BatchQuery B)
             }
             do {
                 const unsigned ent = (unsigned)uni((int)B.list[e]);
-                if (ent != kSkipEntry) general_query<false>(Q, m, ent & ~kNeeds2R, (ent & kNeeds2R) != 0, st);
+                if (ent != kSkipEntry)
+                    general_query<false, 0, false>(Q, m, ent & ~kNeeds2R, (ent & kNeeds2R) != 0, st);
                 e++;
             } while (e < end && uni((int)B.list_item[e]) == item);
         }
@@ -1563,7 +1581,8 @@ __global__ __launch_bounds__(kBlock) void k_query_gather(QueryParams Q, const fl
                                                          int* __restrict__ samples, int* __restrict__ surface_of,
                                                          int* __restrict__ n_samples) {
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    // wave-uniform (so the scratch pointers derived from it live in scalar registers)
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
     const int waves = gridDim.x * kWavesPerBlock;
     const WaveMem m = wave_mem(Q.scratch + (size_t)wave * kWaveScratchWords);
     unsigned long long st[kNumStats];
@@ -1572,19 +1591,19 @@ __global__ __launch_bounds__(kBlock) void k_query_gather(QueryParams Q, const fl
     for (int j = wave; j < n; j += waves) {
         OSample o;
         o.x = o5[5 * j]; o.y = o5[5 * j + 1]; o.u = o5[5 * j + 2]; o.v = o5[5 * j + 3]; o.t = o5[5 * j + 4];
-        const Gather g = gather(Q, m, o, R, st);
+        const Gather g = gather<false>(Q, m, o, R, st);
         if (g.overflow) {
             if (lane == 0) n_leaves[j] = n_samples[j] = -1;
             continue;
         }
         const int* Sid = reinterpret_cast<const int*>(wave_lds()) + 2 * kStage;
         for (int li = lane; li < g.nl && li < leaf_cap; li += 64)
-            leaves[(size_t)j * leaf_cap + li] = li < kStage ? Sid[li] : m.sid[li];
+            leaves[(size_t)j * leaf_cap + li] = li < kStage ? Sid[li] : m.sid()[li];
         for (int si = 0; si < g.nsurf; si++) {
-            const int fs = uni(m.f_start[si]), fn = uni(m.f_size[si]);
+            const int fs = uni(m.f_start()[si]), fn = uni(m.f_size()[si]);
             for (int k = lane; k < fn; k += 64)
                 if (fs + k < sample_cap) {
-                    samples[(size_t)j * sample_cap + fs + k] = m.sidx[fs + k];
+                    samples[(size_t)j * sample_cap + fs + k] = m.sidx()[fs + k];
                     surface_of[(size_t)j * sample_cap + fs + k] = si;
                 }
         }
@@ -1661,7 +1680,7 @@ __global__ void k_take_rows(const uint32_t* __restrict__ src, uint32_t* __restri
 
 int general_waves_per_cu() {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general<false, 0>, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_query_general<false, 0, false>, kBlock, 0) != hipSuccess ||
         blocks < 1)
         blocks = 1;
     return blocks * kWavesPerBlock;
@@ -1671,13 +1690,15 @@ hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     const int blocks = p.num_waves / kWavesPerBlock;
     if (p.mode == 1)
-        k_query_general<false, 1><<<blocks, kBlock, 0, s>>>(p);
+        k_query_general<false, 1, true><<<blocks, kBlock, 0, s>>>(p);
     else if (p.mode == 2)
-        k_query_general<false, 2><<<blocks, kBlock, 0, s>>>(p);
+        k_query_general<false, 2, true><<<blocks, kBlock, 0, s>>>(p);
     else if (p.qdebug)
-        k_query_general<true, 0><<<blocks, kBlock, 0, s>>>(p);
+        k_query_general<true, 0, true><<<blocks, kBlock, 0, s>>>(p);
+    else if (p.collect_stats)
+        k_query_general<false, 0, true><<<blocks, kBlock, 0, s>>>(p);
     else
-        k_query_general<false, 0><<<blocks, kBlock, 0, s>>>(p);
+        k_query_general<false, 0, false><<<blocks, kBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 
