@@ -796,7 +796,8 @@ __device__ int ref_fail_wave(const QueryParams& Q, const RefWaveMem& m, const OS
 // Persistent waves over list[0 .. *count): one output sample per wave.
 __global__ __launch_bounds__(kRefStageBlock) void k_ref_fail_wave(RefLaunch R) {
     const int lane = threadIdx.x & 63;
-    const size_t wave = ((size_t)blockIdx.x * kRefStageBlock + threadIdx.x) >> 6;
+    // wave-uniform (so the scratch pointers derived from it live in scalar registers)
+    const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * kRefStageBlock + threadIdx.x) >> 6));
     const RefWaveMem m = ref_wave_mem(R.wscratch + wave * kRefWaveScratchWords);
     const unsigned total = *R.count;
     for (;;) {
