@@ -263,17 +263,25 @@ def bench_gate(args):
     gt = importlib.import_module(tlr.__name__ + ".gate_train")
     log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = tlr.Ctx(0)
-    t0 = time.perf_counter()
-    train = [(2, 960, 540, 16, 100000 + k) for k in range(args.gate_train)]
-    params, losses = gt.train_gate(tlr, ctx, train, out_spp=16, stride=args.gate_stride, epochs=args.gate_epochs,
-                                   batch=4096, lr=2e-3, log=log)
-    t_train = time.perf_counter() - t0
-    log("trained on %d scenes in %.1f s, losses %s" % (args.gate_train, t_train, losses))
     budget = 1e-3
-    val = tlr.make_scene(2, 960, 540, 16, seed=200000)
-    thr, curve = gt.calibrate_threshold(tlr, ctx, val, budget)
-    log("threshold %.4f; validation curve (thr, accepted hand-offs, mean-abs): %s" % (
-        thr, [(round(a, 3), round(b, 3), "%.2e" % c) for a, b, c in curve[-6:]]))
+    if args.gate_eval_only:
+        # the committed parameters and threshold, evaluated only
+        params = np.fromfile(GATE_PARAMS, dtype="<f4")
+        with open(GATE_META) as f:
+            meta0 = json.load(f)
+        thr, losses, t_train = meta0["threshold"], meta0["losses"], None
+        ctx.gate_set_params(params)
+    else:
+        t0 = time.perf_counter()
+        train = [(2, 960, 540, 16, 100000 + k) for k in range(args.gate_train)]
+        params, losses = gt.train_gate(tlr, ctx, train, out_spp=16, stride=args.gate_stride,
+                                       epochs=args.gate_epochs, batch=4096, lr=2e-3, log=log)
+        t_train = time.perf_counter() - t0
+        log("trained on %d scenes in %.1f s, losses %s" % (args.gate_train, t_train, losses))
+        val = tlr.make_scene(2, 960, 540, 16, seed=200000)
+        thr, curve = gt.calibrate_threshold(tlr, ctx, val, budget)
+        log("threshold %.4f; validation curve (thr, accepted hand-offs, mean-abs): %s" % (
+            thr, [(round(a, 3), round(b, 3), "%.2e" % c) for a, b, c in curve[-6:]]))
     ctx.gate_set_threshold(thr)
     cfg = CONFIGS[4]
     W, H, M = cfg["W"], cfg["H"], cfg["out_spp"]
@@ -303,15 +311,16 @@ def bench_gate(args):
     mean = lambda k: float(np.mean([r[k] for r in rows]))
     gated_ms = mean("gated_ms") + mean("build_ms")
     plain_ms = mean("plain_ms") + mean("build_ms")
-    cmd = "python3 bench.py --gate-train %d --gate-stride %d --gate-epochs %d --gate-eval %d --steps %d" % (
-        args.gate_train, args.gate_stride, args.gate_epochs, args.gate_eval, args.steps)
-    params.astype("<f4").tofile(GATE_PARAMS)
-    meta = {"threshold": thr, "command": cmd, "train_scenes": args.gate_train, "stride": args.gate_stride,
-            "epochs": args.gate_epochs, "losses": [float(x) for x in losses], "budget_mean_abs": budget,
-            "validation_curve_tail": curve[-8:],
-            "label": "provisional first-surface colour within 0.001 of the plain colour in every channel"}
-    with open(GATE_META, "w") as f:
-        json.dump(meta, f, indent=1)
+    if not args.gate_eval_only:
+        cmd = "python3 bench.py --gate-train %d --gate-stride %d --gate-epochs %d --gate-eval %d --steps %d" % (
+            args.gate_train, args.gate_stride, args.gate_epochs, args.gate_eval, args.steps)
+        params.astype("<f4").tofile(GATE_PARAMS)
+        meta = {"threshold": thr, "command": cmd, "train_scenes": args.gate_train, "stride": args.gate_stride,
+                "epochs": args.gate_epochs, "losses": [float(x) for x in losses], "budget_mean_abs": budget,
+                "validation_curve_tail": curve[-8:],
+                "label": "provisional first-surface colour within 0.001 of the plain colour in every channel"}
+        with open(GATE_META, "w") as f:
+            json.dump(meta, f, indent=1)
     res = {
         "metric": "reconstructed Mpix/s @16 out-spp, 1 GPU; learned-gate vs plain at <=1e-3 err",
         "value": W * H / (gated_ms * 1e-3) / 1e6,
@@ -325,8 +334,9 @@ def bench_gate(args):
         "vs_baseline": None,
         "dtype": "f32 (gate MLP: bf16 operands, f32 accumulation)",
         "data": "synthetic (seeded scenes generated on the host, uploaded before timing)",
-        "config": {"workload": "gate trained on %d seeded 960x540 16spp scenes, evaluated on %d held-out "
-                               "1920x1080 16spp scenes at 16 out-spp" % (args.gate_train, args.gate_eval),
+        "config": {"workload": "gate %s, evaluated on %d held-out 1920x1080 16spp scenes at 16 out-spp" % (
+                       "parameters and threshold as committed (gate_params.f32 / .json)" if args.gate_eval_only else
+                       "trained on %d seeded 960x540 16spp scenes" % args.gate_train, args.gate_eval),
                    "width": W, "height": H, "input_spp": 16, "out_spp": M},
         "plain_mpix_per_s": W * H / (plain_ms * 1e-3) / 1e6,
         "speedup_vs_plain": plain_ms / gated_ms,
@@ -338,6 +348,7 @@ def bench_gate(args):
                  "mean_abs_vs_plain": mean("mean_abs"), "worst_scene_mean_abs": max(r["mean_abs"] for r in rows),
                  "max_abs_vs_plain": max(r["max_abs"] for r in rows), "budget": budget,
                  "training_s": t_train, "final_loss": float(losses[-1])},
+        "command": "python3 bench.py " + " ".join(sys.argv[1:]),
     }
     print(json.dumps(res))
 
@@ -618,6 +629,8 @@ def main():
                     help="train the gate on SCENES 960x540 scenes, calibrate it, evaluate it and write "
                          "gate_params.f32 / gate_params.json")
     ap.add_argument("--gate-eval", type=int, default=20, help="--gate-train: held-out evaluation scenes")
+    ap.add_argument("--gate-eval-only", action="store_true",
+                    help="evaluate the committed gate parameters on --gate-eval held-out scenes (no training)")
     ap.add_argument("--gate-stride", type=int, default=2, help="--gate-train: keep every n-th hand-off")
     ap.add_argument("--gate-epochs", type=int, default=4, help="--gate-train: epochs")
     ap.add_argument("--path", type=int, default=0, choices=(0, 2),
@@ -639,7 +652,7 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    if args.gate_train:
+    if args.gate_train or args.gate_eval_only:
         bench_gate(args)
         return
     if args.config == 5:
