@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(64) void k_query_fast_wide(QueryParams Q) {
 
 // Gated fast path (single buffer): as k_query_fast<false>, plus provisional
 // colours and gate features for the hand-offs.
-__global__ __launch_bounds__(kFastBlock, 7) void k_query_fast_gate(QueryParams Q) {
+__global__ __launch_bounds__(kFastBlock) void k_query_fast_gate(QueryParams Q) {
     __shared__ int lid[kFastLeafCap][kFastBlock];
     fast_query<false, kFastLeafCap, kFastBlock, true>(Q, tile_slot<kFastBlock>(Q, blockIdx.x, threadIdx.x), 0u,
                                                       lid);
