@@ -921,7 +921,14 @@ __device__ __forceinline__ void write_gate_features(float* out, bool prov, const
 // hand-off entry gets its gate features in Q.features (row = list position).
 // RELAY (list mode over a hand-off list): entries already marked kNeeds2R
 // are appended to fail_list unchanged, the others are run.
-template <bool STATS, int CAP, int BLOCK, bool GATE = false, bool RELAY = false>
+// MODE (as the general kernel's): 1 = also the extended result of each
+// query it completes (qext); 2 = shadow queries (radius scaled by the
+// query's density, every contributing sample and the nearest-sample fallback
+// kept only where they light the query, no lens/time weighting) with
+// extended results.  In modes 1 and 2 the queries come from qin when it is
+// set (2 float4 each: (x, y, u, v), (t, w, density, 0)), else from the
+// output pattern.
+template <bool STATS, int CAP, int BLOCK, bool GATE = false, bool RELAY = false, int MODE = 0>
 __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, unsigned item, int (*lid)[BLOCK]) {
     const int tid = threadIdx.x;
     const int M = Q.M;
@@ -944,16 +951,30 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
     int g_n = 0, g_nkept = 0, g_nkept0 = 0, g_quad0 = 0, g_nsurf = 0;
     float g_aw = 0.f, g_min0 = 0.f, g_key0 = 0.f, g_key1 = 0.f;
     OSample o{0.f, 0.f, 0.f, 0.f, 0.f};
+    float ow = 0.0f, density = 1.0f;   // modes 1, 2 with explicit queries
     if (active) {
-        const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
-        const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
-        o.x = tp[0] + (float)px;
-        o.y = tp[1] + (float)py;
-        o.u = tp[2];
-        o.v = tp[3];
-        o.t = tp[4];
+        if (MODE != 0 && Q.qin) {
+            const float4 a = Q.qin[2 * (size_t)q], b = Q.qin[2 * (size_t)q + 1];
+            o.x = a.x;
+            o.y = a.y;
+            o.u = a.z;
+            o.v = a.w;
+            o.t = b.x;
+            ow = b.y;
+            density = b.z;
+        } else {
+            const int pat = (int)(hash_bits((uint32_t)px, (uint32_t)py) % 64u);
+            const float* tp = Q.pattern + 5 * ((size_t)pat * M + i);
+            o.x = tp[0] + (float)px;
+            o.y = tp[1] + (float)py;
+            o.u = tp[2];
+            o.v = tp[3];
+            o.t = tp[4];
+        }
     }
-    const float disp = dispersion_for_spp(Q.spp_pow2);
+    float disp = dispersion_for_spp(Q.spp_pow2);
+    // non-uniform view sample density in light space: ReconstructionOur.cpp:78-80
+    if (MODE == 2) disp *= sqrtf(density);
     const float R = disp;
     // --- wave-wide descent: the deepest node whose subtree holds every leaf
     // any lane's walk can reach.  From a node, the walk enters a child only
@@ -1245,6 +1266,9 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
             float ar = 0, ag = 0, ab = 0, aw = 0;
             float min0 = kFltMax;
             float4 near = make_float4(0, 0, 0, 0);
+            // modes 1, 2: the nearest sample's index and reprojected position
+            int kn = 0;
+            float nx = 0.0f, ny = 0.0f;
             {
                 // one loop over (leaf, group) steps, as in pass 1; surface 0's
                 // leaves are the first kept ones, so the first four start at
@@ -1288,13 +1312,27 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                             gacc.sdx += dx;
                             gacc.sdy += dy;
                         }
-                        const float4 col = pc[k];
+                        const float4 c0 = pc[k];
                         if (dist < min0) {
                             min0 = dist;
-                            near = col;
+                            near = c0;
+                            if (MODE != 0) {
+                                kn = k;
+                                nx = x;
+                                ny = y;
+                            }
                         }
                         const float wgt = 1 - dist / disp;
                         if (wgt > 0) {
+                            float4 col = c0;
+                            if (MODE == 2) {
+                                // per-input-sample thresholding: ReconstructionOur.cpp:283-291
+                                const float4 pe = Q.pext[k];
+                                const float key = (gb[e].x != 0.0f || gb[e].y != 0.0f || gb[e].z != 0.0f)
+                                                      ? 1.0f * ga[e].z + (o.t - ga[e].w) * gb[e].z
+                                                      : ga[e].z;
+                                if (!shadow_lit(ow, o.x - x, o.y - y, key, pe.x, pe.y)) col.x = col.y = col.z = 0.0f;
+                            }
                             ar += wgt * col.x;
                             ag += wgt * col.y;
                             ab += wgt * col.z;
@@ -1345,15 +1383,28 @@ __device__ __forceinline__ void fast_query(const QueryParams& Q, unsigned slot, 
                 float cr = ar * oow, cg = ag * oow, cb = ab * oow, ca = aw * oow;
                 if (ca == 0) {
                     cr = near.x; cg = near.y; cb = near.z; ca = 1.0f;
+                    if (MODE == 2 && !fail) {
+                        // ReconstructionOur.cpp:341-347
+                        const float4 pe = Q.pext[kn];
+                        if (!shadow_lit(ow, o.x - nx, o.y - ny, sample_w_at(Q.pts, kn, o.t), pe.x, pe.y))
+                            cr = cg = cb = 0.0f;
+                    }
                 }
-                if (Q.lens_gauss) {
+                if (MODE != 0 && !fail && Q.qext) {
+                    // Filterer::Result's xy, w, wg and density from the found
+                    // surface's nearest sample: ReconstructionOur.cpp:325-331
+                    const float4 pe = Q.pext[kn];
+                    Q.qext[2 * (size_t)q] = make_float4(nx, ny, sample_w_at(Q.pts, kn, o.t), pe.z);
+                    Q.qext[2 * (size_t)q + 1] = make_float4(pe.x, pe.y, 0.0f, 0.0f);
+                }
+                if (MODE != 2 && Q.lens_gauss) {
                     const float e = 2.718281828f;
                     const float gu = powf(e, -((o.u - 0) * (o.u - 0)) / (2 * (0.5f * 0.5f)));
                     const float gv = powf(e, -((o.v - 0) * (o.v - 0)) / (2 * (0.5f * 0.5f)));
                     const float gg = gu * gv;
                     cr *= gg; cg *= gg; cb *= gg; ca *= gg;
                 }
-                if (Q.time_gauss) {
+                if (MODE != 2 && Q.time_gauss) {
                     const float e = 2.718281828f;
                     const float gg = powf(e, -((o.t - 0.5f) * (o.t - 0.5f)) / (2 * (0.25f * 0.25f)));
                     cr *= gg; cg *= gg; cb *= gg; ca *= gg;
@@ -1459,6 +1510,15 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast(QueryParams Q) {
     // Queries in scan order (pixel-major): the whole device sweeps one band
     // of rows, whose samples stay resident in the last-level cache.
     fast_query<STATS, kFastLeafCap, kFastBlock>(Q, tile_slot<kFastBlock>(Q, blockIdx.x, threadIdx.x), 0u, lid);
+}
+
+// 1R lane kernel for the general kernel's modes 1 and 2 (extended results,
+// shadow queries), over the pattern, explicit queries (qin) or a list.
+template <bool STATS, int MODE>
+__global__ __launch_bounds__(kFastBlock) void k_query_fast_mode(QueryParams Q) {
+    __shared__ int lid[kFastLeafCap][kFastBlock];
+    fast_query<STATS, kFastLeafCap, kFastBlock, false, false, MODE>(
+        Q, tile_slot<kFastBlock>(Q, blockIdx.x, threadIdx.x), 0u, lid);
 }
 
 // Second 1R pass over the first pass's hand-off list, for the output samples
@@ -1711,6 +1771,24 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s) {
         k_query_fast<true><<<blocks, kFastBlock, 0, s>>>(p);
     else
         k_query_fast<false><<<blocks, kFastBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_fast_mode(const QueryParams& p, bool stats, hipStream_t s) {
+    if (p.num_queries == 0) return hipSuccess;
+    const size_t slots = p.fast_list ? p.num_queries : fast_slots(p);
+    const unsigned blocks = (unsigned)((slots + kFastBlock - 1) / kFastBlock);
+    if (p.mode == 2) {
+        if (stats)
+            k_query_fast_mode<true, 2><<<blocks, kFastBlock, 0, s>>>(p);
+        else
+            k_query_fast_mode<false, 2><<<blocks, kFastBlock, 0, s>>>(p);
+    } else {
+        if (stats)
+            k_query_fast_mode<true, 1><<<blocks, kFastBlock, 0, s>>>(p);
+        else
+            k_query_fast_mode<false, 1><<<blocks, kFastBlock, 0, s>>>(p);
+    }
     return hipGetLastError();
 }
 

@@ -176,6 +176,10 @@ hipError_t launch_query_fast(const QueryParams& p, bool stats, hipStream_t s);
 // with a first surface at 1R, its filtered colour in qcol (provisional) and,
 // for every hand-off entry, its gate features in p.features.
 hipError_t launch_query_fast_gate(const QueryParams& p, hipStream_t s);
+// The 1R lane kernel in the general kernel's mode p.mode (1: extended
+// results; 2: shadow queries): completes what it can, writing qcol and qext,
+// and appends the rest to p.fail_list for the general kernel in that mode.
+hipError_t launch_query_fast_mode(const QueryParams& p, bool stats, hipStream_t s);
 // Second 1R pass (kWideLeafCap leaves per lane, persistent 64-lane blocks)
 // over p.qlist[0 .. *p.qcount), the first pass's hand-offs; p.next_query is
 // its work counter (zeroed before launch); its hand-offs, with the entries

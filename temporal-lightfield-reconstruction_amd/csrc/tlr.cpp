@@ -1271,16 +1271,18 @@ tlr_status tlr_reconstruct_queries(tlr_ctx* c, const tlr_recon_params* p, const 
     if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
     const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
     const size_t nq = (size_t)n;
-    HIPCHK(c, c->sarena.reserve(nq * 5 * sizeof(float4) + (size_t)waves * kWaveScratchWords * 4 + 8 * 256));
+    HIPCHK(c, c->sarena.reserve(nq * (5 * sizeof(float4) + 4) + (size_t)waves * kWaveScratchWords * 4 + 8 * 256));
     c->sarena.reset();
     float4* qin = c->sarena.alloc<float4>(2 * nq);
     float4* qcol = c->sarena.alloc<float4>(nq);
     float4* qext = c->sarena.alloc<float4>(2 * nq);
+    unsigned* hand = c->sarena.alloc<unsigned>(nq);
     float* scratch = c->sarena.alloc<float>((size_t)waves * kWaveScratchWords);
     if (c->sarena.failed) return c->fail(TLR_E_HIP, "query arena exhausted");
     HIPCHK(c, hipMemcpyAsync(qin, queries, nq * 2 * sizeof(float4), hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
     HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
     QueryParams Q;
     memset(&Q, 0, sizeof(Q));
     tree_params(c, Q);
@@ -1297,8 +1299,16 @@ tlr_status tlr_reconstruct_queries(tlr_ctx* c, const tlr_recon_params* p, const 
     Q.mode = shadow ? 2 : 1;
     Q.qin = qin;
     Q.qext = qext;
+    Q.fail_list = hand;
+    Q.fail_count = c->d_fail_count;
     HIPCHK(c, hipEventRecord(c->ev[2], s));
-    HIPCHK(c, launch_query_general(Q, s));
+    // the 1R lane kernel completes what it can; the general kernel (one wave
+    // per query) takes its hand-offs
+    HIPCHK(c, launch_query_fast_mode(Q, true, s));
+    QueryParams G = Q;
+    G.qlist = hand;
+    G.qcount = c->d_fail_count;
+    HIPCHK(c, launch_query_general(G, s));
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     HIPCHK(c, hipMemcpyAsync(rgba_out, qcol, nq * sizeof(float4), hipMemcpyDeviceToHost, s));
     if (ext_out) HIPCHK(c, hipMemcpyAsync(ext_out, qext, nq * 2 * sizeof(float4), hipMemcpyDeviceToHost, s));
@@ -1372,11 +1382,18 @@ tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const 
     S.cw = W; S.ch = H;
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     // 1. the camera's output samples, with the nearest sample's xy, w, wg
-    //    and density (the general kernel over the whole pattern)
+    //    and density (the 1R lane kernel over the pattern, the general kernel
+    //    over its hand-offs)
     Q.mode = 1;
     Q.pext = c->pext;
     Q.qext = qext;
-    HIPCHK(c, launch_query_general(Q, s));
+    HIPCHK(c, launch_query_fast_mode(Q, Q.collect_stats != 0, s));
+    {
+        QueryParams G = Q;
+        G.qlist = Q.fail_list;
+        G.qcount = Q.fail_count;
+        HIPCHK(c, launch_query_general(G, s));
+    }
     HIPCHK(c, hipEventRecord(c->ev[5], s));
     // 2. shadow rays into the light's window
     ShadowRays R;
@@ -1407,8 +1424,19 @@ tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const 
     L.num_waves = Q.num_waves;
     L.mode = 2;
     L.qin = lq;
-    L.qlist = list;
-    L.qcount = cnt;
+    // the 1R lane kernel over the listed light queries, the general kernel
+    // over its hand-offs (the context's second hand-off list)
+    {
+        QueryParams F = L;
+        F.fast_list = 1;
+        F.qlist = list;
+        F.qcount = cnt;
+        F.fail_list = c->fail_list2;
+        F.fail_count = c->d_fail_count2;
+        HIPCHK(c, launch_query_fast_mode(F, L.collect_stats != 0, s));
+    }
+    L.qlist = c->fail_list2;
+    L.qcount = c->d_fail_count2;
     HIPCHK(c, launch_query_general(L, s));
     HIPCHK(c, hipEventRecord(c->ev[6], s));
     // 4. shading, per pixel in output-sample order
