@@ -608,8 +608,10 @@ def bench_shadows(args):
                    "height": H, "input_spp": spp, "out_spp": M, "seed": args.seed},
         "device_ms": {"camera_pass": dev["general_ms"], "light_pass": dev["fast_ms"], "shading": dev["combine_ms"],
                       "total": dev["query_ms"] + dev["combine_ms"]},
-        "light_queries": float(np.mean([s["general_queries"] for s in sts])),
+        "light_queries": float(np.mean([s["light_queries"] for s in sts])),
         "output_samples": W * H * M,
+        "lane_pass_hand_offs": float(np.mean([s["wide_queries"] for s in sts])),
+        "general_queries": float(np.mean([s["general_queries"] for s in sts])),
     }
     print(json.dumps(res))
 

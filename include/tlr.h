@@ -98,6 +98,9 @@ typedef struct tlr_stats {
      * samples_in_leaf_nodes and samples_within_1r */
     double wide_ms, wide_queries;
     double wide_leaf_nodes, wide_samples_in_leaf_nodes, wide_samples_within_1r;
+    /* tlr_reconstruct_dof_motion_shadows: the light-space queries its shadow
+     * rays made (its wide_queries and general_queries count both passes) */
+    double light_queries;
 } tlr_stats;
 
 typedef struct tlr_build_info {

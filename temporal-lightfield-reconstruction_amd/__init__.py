@@ -72,7 +72,8 @@ class Stats(C.Structure):
         "build_ms", "query_ms", "combine_ms", "total_ms", "num_leaves", "num_samples",
         "fast_ms", "general_ms", "general_queries", "gate_candidates", "gate_ms", "gate_rejected",
         "fast_leaf_nodes", "fast_samples_in_leaf_nodes", "fast_samples_within_1r",
-        "wide_ms", "wide_queries", "wide_leaf_nodes", "wide_samples_in_leaf_nodes", "wide_samples_within_1r")]
+        "wide_ms", "wide_queries", "wide_leaf_nodes", "wide_samples_in_leaf_nodes", "wide_samples_within_1r",
+        "light_queries")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -450,11 +451,12 @@ class Ctx:
         return rgba, ext, st.as_dict()
 
     def reconstruct_dof_motion_shadows(self, light, sp, out_spp=16, override_uvt=None, lens_gaussian=False,
-                                       time_gaussian=False):
+                                       time_gaussian=False, collect_stats=False):
         """reconstructDofMotionShadows with `light` (a built Ctx of the same
         size) as the shadow TreeGather: (image, stats)."""
         p = ReconParams()
         p.out_spp = out_spp
+        p.collect_stats = int(collect_stats)
         p.override_uvt = 1 if override_uvt is not None else 0
         if override_uvt is not None:
             for k in range(3):

@@ -178,12 +178,11 @@ __global__ void k_grid_setup(GridInfo* gi) {
     gi->gh = iy1 - iy0;
 }
 
-// Bucket index of each kept sample; counts per cell (grid-stride, one
-// num_bucketed atomic per block).
+// Bucket index of each kept sample (grid-stride, one num_bucketed atomic
+// per block); the cells' counts come from the sorted keys (k_cell_runs).
 __global__ __launch_bounds__(kBlock) void k_cell_keys(const float* __restrict__ rx, const float* __restrict__ ry,
                                                       const float* __restrict__ rw, int n, GridInfo* gi,
-                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                      uint32_t* __restrict__ counts) {
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const float scale = gi->scale;
     const int bx = gi->bbmin_i[0], by = gi->bbmin_i[1], gw = gi->gw, gh = gi->gh;
     unsigned nin = 0;
@@ -194,7 +193,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(const float* __restrict__ 
             const int sy = floor_i(scale * ry[i]) - by;
             if (!(sx < 0 || sy < 0 || sx >= gw || sy >= gh)) {
                 key = (uint32_t)(sy * gw + sx);
-                atomicAdd(&counts[key], 1u);
                 nin++;
             }
         }
@@ -211,9 +209,24 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(const float* __restrict__ 
     }
 }
 
-// Summed-area table over the cell counts, (gh+1) x (gw+1), zero first row/col.
-// Rows: one block per grid row.
-__global__ __launch_bounds__(kBlock) void k_sat_rows(const uint32_t* __restrict__ counts, int gw, int gh,
+// Each non-empty cell's run in the sorted keys: start[cell] = its first
+// position, end[cell] = one past its last (both zero for an empty cell, from
+// the memsets before).  Keys of samples outside the grid sort after every
+// cell and are skipped.
+__global__ __launch_bounds__(kBlock) void k_cell_runs(const uint32_t* __restrict__ skeys, int n, uint32_t cells,
+                                                      uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = skeys[i];
+    if (k >= cells) return;
+    if (i == 0 || skeys[i - 1] != k) start[k] = (uint32_t)i;
+    if (i == n - 1 || skeys[i + 1] != k) end[k] = (uint32_t)i + 1;
+}
+
+// Summed-area table over the cell counts (end - start of each cell's run),
+// (gh+1) x (gw+1), zero first row/col.  Rows: one block per grid row.
+__global__ __launch_bounds__(kBlock) void k_sat_rows(const uint32_t* __restrict__ cstart,
+                                                     const uint32_t* __restrict__ cend, int gw, int gh,
                                                      uint32_t* __restrict__ sat) {
     const int y = blockIdx.x;
     __shared__ uint32_t carry;
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_sat_rows(const uint32_t* __restrict_
     if (threadIdx.x == 0) row[0] = 0;
     for (int base = 0; base < gw; base += kBlock) {
         const int x = base + threadIdx.x;
-        uint32_t v = (x < gw) ? counts[(size_t)y * gw + x] : 0u;
+        uint32_t v = (x < gw) ? cend[(size_t)y * gw + x] - cstart[(size_t)y * gw + x] : 0u;
         // inclusive block scan
         uint32_t inc = v;
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1132,13 +1145,17 @@ void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, 
 }
 
 void launch_cell_keys(const float* rx, const float* ry, const float* rw, int n, GridInfo* gi, uint32_t* keys,
-                      uint32_t* vals, uint32_t* counts, hipStream_t s) {
+                      uint32_t* vals, hipStream_t s) {
     const unsigned cb = (unsigned)std::min<long long>(kReduceBlocks, ((long long)n + kBlock - 1) / kBlock);
-    if (n > 0) k_cell_keys<<<cb, kBlock, 0, s>>>(rx, ry, rw, n, gi, keys, vals, counts);
+    if (n > 0) k_cell_keys<<<cb, kBlock, 0, s>>>(rx, ry, rw, n, gi, keys, vals);
 }
 
-void launch_sat(const uint32_t* counts, int gw, int gh, uint32_t* sat, hipStream_t s) {
-    k_sat_rows<<<gh, kBlock, 0, s>>>(counts, gw, gh, sat);
+void launch_cell_runs(const uint32_t* skeys, int n, uint32_t cells, uint32_t* start, uint32_t* end, hipStream_t s) {
+    if (n > 0) k_cell_runs<<<TLR_GRID(n), 0, s>>>(skeys, n, cells, start, end);
+}
+
+void launch_sat(const uint32_t* cstart, const uint32_t* cend, int gw, int gh, uint32_t* sat, hipStream_t s) {
+    k_sat_rows<<<gh, kBlock, 0, s>>>(cstart, cend, gw, gh, sat);
     k_sat_cols<<<TLR_GRID(gw + 1), 0, s>>>(sat, gw, gh);
 }
 

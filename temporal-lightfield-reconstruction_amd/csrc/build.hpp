@@ -14,9 +14,14 @@ namespace tlr {
 // (x, y, w, t | mv_x, mv_y, mv_w, r | g, b, 0, 0) for the tree-order gathers.
 void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, float* rx, float* ry, float* rw,
                       float* rt, float4* packed, int* hits, GridInfo* gi, hipStream_t s);
+// Bucket index (cell) of each kept sample, 0xFFFFFFFF outside the grid, and
+// the identity as values for the stable sort by cell.
 void launch_cell_keys(const float* rx, const float* ry, const float* rw, int n, GridInfo* gi, uint32_t* keys,
-                      uint32_t* vals, uint32_t* counts, hipStream_t s);
-void launch_sat(const uint32_t* counts, int gw, int gh, uint32_t* sat, hipStream_t s);
+                      uint32_t* vals, hipStream_t s);
+// Each cell's run [start, end) in the sorted keys (zero-filled beforehand).
+void launch_cell_runs(const uint32_t* skeys, int n, uint32_t cells, uint32_t* start, uint32_t* end, hipStream_t s);
+// Summed-area table of the cell counts end - start.
+void launch_sat(const uint32_t* cstart, const uint32_t* cend, int gw, int gh, uint32_t* sat, hipStream_t s);
 void launch_kd(const uint32_t* sat, int gw, int gh, int4* owner, uint32_t* own_flag, int* own_y1, hipStream_t s);
 void launch_emit_initial(const int4* owner, const uint32_t* own_flag, const int* own_y1, const uint32_t* idx,
                          const uint32_t* sat, int gw, int gh, int4* rect, int2* offcnt, hipStream_t s);

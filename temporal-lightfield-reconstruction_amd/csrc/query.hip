@@ -1527,7 +1527,7 @@ __global__ __launch_bounds__(kFastBlock) void k_query_fast_mode(QueryParams Q) {
 // the next 64 entries until the list is done); entries the first pass marked
 // for 2R are passed on unchanged.  Its hand-offs (2R fetches and output
 // samples with more than CAP leaves) go to fail_list for the general kernel.
-template <bool STATS, int CAP>
+template <bool STATS, int CAP, int MODE>
 __global__ __launch_bounds__(64) void k_query_fast_wide(QueryParams Q) {
     __shared__ int lid[CAP][64];
     const unsigned n = *Q.qcount;
@@ -1536,7 +1536,7 @@ __global__ __launch_bounds__(64) void k_query_fast_wide(QueryParams Q) {
         if (threadIdx.x == 0) base = atomicAdd(Q.next_query, 64u);
         base = (unsigned)uni((int)base);
         if (base >= n) break;
-        fast_query<STATS, CAP, 64, false, true>(Q, base + threadIdx.x, 0u, lid);
+        fast_query<STATS, CAP, 64, false, true, MODE>(Q, base + threadIdx.x, 0u, lid);
     }
 }
 
@@ -1749,10 +1749,14 @@ int general_waves_per_cu() {
 hipError_t launch_query_general(const QueryParams& p, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
     const int blocks = p.num_waves / kWavesPerBlock;
-    if (p.mode == 1)
+    if (p.mode == 1 && p.collect_stats)
         k_query_general<false, 1, true><<<blocks, kBlock, 0, s>>>(p);
-    else if (p.mode == 2)
+    else if (p.mode == 1)
+        k_query_general<false, 1, false><<<blocks, kBlock, 0, s>>>(p);
+    else if (p.mode == 2 && p.collect_stats)
         k_query_general<false, 2, true><<<blocks, kBlock, 0, s>>>(p);
+    else if (p.mode == 2)
+        k_query_general<false, 2, false><<<blocks, kBlock, 0, s>>>(p);
     else if (p.qdebug)
         k_query_general<true, 0, true><<<blocks, kBlock, 0, s>>>(p);
     else if (p.collect_stats)
@@ -1794,16 +1798,20 @@ hipError_t launch_query_fast_mode(const QueryParams& p, bool stats, hipStream_t 
 
 hipError_t launch_query_fast_wide(const QueryParams& p, bool stats, int blocks, hipStream_t s) {
     if (p.num_queries == 0) return hipSuccess;
-    if (stats)
-        k_query_fast_wide<true, kWideLeafCap><<<blocks, 64, 0, s>>>(p);
-    else
-        k_query_fast_wide<false, kWideLeafCap><<<blocks, 64, 0, s>>>(p);
+    switch (p.mode * 2 + (stats ? 1 : 0)) {
+        case 0: k_query_fast_wide<false, kWideLeafCap, 0><<<blocks, 64, 0, s>>>(p); break;
+        case 1: k_query_fast_wide<true, kWideLeafCap, 0><<<blocks, 64, 0, s>>>(p); break;
+        case 2: k_query_fast_wide<false, kWideLeafCap, 1><<<blocks, 64, 0, s>>>(p); break;
+        case 3: k_query_fast_wide<true, kWideLeafCap, 1><<<blocks, 64, 0, s>>>(p); break;
+        case 4: k_query_fast_wide<false, kWideLeafCap, 2><<<blocks, 64, 0, s>>>(p); break;
+        default: k_query_fast_wide<true, kWideLeafCap, 2><<<blocks, 64, 0, s>>>(p); break;
+    }
     return hipGetLastError();
 }
 
 int fast_wide_blocks_per_cu() {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_query_fast_wide<false, kWideLeafCap>, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_query_fast_wide<false, kWideLeafCap, 0>, 64, 0) !=
             hipSuccess ||
         b <= 0)
         b = 1;

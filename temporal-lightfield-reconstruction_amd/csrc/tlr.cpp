@@ -489,7 +489,7 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
         return c->fail(TLR_E_INVALID, "degenerate bucketing grid %dx%d", gw, gh);
     const size_t cells = (size_t)gw * gh;
 
-    uint32_t* counts = A.alloc<uint32_t>(cells);
+    uint32_t* cell_end = A.alloc<uint32_t>(cells);
     uint32_t* cell_start = A.alloc<uint32_t>(cells + 1);
     uint32_t* sat = A.alloc<uint32_t>((size_t)(gw + 1) * (gh + 1));
     int4* owner = A.alloc<int4>(cells);
@@ -498,20 +498,22 @@ static tlr_status build_impl(tlr_ctx* c, float aperture_adjust, float focal_adju
     uint32_t* own_idx = A.alloc<uint32_t>(cells + 1);
     if (A.failed) return c->fail(TLR_E_HIP, "arena exhausted (grid)");
 
-    // ---- bucketing: stable sort of (cell, input index) ----
-    HIPCHK(c, hipMemsetAsync(counts, 0, sizeof(uint32_t) * cells, s));
-    launch_cell_keys(c->rx, c->ry, c->rw, N, c->d_gi, k0, v0, counts, s);
+    // ---- bucketing: stable sort of (cell, input index); each cell's run ----
+    launch_cell_keys(c->rx, c->ry, c->rw, N, c->d_gi, k0, v0, s);
     HIPCHK(c, hipGetLastError());
     bool first = true;
     HIPCHK(c, radix_sort_pairs(k0, v0, k1, v1, (size_t)N, bits_for(cells + 1), hist, stmp, &first, s));
     const uint32_t* skeys = first ? k0 : k1;
     const uint32_t* svals = first ? v0 : v1;
-    HIPCHK(c, exclusive_scan_u32(counts, cell_start, cell_start + cells, cells, stmp, s));
-    const int nb = (int)read_u32(c, cell_start + cells);
+    HIPCHK(c, hipMemsetAsync(cell_start, 0, sizeof(uint32_t) * cells, s));
+    HIPCHK(c, hipMemsetAsync(cell_end, 0, sizeof(uint32_t) * cells, s));
+    launch_cell_runs(skeys, N, (uint32_t)cells, cell_start, cell_end, s);
+    HIPCHK(c, hipGetLastError());
+    const int nb = (int)read_u32(c, &c->d_gi->num_bucketed);
     c->nb = nb;
 
     // ---- initial kd split ----
-    launch_sat(counts, gw, gh, sat, s);
+    launch_sat(cell_start, cell_end, gw, gh, sat, s);
     launch_kd(sat, gw, gh, owner, own_flag, own_y1, s);
     HIPCHK(c, exclusive_scan_u32(own_flag, own_idx, own_idx + cells, cells, stmp, s));
     const int nil = (int)read_u32(c, own_idx + cells);
@@ -698,6 +700,41 @@ static tlr_status run_wide(tlr_ctx* c, const QueryParams& Q0, bool stats_mode) {
     V.features = nullptr;
     HIPCHK(c, hipEventRecord(c->ev[7], s));
     HIPCHK(c, launch_query_fast_wide(V, stats_mode, c->wide_blocks, s));
+    return TLR_OK;
+}
+
+// Modes 1 and 2 (extended results, shadow queries): the 1R lane kernel over
+// F's queries (F.fast_list: over F.qlist), the second lane pass over its
+// hand-offs, the general kernel over what is left.  list1/cnt1 and
+// list2/cnt2 are the two hand-off lists (counts zeroed here), wide_next the
+// second pass's work counter, gen_next the general kernel's.
+static tlr_status run_mode_passes(tlr_ctx* c, const QueryParams& F0, unsigned* list1, unsigned* cnt1,
+                                  unsigned* list2, unsigned* cnt2, unsigned* wide_next, unsigned* gen_next,
+                                  bool stats_mode) {
+    hipStream_t s = c->stream;
+    if (c->wide_blocks == 0) c->wide_blocks = c->prop.multiProcessorCount * fast_wide_blocks_per_cu();
+    HIPCHK(c, hipMemsetAsync(cnt1, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(cnt2, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(wide_next, 0, sizeof(unsigned), s));
+    HIPCHK(c, hipMemsetAsync(gen_next, 0, sizeof(unsigned), s));
+    QueryParams F = F0;
+    F.fail_list = list1;
+    F.fail_count = cnt1;
+    HIPCHK(c, launch_query_fast_mode(F, stats_mode, s));
+    QueryParams V = F0;
+    V.fast_list = 1;
+    V.qlist = list1;
+    V.qcount = cnt1;
+    V.next_query = wide_next;
+    V.fail_list = list2;
+    V.fail_count = cnt2;
+    HIPCHK(c, launch_query_fast_wide(V, stats_mode, c->wide_blocks, s));
+    QueryParams G = F0;
+    G.fast_list = 0;
+    G.qlist = list2;
+    G.qcount = cnt2;
+    G.next_query = gen_next;
+    HIPCHK(c, launch_query_general(G, s));
     return TLR_OK;
 }
 
@@ -1271,18 +1308,17 @@ tlr_status tlr_reconstruct_queries(tlr_ctx* c, const tlr_recon_params* p, const 
     if (c->general_wpc == 0) c->general_wpc = general_waves_per_cu();
     const int waves = std::max(4, c->prop.multiProcessorCount * c->general_wpc);
     const size_t nq = (size_t)n;
-    HIPCHK(c, c->sarena.reserve(nq * (5 * sizeof(float4) + 4) + (size_t)waves * kWaveScratchWords * 4 + 8 * 256));
+    HIPCHK(c, c->sarena.reserve(nq * (5 * sizeof(float4) + 8) + (size_t)waves * kWaveScratchWords * 4 + 9 * 256));
     c->sarena.reset();
     float4* qin = c->sarena.alloc<float4>(2 * nq);
     float4* qcol = c->sarena.alloc<float4>(nq);
     float4* qext = c->sarena.alloc<float4>(2 * nq);
     unsigned* hand = c->sarena.alloc<unsigned>(nq);
+    unsigned* hand2 = c->sarena.alloc<unsigned>(nq);
     float* scratch = c->sarena.alloc<float>((size_t)waves * kWaveScratchWords);
     if (c->sarena.failed) return c->fail(TLR_E_HIP, "query arena exhausted");
     HIPCHK(c, hipMemcpyAsync(qin, queries, nq * 2 * sizeof(float4), hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kNumStats, s));
-    HIPCHK(c, hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), s));
-    HIPCHK(c, hipMemsetAsync(c->d_fail_count, 0, sizeof(unsigned), s));
     QueryParams Q;
     memset(&Q, 0, sizeof(Q));
     tree_params(c, Q);
@@ -1299,16 +1335,11 @@ tlr_status tlr_reconstruct_queries(tlr_ctx* c, const tlr_recon_params* p, const 
     Q.mode = shadow ? 2 : 1;
     Q.qin = qin;
     Q.qext = qext;
-    Q.fail_list = hand;
-    Q.fail_count = c->d_fail_count;
     HIPCHK(c, hipEventRecord(c->ev[2], s));
-    // the 1R lane kernel completes what it can; the general kernel (one wave
-    // per query) takes its hand-offs
-    HIPCHK(c, launch_query_fast_mode(Q, true, s));
-    QueryParams G = Q;
-    G.qlist = hand;
-    G.qcount = c->d_fail_count;
-    HIPCHK(c, launch_query_general(G, s));
+    // the 1R lane passes complete what they can; the general kernel (one
+    // wave per query) takes the rest
+    st = run_mode_passes(c, Q, hand, c->d_fail_count, hand2, c->d_fail_count2, c->d_wide_next, c->d_counter, true);
+    if (st != TLR_OK) return st;
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     HIPCHK(c, hipMemcpyAsync(rgba_out, qcol, nq * sizeof(float4), hipMemcpyDeviceToHost, s));
     if (ext_out) HIPCHK(c, hipMemcpyAsync(ext_out, qext, nq * 2 * sizeof(float4), hipMemcpyDeviceToHost, s));
@@ -1387,13 +1418,11 @@ tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const 
     Q.mode = 1;
     Q.pext = c->pext;
     Q.qext = qext;
-    HIPCHK(c, launch_query_fast_mode(Q, Q.collect_stats != 0, s));
-    {
-        QueryParams G = Q;
-        G.qlist = Q.fail_list;
-        G.qcount = Q.fail_count;
-        HIPCHK(c, launch_query_general(G, s));
-    }
+    st = run_mode_passes(c, Q, c->fail_list, c->d_fail_count, c->fail_list2, c->d_fail_count2, c->d_wide_next,
+                         c->d_counter, Q.collect_stats != 0);
+    if (st != TLR_OK) return st;
+    HIPCHK(c, hipMemcpyAsync(cnt + 40, c->d_fail_count, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(cnt + 41, c->d_fail_count2, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipEventRecord(c->ev[5], s));
     // 2. shadow rays into the light's window
     ShadowRays R;
@@ -1424,20 +1453,16 @@ tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const 
     L.num_waves = Q.num_waves;
     L.mode = 2;
     L.qin = lq;
-    // the 1R lane kernel over the listed light queries, the general kernel
-    // over its hand-offs (the context's second hand-off list)
-    {
-        QueryParams F = L;
-        F.fast_list = 1;
-        F.qlist = list;
-        F.qcount = cnt;
-        F.fail_list = c->fail_list2;
-        F.fail_count = c->d_fail_count2;
-        HIPCHK(c, launch_query_fast_mode(F, L.collect_stats != 0, s));
-    }
-    L.qlist = c->fail_list2;
-    L.qcount = c->d_fail_count2;
-    HIPCHK(c, launch_query_general(L, s));
+    // the listed light queries through the same passes (the camera pass's
+    // hand-off lists are free again: the stream has finished with them)
+    L.fast_list = 1;
+    L.qlist = list;
+    L.qcount = cnt;
+    st = run_mode_passes(c, L, c->fail_list, c->d_fail_count, c->fail_list2, c->d_fail_count2, c->d_wide_next,
+                         cnt + 32, L.collect_stats != 0);
+    if (st != TLR_OK) return st;
+    HIPCHK(c, hipMemcpyAsync(cnt + 42, c->d_fail_count, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(cnt + 43, c->d_fail_count2, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipEventRecord(c->ev[6], s));
     // 4. shading, per pixel in output-sample order
     ShadowShade D;
@@ -1454,9 +1479,10 @@ tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const 
     HIPCHK(c, hipEventRecord(c->ev[4], s));
     if (!out_on_device && out) HIPCHK(c, hipMemcpyAsync(out, c->d_img, (size_t)W * H * 16, hipMemcpyDeviceToHost, s));
     unsigned long long hs[kNumStats];
-    unsigned nl = 0;
+    unsigned nl = 0, hand[4] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(hs, c->d_stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(&nl, cnt, sizeof(nl), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(hand, cnt + 40, sizeof(hand), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (stats) {
         stats_from(hs, stats);
@@ -1465,7 +1491,9 @@ tlr_status tlr_reconstruct_dof_motion_shadows(tlr_ctx* c, tlr_ctx* light, const 
         if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) stats->combine_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[2], c->ev[5]) == hipSuccess) stats->general_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) stats->fast_ms = ms;
-        stats->general_queries = (double)nl;   // light queries
+        stats->light_queries = (double)nl;
+        stats->wide_queries = (double)hand[0] + hand[2];
+        stats->general_queries = (double)hand[1] + hand[3];
         stats->total_ms = stats->query_ms + stats->combine_ms;
     }
     if (hs[ST_OVERFLOW])

@@ -89,8 +89,8 @@ def test_dof_motion_shadows_matches_oracle(tlr, orc):
     print("dof motion shadows %dx%d" % (W, H), flush=True)
     img, st = cam.reconstruct_dof_motion_shadows(light, sp, out_spp=8)
     ref = ocam.reconstruct_dof_motion_shadows(olight, osp, out_spp=8)
-    print("light queries %d of %d" % (st["general_queries"], W * H * 8), flush=True)
-    assert st["general_queries"] > 0.3 * W * H * 8
+    print("light queries %d of %d" % (st["light_queries"], W * H * 8), flush=True)
+    assert st["light_queries"] > 0.3 * W * H * 8
     assert np.array_equal(_bits(img), _bits(ref))
 
 
