@@ -72,8 +72,10 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
     int kmin_x = 0x7FFFFFFF, kmin_y = 0x7FFFFFFF;
     int kmax_x = (int)0x80000000, kmax_y = (int)0x80000000;
     unsigned nok = 0, nbad = 0;
+    const int lane = threadIdx.x & 63;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         bool ok = false;
+        int pix = -1;   // the pixel this sample's reprojection hits
         const float x = in.x[i], y = in.y[i], w = in.w[i], u = in.u[i], v = in.v[i], t = in.t[i];
         const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
         const float coc = C0 / w + C1;
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
                 packed[3 * (size_t)i + 1] = make_float4(mvx, mvy, mvw, in.r[i]);
                 packed[3 * (size_t)i + 2] = make_float4(in.g[i], in.b[i], 0.0f, 0.0f);
                 const int sx = floor_i(ox), sy = floor_i(oy);
-                if (sx >= 0 && sy >= 0 && sx < W && sy < H) atomicAdd(&hits[sy * W + sx], 1);
+                if (sx >= 0 && sy >= 0 && sx < W && sy < H) pix = sy * W + sx;
                 kmin_x = min(kmin_x, order_key(ox));
                 kmin_y = min(kmin_y, order_key(oy));
                 kmax_x = max(kmax_x, order_key(ox));
@@ -113,6 +115,18 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
             }
         }
         if (!ok) rw[i] = 0.0f;  // marks a discarded sample
+        // per-pixel hit counts: a wave's samples are a few neighbouring
+        // pixels' and often land in the same pixel, so up to four pixels
+        // get one atomic for all their lanes; the other lanes add their own
+        uint64_t pend = __ballot(pix >= 0);
+        for (int r = 0; r < 4 && pend; r++) {
+            const int leader = __ffsll((unsigned long long)pend) - 1;
+            const int key = __shfl(pix, leader, 64);
+            const uint64_t same = __ballot(pix == key);
+            if (lane == leader) atomicAdd(&hits[key], (int)__popcll(same));
+            pend &= ~same;
+        }
+        if ((pend >> lane) & 1ull) atomicAdd(&hits[pix], 1);
     }
     kmin_x = wave_min(kmin_x); kmin_y = wave_min(kmin_y);
     kmax_x = wave_max(kmax_x); kmax_y = wave_max(kmax_y);
@@ -383,7 +397,25 @@ struct CandView {
     }
 };
 
-__device__ void fwsort_insertion(const CandView& v, int start, int size) {
+// The same order on one 64-bit key per candidate: t's bits above w's.  For
+// t in [0, 1] and w > 0 (a kept sample's reprojection time and depth) the
+// bit patterns order as the values do, so comparing keys is the (t, w)
+// comparison above, with half the loads and one array less to swap.
+struct CandKey64 {
+    uint32_t* idx;
+    uint64_t* key;
+    __device__ __forceinline__ int cmp(int a, int b) const {
+        const uint64_t ka = key[a], kb = key[b];
+        return ka < kb ? -1 : (ka > kb ? 1 : 0);
+    }
+    __device__ __forceinline__ void swp(int a, int b) const {
+        uint32_t i0 = idx[a]; idx[a] = idx[b]; idx[b] = i0;
+        uint64_t k0 = key[a]; key[a] = key[b]; key[b] = k0;
+    }
+};
+
+template <class V>
+__device__ void fwsort_insertion(const V& v, int start, int size) {
     for (int i = 1; i < size; i++) {
         int j = start + i - 1;
         while (j >= start && v.cmp(j, j + 1) > 0) {
@@ -393,7 +425,8 @@ __device__ void fwsort_insertion(const CandView& v, int start, int size) {
     }
 }
 
-__device__ int fwsort_partition(const CandView& v, int low, int high) {
+template <class V>
+__device__ int fwsort_partition(const V& v, int low, int high) {
     int l = low, c = (low + high) >> 1, h = high - 2;
     if (v.cmp(l, h) > 0) {
         int tmp = l; l = h; h = tmp;
@@ -412,7 +445,8 @@ __device__ int fwsort_partition(const CandView& v, int low, int high) {
     return i;
 }
 
-__device__ void fwsort(const CandView& v, int start, int end) {
+template <class V>
+__device__ void fwsort(const V& v, int start, int end) {
     if (end - start < 2) return;
     int low = start, high = end;
     int stack[32];
@@ -434,18 +468,18 @@ __device__ void fwsort(const CandView& v, int start, int end) {
 
 // One wave of 64 initial leaves (one thread each).  The wave copies its
 // leaves into LDS one leaf at a time (each copy a contiguous, coalesced
-// range), every thread runs the sort on its own leaf there, and the sorted
-// input indices go back the same way; the sorted (t, w) copies are not needed
-// afterwards.  A wave whose leaves hold more than kSortLds candidates sorts in
-// global memory instead.  Either way the permutation is FW::sort's.
+// range; (t, w) as one 64-bit key), every thread runs the sort on its own
+// leaf there, and the sorted input indices go back the same way; the sorted
+// keys are not needed afterwards.  A wave whose leaves hold more than
+// kSortLds candidates sorts in global memory instead.  Either way the
+// permutation is FW::sort's.
 constexpr int kSortBlock = 64;
 constexpr int kSortLds = 2304;
 
 __global__ __launch_bounds__(kSortBlock) void k_leaf_sort(const int2* __restrict__ offcnt, int nil, uint32_t* cand,
                                                           float* ct, float* cw) {
+    __shared__ uint64_t s_key[kSortLds];
     __shared__ uint32_t s_idx[kSortLds];
-    __shared__ float s_t[kSortLds];
-    __shared__ float s_w[kSortLds];
     const int lane = threadIdx.x;
     const int k = blockIdx.x * kSortBlock + lane;
     const int2 oc = k < nil ? offcnt[k] : make_int2(0, 0);
@@ -466,12 +500,11 @@ __global__ __launch_bounds__(kSortBlock) void k_leaf_sort(const int2* __restrict
         const int off = __shfl(oc.x, j, 64), n = __shfl(oc.y, j, 64), b = __shfl(base, j, 64);
         for (int e = lane; e < n; e += 64) {
             s_idx[b + e] = cand[off + e];
-            s_t[b + e] = ct[off + e];
-            s_w[b + e] = cw[off + e];
+            s_key[b + e] = ((uint64_t)__float_as_uint(ct[off + e]) << 32) | (uint64_t)__float_as_uint(cw[off + e]);
         }
     }
     __syncthreads();
-    if (k < nil) fwsort(CandView{s_idx + base, s_t + base, s_w + base}, 0, oc.y);
+    if (k < nil) fwsort(CandKey64{s_idx + base, s_key + base}, 0, oc.y);
     __syncthreads();
     for (int j = 0; j < kSortBlock; j++) {
         const int off = __shfl(oc.x, j, 64), n = __shfl(oc.y, j, 64), b = __shfl(base, j, 64);
