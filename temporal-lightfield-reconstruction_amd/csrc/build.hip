@@ -72,10 +72,8 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
     int kmin_x = 0x7FFFFFFF, kmin_y = 0x7FFFFFFF;
     int kmax_x = (int)0x80000000, kmax_y = (int)0x80000000;
     unsigned nok = 0, nbad = 0;
-    const int lane = threadIdx.x & 63;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         bool ok = false;
-        int pix = -1;   // the pixel this sample's reprojection hits
         const float x = in.x[i], y = in.y[i], w = in.w[i], u = in.u[i], v = in.v[i], t = in.t[i];
         const float mvx = in.mvx[i], mvy = in.mvy[i], mvw = in.mvw[i];
         const float coc = C0 / w + C1;
@@ -104,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
                 packed[3 * (size_t)i + 1] = make_float4(mvx, mvy, mvw, in.r[i]);
                 packed[3 * (size_t)i + 2] = make_float4(in.g[i], in.b[i], 0.0f, 0.0f);
                 const int sx = floor_i(ox), sy = floor_i(oy);
-                if (sx >= 0 && sy >= 0 && sx < W && sy < H) pix = sy * W + sx;
+                if (sx >= 0 && sy >= 0 && sx < W && sy < H) atomicAdd(&hits[sy * W + sx], 1);
                 kmin_x = min(kmin_x, order_key(ox));
                 kmin_y = min(kmin_y, order_key(oy));
                 kmax_x = max(kmax_x, order_key(ox));
@@ -115,18 +113,6 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
             }
         }
         if (!ok) rw[i] = 0.0f;  // marks a discarded sample
-        // per-pixel hit counts: a wave's samples are a few neighbouring
-        // pixels' and often land in the same pixel, so up to four pixels
-        // get one atomic for all their lanes; the other lanes add their own
-        uint64_t pend = __ballot(pix >= 0);
-        for (int r = 0; r < 4 && pend; r++) {
-            const int leader = __ffsll((unsigned long long)pend) - 1;
-            const int key = __shfl(pix, leader, 64);
-            const uint64_t same = __ballot(pix == key);
-            if (lane == leader) atomicAdd(&hits[key], (int)__popcll(same));
-            pend &= ~same;
-        }
-        if ((pend >> lane) & 1ull) atomicAdd(&hits[pix], 1);
     }
     kmin_x = wave_min(kmin_x); kmin_y = wave_min(kmin_y);
     kmax_x = wave_max(kmax_x); kmax_y = wave_max(kmax_y);
@@ -583,55 +569,72 @@ __global__ __launch_bounds__(kBlock) void k_surface_scan(const int2* __restrict_
     bool reset = true;
     float cb00 = 0, cb01 = 0, cb10 = 0, cb11 = 0, mn0 = 0, mn1 = 0, mx0 = 0, mx1 = 0, st = 0;
     const float TH = 1 * 2.0f;
-    for (int k = 1; k < n; ++k) {
-        if (reset) {
-            const float4 a = B[2 * sb], c = B[2 * sb + 1];
-            cb00 = cb01 = 0.5f * (a.x - a.z);
-            cb10 = cb11 = 0.5f * (c.x - c.z);
-            mn0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
-            mn1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
-            mx0 = mn0; mx1 = mn1;
-            st = T[2 * sb].w;
-            reset = false;
+    // samples in groups of four, each group's records loaded together: the
+    // wave's threads stream 64 leaves at once, and a line read one sample at
+    // a time would leave the cache before its thread came back for the next
+    for (int k0 = 1; k0 < n; k0 += 4) {
+        float4 ga[4], gc[4];
+        float gt[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + q < n ? k0 + q : n - 1;
+            ga[q] = B[2 * k];
+            gc[q] = B[2 * k + 1];
+            gt[q] = T[2 * k].w;
         }
-        if (st != T[2 * k].w) {
-            sb = k;
-            start_flag[base + k] = 1u;
-            reset = true;
-            continue;
-        }
-        const float4 a = B[2 * k], c = B[2 * k + 1];
-        const float c0 = 0.5f * (a.x - a.z);
-        const float c1 = 0.5f * (c.x - c.z);
-        const float m0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
-        const float m1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
-        const float dmv = fmx(fmx(fabsf(mn0 - m0), fabsf(mn1 - m1)), fmx(fabsf(mx0 - m0), fabsf(mx1 - m1)));
-        const bool early = fmx(fabsf(c0 - cb00), fabsf(c0 - cb01)) < 0.5f * TH &&
-                           fmx(fabsf(c1 - cb10), fabsf(c1 - cb11)) < 0.5f * TH && dmv < TH;
-        bool conflict = false;
-        if (!early) {
-            for (int j = sb; j < k; ++j) {
-                const float4 ja = B[2 * j], jc = B[2 * j + 1];
-                const bool bX = tern_eq4(tern_cmp(ja.x, a.x, TH), tern_cmp(ja.z, a.z, TH), tern_cmp(jc.x, c.x, TH),
-                                         tern_cmp(jc.z, c.z, TH));
-                const bool bY = tern_eq4(tern_cmp(ja.y, a.y, TH), tern_cmp(ja.w, a.w, TH), tern_cmp(jc.y, c.y, TH),
-                                         tern_cmp(jc.w, c.w, TH));
-                if (!(bX && bY)) {
-                    sb = k;
-                    start_flag[base + k] = 1u;
-                    conflict = true;
-                    reset = true;
-                    break;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + q;
+            if (k >= n) break;
+            if (reset) {
+                const float4 a = B[2 * sb], c = B[2 * sb + 1];
+                cb00 = cb01 = 0.5f * (a.x - a.z);
+                cb10 = cb11 = 0.5f * (c.x - c.z);
+                mn0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
+                mn1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
+                mx0 = mn0; mx1 = mn1;
+                st = T[2 * sb].w;
+                reset = false;
+            }
+            if (st != gt[q]) {
+                sb = k;
+                start_flag[base + k] = 1u;
+                reset = true;
+                continue;
+            }
+            const float4 a = ga[q], c = gc[q];
+            const float c0 = 0.5f * (a.x - a.z);
+            const float c1 = 0.5f * (c.x - c.z);
+            const float m0 = 0.5f * (c.x + c.z) - 0.5f * (a.x + a.z);
+            const float m1 = 0.5f * (c.y + c.w) - 0.5f * (a.y + a.w);
+            const float dmv = fmx(fmx(fabsf(mn0 - m0), fabsf(mn1 - m1)), fmx(fabsf(mx0 - m0), fabsf(mx1 - m1)));
+            const bool early = fmx(fabsf(c0 - cb00), fabsf(c0 - cb01)) < 0.5f * TH &&
+                               fmx(fabsf(c1 - cb10), fabsf(c1 - cb11)) < 0.5f * TH && dmv < TH;
+            bool conflict = false;
+            if (!early) {
+                for (int j = sb; j < k; ++j) {
+                    const float4 ja = B[2 * j], jc = B[2 * j + 1];
+                    const bool bX = tern_eq4(tern_cmp(ja.x, a.x, TH), tern_cmp(ja.z, a.z, TH),
+                                             tern_cmp(jc.x, c.x, TH), tern_cmp(jc.z, c.z, TH));
+                    const bool bY = tern_eq4(tern_cmp(ja.y, a.y, TH), tern_cmp(ja.w, a.w, TH),
+                                             tern_cmp(jc.y, c.y, TH), tern_cmp(jc.w, c.w, TH));
+                    if (!(bX && bY)) {
+                        sb = k;
+                        start_flag[base + k] = 1u;
+                        conflict = true;
+                        reset = true;
+                        break;
+                    }
                 }
             }
-        }
-        if (!conflict) {
-            cb00 = fmn(c0, cb00);
-            cb01 = fmx(c0, cb00);
-            cb10 = fmn(c1, cb10);
-            cb11 = fmx(c1, cb10);
-            mn0 = fmn(mn0, m0); mn1 = fmn(mn1, m1);
-            mx0 = fmx(mx0, m0); mx1 = fmx(mx1, m1);
+            if (!conflict) {
+                cb00 = fmn(c0, cb00);
+                cb01 = fmx(c0, cb00);
+                cb10 = fmn(c1, cb10);
+                cb11 = fmx(c1, cb10);
+                mn0 = fmn(mn0, m0); mn1 = fmn(mn1, m1);
+                mx0 = fmx(mx0, m0); mx1 = fmx(mx1, m1);
+            }
         }
     }
 }
