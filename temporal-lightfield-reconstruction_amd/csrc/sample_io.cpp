@@ -3,6 +3,8 @@
 // (SampleBuffer.cpp:404-564), with their quirks: alpha is forced to 1 on
 // read; the writer stores b in the alpha slot; binary payloads need the
 // separate "<file>.header".
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,12 +15,99 @@
 
 namespace {
 
-// Parses one float the way a C locale strtod does and advances past it.
+bool take_literal(const char*& p, const char* lit) {
+    const char* t = p;
+    while (*lit && *t == *lit) {
+        t++;
+        lit++;
+    }
+    if (*lit) return false;
+    p = t;
+    return true;
+}
+
+float float_of_bits(uint32_t b) {
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+// FW::parseFloat (src/framework/base/String.cpp:428-487) step for step, in
+// float arithmetic with every operation rounded (no contraction): an optional
+// '+' or '-', integer digits accumulated as v = v * 10 + d, fraction digits
+// added as v += scale * d with scale *= 0.1f before each, the literals #INF,
+// #SNAN, #QNAN, #IND after the digits, then an optional exponent (e or E and
+// an integer) applied as v *= powf(10, e).  No whitespace is skipped.  The
+// result is not the correctly rounded value of the decimal string in general
+// (strtof's).  Without a digit the reference leaves the value unset and the
+// position unchanged; here the value is 0.
 const char* parse_float(const char* p, float& out) {
-    char* end = nullptr;
-    out = strtof(p, &end);
-    if (end == p) out = 0.0f;
-    return end;
+    const char* t = p;
+    bool neg = false;
+    if (*t == '+') {
+        t++;
+    } else if (*t == '-') {
+        neg = true;
+        t++;
+    }
+    float v = 0.0f;
+    int digits = 0;
+    while (*t >= '0' && *t <= '9') {
+        v = v * 10.0f + (float)(*t++ - '0');
+        digits++;
+    }
+    if (*t == '.') {
+        t++;
+        float scale = 1.0f;
+        while (*t >= '0' && *t <= '9') {
+            scale *= 0.1f;
+            v += scale * (float)(*t++ - '0');
+            digits++;
+        }
+    }
+    if (!digits) {
+        out = 0.0f;
+        return p;
+    }
+    p = t;
+    if (*p == '#') {
+        if (take_literal(p, "#INF")) {
+            out = float_of_bits(neg ? 0xFF800000u : 0x7F800000u);
+            return p;
+        }
+        if (take_literal(p, "#SNAN")) {
+            out = float_of_bits(neg ? 0xFF800001u : 0x7F800001u);
+            return p;
+        }
+        if (take_literal(p, "#QNAN")) {
+            out = float_of_bits(neg ? 0xFFC00001u : 0x7FC00001u);
+            return p;
+        }
+        if (take_literal(p, "#IND")) {
+            out = float_of_bits(neg ? 0xFFC00000u : 0x7FC00000u);
+            return p;
+        }
+    }
+    if (*t == 'e' || *t == 'E') {
+        // FW::parseInt (String.cpp:372-385): optional sign, at least one digit
+        const char* q = t + 1;
+        bool eneg = false;
+        if (*q == '+') {
+            q++;
+        } else if (*q == '-') {
+            eneg = true;
+            q++;
+        }
+        if (*q >= '0' && *q <= '9') {
+            int32_t e = 0;
+            while (*q >= '0' && *q <= '9') e = (int32_t)((uint32_t)e * 10u + (uint32_t)(*q++ - '0'));
+            if (eneg) e = -e;
+            p = q;
+            if (e) v *= powf(10.0f, (float)e);
+        }
+    }
+    out = neg ? -v : v;
+    return p;
 }
 
 }  // namespace

@@ -84,3 +84,60 @@ def test_product_pattern_matches_oracle(tlr, orc, M):
     a = tlr.output_pattern(M, override_uvt=(0.1, 0.2, 0.3))
     b = orc.output_pattern(M, override_uvt=(0.1, 0.2, 0.3))
     assert np.array_equal(a, b[:, :5])
+
+
+def _fw_parse_float32(s):
+    """FW::parseFloat (src/framework/base/String.cpp:428-487) on a plain
+    number, each step an IEEE single-precision operation (numpy float32)."""
+    f = np.float32
+    t, neg = 0, False
+    if s[:1] == "+":
+        t = 1
+    elif s[:1] == "-":
+        t, neg = 1, True
+    v = f(0)
+    while t < len(s) and s[t].isdigit():
+        v = f(f(v * f(10)) + f(int(s[t])))
+        t += 1
+    if t < len(s) and s[t] == ".":
+        t += 1
+        sc = f(1)
+        while t < len(s) and s[t].isdigit():
+            sc = f(sc * f(0.1))
+            v = f(v + f(sc * f(int(s[t]))))
+            t += 1
+    if t < len(s) and s[t] in "eE":
+        e = int(s[t + 1:])
+        if e:
+            v = f(v * np.power(f(10), f(e)))
+    return f(-v) if neg else v
+
+
+def test_text_records_parse_as_fw_parsefloat(tlr, tmp_path):
+    """Text records go through the reference's float parser, not a correctly
+    rounded one: digits accumulate in float, fraction digits as scale * d with
+    scale *= 0.1f, exponents as powf(10, e), and the #INF / #IND / #QNAN /
+    #SNAN literals give fixed bit patterns.  Three of the fields below round
+    differently from the decimal value (strtof), so a correctly rounded parser
+    fails this test."""
+    sb = tlr.make_scene(1, 1, 1, 1, seed=3)
+    path = tmp_path / "one.txt"
+    sb.save(str(path), binary=False)
+    lines = path.read_text().splitlines()
+    fields = ["0.123456", "123.456789", "-7.654321", "1.5e2", "2.5E-3", "16777217", "1#INF", "-1#IND",
+              "3#QNAN", "0.1", "9", "+42", "-0.000001", "3.14159", "2#SNAN", "1e1"]
+    lines[-1] = ",".join(fields)
+    path.write_text("\n".join(lines) + "\n")
+    got = tlr.UVTSampleBuffer.load(str(path)).records[0].view(np.uint32)
+    special = {6: 0x7F800000, 7: 0xFFC00000, 8: 0x7FC00001, 14: 0x7F800001}
+    differ = 0
+    for k, s in enumerate(fields):
+        if k == 10:
+            assert got[k] == np.float32(1).view(np.uint32)   # alpha forced to 1
+        elif k in special:
+            assert got[k] == special[k], (s, hex(got[k]))
+        else:
+            want = _fw_parse_float32(s).view(np.uint32)
+            assert got[k] == want, (s, hex(got[k]), hex(want))
+            differ += int(want != np.float32(float(s)).view(np.uint32))
+    assert differ == 3
