@@ -1582,8 +1582,9 @@ __global__ __launch_bounds__(256) void k_gate_accept(const float* __restrict__ l
         const unsigned e = e0 + threadIdx.x;
         bool acc = false;
         if (e < n) {
-            const float z = logits[e], f0 = features[(size_t)e * kGateFeatures];
-            acc = z >= thr && f0 >= 0.0f;
+            // the feature row (one cache line per entry) only where the logit passes
+            const float z = logits[e];
+            if (z >= thr) acc = features[(size_t)e * kGateFeatures] >= 0.0f;
         }
         if (acc) entries[e] = kSkipEntry;
         const uint64_t m = __ballot(acc);
