@@ -1578,6 +1578,7 @@ __global__ __launch_bounds__(256) void k_gate_accept(const float* __restrict__ l
                                                      const unsigned* __restrict__ n_entries, float thr,
                                                      unsigned* __restrict__ accepted) {
     const unsigned n = *n_entries;
+    unsigned mine = 0;
     for (unsigned e0 = blockIdx.x * 256; e0 < n; e0 += gridDim.x * 256) {
         const unsigned e = e0 + threadIdx.x;
         bool acc = false;
@@ -1586,9 +1587,21 @@ __global__ __launch_bounds__(256) void k_gate_accept(const float* __restrict__ l
             const float z = logits[e];
             if (z >= thr) acc = features[(size_t)e * kGateFeatures] >= 0.0f;
         }
-        if (acc) entries[e] = kSkipEntry;
-        const uint64_t m = __ballot(acc);
-        if (m && (threadIdx.x & 63) == 0) atomicAdd(accepted, (unsigned)__popcll(m));
+        if (acc) {
+            entries[e] = kSkipEntry;
+            mine++;
+        }
+    }
+    // one atomic per block on the shared count (atomics on one address serialise)
+    __shared__ unsigned wsum[4];
+    unsigned v = mine;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(accepted, t);
     }
 }
 
@@ -1680,14 +1693,36 @@ __global__ __launch_bounds__(kBlock) void k_query_gather(QueryParams Q, const fl
 
 // Pixel = sum of its output samples in order, normalised by alpha; red if
 // nothing was found (Reconstruction.cpp:665-719).
+// Per pixel: the sum of its M output samples in sample order, normalised by
+// the weight (red where it is 0).  When M divides 64, one thread per output
+// sample: a wave loads 64 consecutive output samples (64 / M pixels) in one
+// coalesced access and every lane sums its pixel's samples, in order, from
+// the lanes that hold them; otherwise one thread per pixel.
 __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ qcol, int W, int H, int M,
                                                     float* __restrict__ out) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= W * H) return;
+    const int npix = W * H;
     float r = 0, g = 0, b = 0, a = 0;
-    for (int i = 0; i < M; i++) {
-        const float4 c = qcol[(size_t)p * M + i];
-        r += c.x; g += c.y; b += c.z; a += c.w;
+    int p;
+    if (M <= 64 && (64 % M) == 0) {
+        const size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x;
+        const bool in = k < (size_t)npix * (size_t)M;
+        const float4 c = in ? qcol[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int lane = threadIdx.x & 63, first = lane & ~(M - 1);
+        for (int i = 0; i < M; i++) {
+            r += __shfl(c.x, first + i, 64);
+            g += __shfl(c.y, first + i, 64);
+            b += __shfl(c.z, first + i, 64);
+            a += __shfl(c.w, first + i, 64);
+        }
+        if (!in || lane != first) return;
+        p = (int)(k / (size_t)M);
+    } else {
+        p = blockIdx.x * kBlock + threadIdx.x;
+        if (p >= npix) return;
+        for (int i = 0; i < M; i++) {
+            const float4 c = qcol[(size_t)p * M + i];
+            r += c.x; g += c.y; b += c.z; a += c.w;
+        }
     }
     if (a == 0) {
         r = 1; g = 0; b = 0; a = 1;
@@ -1875,7 +1910,8 @@ hipError_t launch_query_gather(const QueryParams& p, const float* o5, int n, flo
 hipError_t launch_combine(const float4* qcol, int W, int H, int M, float* out, hipStream_t s) {
     const int n = W * H;
     if (n == 0) return hipSuccess;
-    k_combine<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(qcol, W, H, M, out);
+    const size_t threads = (M <= 64 && (64 % M) == 0) ? (size_t)n * M : (size_t)n;
+    k_combine<<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(qcol, W, H, M, out);
     return hipGetLastError();
 }
 
