@@ -757,6 +757,8 @@ __device__ __forceinline__ void flush_stats(unsigned long long* dst, const unsig
 // 7 waves per SIMD (the register limit this sets spills a little; the
 // kernel waits on memory and gains more from resident waves).  Persistent:
 // waves take queries from an atomic counter until the list is done.
+constexpr unsigned kGeneralChunk = 4;
+
 template <bool DEBUG, int MODE, bool STATS>
 __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
@@ -768,19 +770,25 @@ __global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
     for (int k = 0; k < kNumStats; k++) st[k] = 0;
     const unsigned nq = Q.qlist ? *Q.qcount : Q.num_queries;
     for (;;) {
-        unsigned qi = 0;
-        if (lane == 0) qi = atomicAdd(Q.next_query, 1u);
-        qi = (unsigned)uni((int)qi);
-        if (qi >= nq) break;
-        bool skip1r = false;
-        if (Q.qlist) {
-            qi = (unsigned)uni((int)Q.qlist[qi]);
-            if (qi == kSkipEntry) continue;  // accepted by the learned gate
-            // top bit: the fast kernel already found that 2R is needed
-            skip1r = (qi & kNeeds2R) != 0;
-            qi &= ~kNeeds2R;
+        // kGeneralChunk consecutive queries per claim (atomics on the one
+        // counter serialise)
+        unsigned q0 = 0;
+        if (lane == 0) q0 = atomicAdd(Q.next_query, kGeneralChunk);
+        q0 = (unsigned)uni((int)q0);
+        if (q0 >= nq) break;
+        const unsigned q1 = nq - q0 < kGeneralChunk ? nq : q0 + kGeneralChunk;
+        for (unsigned e = q0; e < q1; e++) {
+            unsigned qi = e;
+            bool skip1r = false;
+            if (Q.qlist) {
+                qi = (unsigned)uni((int)Q.qlist[e]);
+                if (qi == kSkipEntry) continue;  // accepted by the learned gate
+                // top bit: the fast kernel already found that 2R is needed
+                skip1r = (qi & kNeeds2R) != 0;
+                qi &= ~kNeeds2R;
+            }
+            general_query<DEBUG, MODE, STATS>(Q, m, qi, skip1r, st);
         }
-        general_query<DEBUG, MODE, STATS>(Q, m, qi, skip1r, st);
     }
     flush_stats(Q.stats, st, STATS && Q.collect_stats != 0);
 }

@@ -800,12 +800,19 @@ __global__ __launch_bounds__(kRefStageBlock) void k_ref_fail_wave(RefLaunch R) {
     const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * kRefStageBlock + threadIdx.x) >> 6));
     const RefWaveMem m = ref_wave_mem(R.wscratch + wave * kRefWaveScratchWords);
     const unsigned total = *R.count;
+    // four consecutive entries per claim (atomics on the one counter serialise)
+    constexpr unsigned kChunk = 4;
+    unsigned e = 0, e_end = 0;
     for (;;) {
-        unsigned e = 0;
-        if (lane == 0) e = atomicAdd(R.next, 1u);
-        e = (unsigned)wuni((int)e);
-        if (e >= total) break;
+        if (e == e_end) {
+            unsigned e0 = 0;
+            if (lane == 0) e0 = atomicAdd(R.next, kChunk);
+            e = (unsigned)wuni((int)e0);
+            if (e >= total) break;
+            e_end = total - e < kChunk ? total : e + kChunk;
+        }
         const unsigned q = (unsigned)wuni((int)R.list[e]);
+        e++;
         const OSample o = make_osample(R.q, q);
         float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
         const int r = ref_fail_wave(R.q, m, o, R.leaf_cap, R.point_cap, col);
