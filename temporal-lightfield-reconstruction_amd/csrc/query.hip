@@ -754,13 +754,14 @@ __device__ __forceinline__ void flush_stats(unsigned long long* dst, const unsig
     }
 }
 
-// 7 waves per SIMD (the register limit this sets spills a little; the
-// kernel waits on memory and gains more from resident waves).  Persistent:
-// waves take queries from an atomic counter until the list is done.
+// 8 waves per SIMD (the register limit this sets spills; the kernel waits
+// on memory and gains more from resident waves: 7 and 6 were 6 % and 15 %
+// slower).  Persistent: waves take queries from an atomic counter until the
+// list is done.
 constexpr unsigned kGeneralChunk = 4;
 
 template <bool DEBUG, int MODE, bool STATS>
-__global__ __launch_bounds__(kBlock, 7) void k_query_general(QueryParams Q) {
+__global__ __launch_bounds__(kBlock, 8) void k_query_general(QueryParams Q) {
     const int lane = threadIdx.x & 63;
     // wave-uniform (so the scratch pointers derived from it live in scalar registers)
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
