@@ -80,60 +80,109 @@ __global__ __launch_bounds__(kRayBlock) void k_shadow_rays(ShadowRays R) {
         }
         R.flag[q] = f;
     }
-    // append the light queries, one atomic per wave
+    // append the light queries, one atomic per block (atomics on the one
+    // counter serialise): the waves' counts in LDS, the block's base from one
+    // atomic, each wave's base past the earlier waves' entries
+    __shared__ unsigned wcount[kRayBlock / 64];
+    __shared__ unsigned bbase;
     const uint64_t m = __ballot(query);
-    if (m) {
-        const int lane = threadIdx.x & 63;
-        unsigned base = 0;
-        if (lane == 0) base = atomicAdd(R.count, (unsigned)__popcll(m));
-        base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-        if (query) R.list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = q;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wcount[w] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int k = 0; k < kRayBlock / 64; k++) t += wcount[k];
+        bbase = t ? atomicAdd(R.count, t) : 0u;
+    }
+    __syncthreads();
+    if (query) {
+        unsigned base = bbase;
+        for (int k = 0; k < w; k++) base += wcount[k];
+        R.list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = q;
     }
 }
 
 constexpr int kShadeBlock = 128;
 
-// One pixel per lane, its output samples in order: the light pass's colour
-// scaled by the Lambert term (light-space dot of the normal from the camera
-// result's w-gradients), plus the ambient term from the world-space normal;
-// a sample with no surface makes the pixel red (Reconstruction.cpp:762-766,
-// 795-833).
-__global__ __launch_bounds__(kShadeBlock) void k_shadow_shade(ShadowShade D) {
-    const int p = blockIdx.x * kShadeBlock + threadIdx.x;
-    if (p >= D.W * D.H) return;
+// One output sample's shaded colour: the light pass's colour scaled by the
+// Lambert term (light-space dot of the normal from the camera result's
+// w-gradients), plus the ambient term from the world-space normal
+// (Reconstruction.cpp:795-833); f: its flag (2: no surface).
+__device__ __forceinline__ void shade_sample(const ShadowShade& D, size_t q, int& f, float col[4]) {
     const float LIGHTING_SCALE = 1.5f, AMBIENT_SCALE = 0.5f;  // Reconstruction.cpp:34-35
-    float pc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int i = 0; i < D.M; i++) {
-        const size_t q = (size_t)p * D.M + i;
-        const int f = D.flag[q];
-        if (f == 2) {
-            pc[0] = 1.f; pc[1] = 0.f; pc[2] = 0.f; pc[3] = 1.f;
-            break;
-        }
-        const float4 c = D.qcol[q];
-        float col[4] = {c.x, c.y, c.z, c.w};
-        if (f == 1) {
-            const float4 sc = D.scol[q];
-            col[0] = sc.x; col[1] = sc.y; col[2] = sc.z; col[3] = sc.w;
-            const float4 e1 = D.qext[2 * q + 1];
-            const float4 v = D.xyw[q];
-            const float v4[4] = {v.x, v.y, v.z, v.w};
-            float n[3];
-            const float ldotn = light_space_dot(v4, e1.x, e1.y, D.S, n);
-            const float n4[4] = {n[0], n[1], n[2], 0.0f};
-            float wn[4];
-            mat4_vec(D.S.world_inv_t, n4, wn);
-            const float ambient = wn[1] * 0.5f + 0.5f;
-            const float cosine = ldotn > 0.0f ? ldotn : 0.0f;
-            const float s3 = LIGHTING_SCALE * cosine, s4 = LIGHTING_SCALE * 1.0f;
-            col[0] *= s3; col[1] *= s3; col[2] *= s3; col[3] *= s4;
-            const float a = ambient * AMBIENT_SCALE;
-            col[0] += a * c.x; col[1] += a * c.y; col[2] += a * c.z; col[3] += a * c.w;
-            col[3] = 1.0f;
-        }
-        for (int k = 0; k < 4; k++) pc[k] += col[k];
+    f = D.flag[q];
+    col[0] = col[1] = col[2] = col[3] = 0.0f;
+    if (f == 2) return;
+    const float4 c = D.qcol[q];
+    col[0] = c.x; col[1] = c.y; col[2] = c.z; col[3] = c.w;
+    if (f == 1) {
+        const float4 sc = D.scol[q];
+        col[0] = sc.x; col[1] = sc.y; col[2] = sc.z; col[3] = sc.w;
+        const float4 e1 = D.qext[2 * q + 1];
+        const float4 v = D.xyw[q];
+        const float v4[4] = {v.x, v.y, v.z, v.w};
+        float n[3];
+        const float ldotn = light_space_dot(v4, e1.x, e1.y, D.S, n);
+        const float n4[4] = {n[0], n[1], n[2], 0.0f};
+        float wn[4];
+        mat4_vec(D.S.world_inv_t, n4, wn);
+        const float ambient = wn[1] * 0.5f + 0.5f;
+        const float cosine = ldotn > 0.0f ? ldotn : 0.0f;
+        const float s3 = LIGHTING_SCALE * cosine, s4 = LIGHTING_SCALE * 1.0f;
+        col[0] *= s3; col[1] *= s3; col[2] *= s3; col[3] *= s4;
+        const float a = ambient * AMBIENT_SCALE;
+        col[0] += a * c.x; col[1] += a * c.y; col[2] += a * c.z; col[3] += a * c.w;
+        col[3] = 1.0f;
     }
-    if (pc[3] == 0) {
+}
+
+// Per pixel, its output samples' shaded colours summed in order; a sample
+// with no surface makes the pixel red (Reconstruction.cpp:762-766).  When M
+// divides 64, one thread per output sample: a wave shades 64 consecutive
+// output samples (64 / M pixels) with coalesced loads and every lane sums its
+// pixel's samples, in order, from the lanes that hold them; otherwise one
+// thread per pixel.
+__global__ __launch_bounds__(kShadeBlock) void k_shadow_shade(ShadowShade D) {
+    const int npix = D.W * D.H, M = D.M;
+    float pc[4] = {0.f, 0.f, 0.f, 0.f};
+    bool red = false;
+    int p;
+    if (M <= 64 && (64 % M) == 0) {
+        const size_t k = (size_t)blockIdx.x * kShadeBlock + threadIdx.x;
+        const bool in = k < (size_t)npix * (size_t)M;
+        int f = 0;
+        float col[4] = {0.f, 0.f, 0.f, 0.f};
+        if (in) shade_sample(D, k, f, col);
+        const int lane = threadIdx.x & 63, first = lane & ~(M - 1);
+        for (int i = 0; i < M; i++) {
+            const int fi = __shfl(f, first + i, 64);
+            float ci[4];
+            for (int c = 0; c < 4; c++) ci[c] = __shfl(col[c], first + i, 64);
+            if (!red) {
+                if (fi == 2) {
+                    red = true;
+                } else {
+                    for (int c = 0; c < 4; c++) pc[c] += ci[c];
+                }
+            }
+        }
+        if (!in || lane != first) return;
+        p = (int)(k / (size_t)M);
+    } else {
+        p = blockIdx.x * kShadeBlock + threadIdx.x;
+        if (p >= npix) return;
+        for (int i = 0; i < M && !red; i++) {
+            int f;
+            float col[4];
+            shade_sample(D, (size_t)p * M + i, f, col);
+            if (f == 2) {
+                red = true;
+            } else {
+                for (int c = 0; c < 4; c++) pc[c] += col[c];
+            }
+        }
+    }
+    if (red || pc[3] == 0) {
         pc[0] = 1.f; pc[1] = 0.f; pc[2] = 0.f; pc[3] = 1.f;
     }
     const float rw = rcp(pc[3]);
@@ -158,7 +207,8 @@ hipError_t launch_shadow_rays(const ShadowRays& r, hipStream_t s) {
 hipError_t launch_shadow_shade(const ShadowShade& sh, hipStream_t s) {
     const int n = sh.W * sh.H;
     if (n == 0) return hipSuccess;
-    k_shadow_shade<<<(n + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(sh);
+    const size_t threads = (sh.M <= 64 && (64 % sh.M) == 0) ? (size_t)n * sh.M : (size_t)n;
+    k_shadow_shade<<<(unsigned)((threads + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, s>>>(sh);
     return hipGetLastError();
 }
 
