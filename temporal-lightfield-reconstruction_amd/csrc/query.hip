@@ -117,9 +117,11 @@ static_assert(kStage >= 1 && kLdsStack >= 64, "the depth-6 start puts up to 64 e
 constexpr int kTriCap = 200;
 constexpr int kGatherLdsWords = 5 * kStage + kLdsStack;
 constexpr int kTriLdsWords = 3 * kTriCap + kTriCap / 2;   // tx, ty, tr (words), tl (shorts)
-constexpr int kWaveLdsWords = kGatherLdsWords > kTriLdsWords ? kGatherLdsWords : kTriLdsWords;
+// (a multiple of 4 words: the leaf keys and ids are read 16 bytes at a time)
+constexpr int kWaveLdsWords = ((kGatherLdsWords > kTriLdsWords ? kGatherLdsWords : kTriLdsWords) + 3) & ~3;
+static_assert(kStage % 4 == 0, "leaf ids start 16-byte aligned after the keys");
 __device__ __forceinline__ float* wave_lds() {
-    __shared__ float area[kWavesPerBlock][kWaveLdsWords];
+    __shared__ __align__(16) float area[kWavesPerBlock][kWaveLdsWords];
     return area[threadIdx.x >> 6];
 }
 
@@ -242,15 +244,27 @@ __device__ Gather gather(const QueryParams& Q, const WaveMem& m, const OSample& 
     if (g.overflow) return g;
     if (STATS) st[ST_LEAF_NODES] += (unsigned long long)nl;
     // --- front-to-back order: rank by (key, leaf id) ---
+    const int nls = nl < kStage ? nl : kStage;
     for (int j = lane; j < nl; j += 64) {
         const float kj = j < kStage ? Lkey[j] : m.lkey()[j];
         const int ij = j < kStage ? Lid[j] : m.lid()[j];
         const LeafRec& lr = Q.leaves[ij];
         const int s0 = lr.s0, s1 = lr.s1;
         int rank = 0;
-        for (int i = 0; i < nl; i++) {
-            const float ki = i < kStage ? Lkey[i] : m.lkey()[i];
-            const int ii = i < kStage ? Lid[i] : m.lid()[i];
+        // the staged entries four at a time (16-byte LDS reads), then the rest
+        int i = 0;
+        for (; i + 4 <= nls; i += 4) {
+            const float4 k4 = *reinterpret_cast<const float4*>(Lkey + i);
+            const int4 i4 = *reinterpret_cast<const int4*>(Lid + i);
+            rank += (k4.x < kj || (!(kj < k4.x) && i4.x < ij)) ? 1 : 0;
+            rank += (k4.y < kj || (!(kj < k4.y) && i4.y < ij)) ? 1 : 0;
+            rank += (k4.z < kj || (!(kj < k4.z) && i4.z < ij)) ? 1 : 0;
+            rank += (k4.w < kj || (!(kj < k4.w) && i4.w < ij)) ? 1 : 0;
+        }
+        for (; i < nls; i++) rank += (Lkey[i] < kj || (!(kj < Lkey[i]) && Lid[i] < ij)) ? 1 : 0;
+        for (i = kStage; i < nl; i++) {
+            const float ki = m.lkey()[i];
+            const int ii = m.lid()[i];
             rank += (ki < kj || (!(kj < ki) && ii < ij)) ? 1 : 0;
         }
         if (rank < kStage) {
