@@ -142,11 +142,19 @@ __global__ __launch_bounds__(kBlock) void k_reproject(InSoA in, int n, int W, in
     }
 }
 
+// Largest per-pixel hit count: grid-stride, reduced per block, one atomic per
+// block (atomics on one address serialise).
 __global__ __launch_bounds__(kBlock) void k_hits_max(const int* __restrict__ hits, int n, GridInfo* gi) {
     int m = 0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) m = max(m, hits[i]);
     m = wave_max(m);
-    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&gi->max_hits, m);
+    __shared__ int red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; k++) m = max(m, red[k]);
+        if (m > 0) atomicMax(&gi->max_hits, m);
+    }
 }
 
 // Grid sizing: Reconstruction.cpp:587-603
@@ -1174,7 +1182,7 @@ void launch_reproject(const InSoA& in, int n, int W, int H, float C0, float C1, 
     const unsigned rb = (unsigned)std::min<long long>(kReduceBlocks, ((long long)n + kBlock - 1) / kBlock);
     if (n > 0) k_reproject<<<rb, kBlock, 0, s>>>(in, n, W, H, C0, C1, rx, ry, rw, rt, packed, hits, gi);
     const int nh = W * H;
-    unsigned blocks = (unsigned)std::min(4096, (nh + kBlock - 1) / kBlock);
+    unsigned blocks = (unsigned)std::min(1024, (nh + kBlock - 1) / kBlock);
     if (blocks == 0) blocks = 1;
     k_hits_max<<<blocks, kBlock, 0, s>>>(hits, nh, gi);
     k_grid_setup<<<1, 64, 0, s>>>(gi);
